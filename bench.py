@@ -1,0 +1,162 @@
+"""Benchmark of the particle step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one frame (collide + integrate + stable re-bin) over the whole
+world, inputs resident in HBM.  Default workload: 1,000,000 particles on a
+jittered lattice filling the lower half of a 2048x1024 world, cell size 1,
+gravity (0, -0.1).  Prints one JSON line.
+
+roofline: the dominant kernel's algorithmic bytes per launch (per particle
+and per cell figures below) over its average duration, measured with HIP
+events recorded on the library's own stream around each launch.
+cpu_baseline: the oracle's multi-threaded frame on the host, on a bounded
+number of frames of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+METRIC = "particle-steps/s & %HBM roofline @1M"
+
+
+def algorithmic_bytes(n: int, cells: int) -> dict:
+    """Bytes each kernel must move per launch.  Per particle: collide reads
+    position+velocity (16) and writes them (16) plus the cell key (4); the
+    scatter reads key (4) + position+velocity (16) and writes them (16).
+    Per cell (C + 2 offsets): collide reads one offset (4); the scan reads a
+    count (4), writes an offset (4) and clears the count (4)."""
+    c2 = cells + 2
+    return {"collide": 36 * n + 4 * c2, "scan": 12 * c2, "scatter": 36 * n}
+
+
+def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
+    from oracle import oracle as o
+    threads = min(16, os.cpu_count() or 1)
+    w, h = wl.config.dimensions
+    idx, p, v = o.pack(wl.pos, wl.vel, (0, 0, w, h), wl.config.cell_size)
+    P = o.params(w, h, wl.config.cell_size, len(p), gravity=wl.config.gravity, dt=wl.config.dt)
+    t0 = time.perf_counter()
+    idx, p, v = o.step(P, idx, p, v, nsteps=1, nthreads=threads)
+    one = time.perf_counter() - t0
+    frames = int(max(3, min(200, budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(P, idx, p, v, nsteps=frames, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": len(p) * frames / dt, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{frames} frames of {wl.name} ({len(p)} particles) after 1 warm-up frame, "
+                      f"oracle wo_step_mt on {threads} host threads, {dt:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="lattice-1m-2048x1024")
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the world would shard by row strips with a migrant exchange; not built).",
+              file=sys.stderr)
+        return 2
+
+    from wrach_amd import World, workloads
+
+    wl = workloads.BY_NAME[args.workload]()
+    n, cells = wl.n, wl.cells
+    print(f"[bench] {wl.name}: N={n} cells={cells}", file=sys.stderr, flush=True)
+    world = World(wl.config)
+    world.add_particles(wl.pos, wl.vel)
+    world.set_graph(not args.no_graph)
+    world.step(args.warmup)
+    world.sync()
+
+    t0 = time.perf_counter()
+    world.step(args.steps)
+    world.sync()
+    elapsed = time.perf_counter() - t0
+    value = n * args.steps / elapsed
+    print(f"[bench] timed {args.steps} frames in {elapsed * 1e3:.2f} ms", file=sys.stderr, flush=True)
+
+    # Per-kernel durations: HIP events around each launch, plain launches.
+    ms = world.step_profiled(args.profile_steps)
+    avg_us = {k: 1e3 * v / args.profile_steps for k, v in ms.items()}
+    bytes_ = algorithmic_bytes(n, cells)
+    dom = max(avg_us, key=avg_us.get)
+    achieved = bytes_[dom] / (avg_us[dom] * 1e-6) / 1e9
+    step_bytes = sum(bytes_.values())
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{wl.name}.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath)).get(dom)
+        traffic = t.get("hbm_bytes_per_launch") if isinstance(t, dict) else None
+
+    world.close()
+    print(f"[bench] kernel avg us: {json.dumps({k: round(v, 3) for k, v in avg_us.items()})}",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl)
+        print(f"[bench] cpu baseline {cpu['value']:.3e} p-steps/s", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name,
+            "particles": n,
+            "world": list(wl.config.dimensions),
+            "cell_size": wl.config.cell_size,
+            "cells": cells,
+            "gravity": list(wl.config.gravity),
+            "dt": wl.config.dt,
+            "graph": not args.no_graph,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_[dom],
+            "kernel_avg_us": avg_us,
+            "step_algorithmic_bytes": step_bytes,
+            "step_achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
+            "step_frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,159 @@
+/*
+ * wrach.h -- C ABI of the MI355X (gfx950) particle step.
+ *
+ * Drop-in boundary for wrach's `runners/api` surface: the Rust host's
+ * `WrachAPI` (runners/api/src/lib.rs:17-88) and the wgpu compute worker it
+ * drives (runners/bevy/src/compute/builder.rs:24-92, plugin/build.rs:88-158)
+ * are replaced by these entry points.  Plain pointers and sizes only; the
+ * library copies host arrays in and out and retains no caller pointer.
+ *
+ * Errors: every int-returning call returns WRACH_OK (0) or a negative
+ * wrach_status; wrach_last_error() then describes the failure.  Nothing
+ * unwinds across the ABI.  Where the reference panics (an assert or an
+ * `expect`), the matching call returns an error code instead.
+ *
+ * Threading: a context is single-owner, like `&mut self` on WrachAPI
+ * (runners/api/src/lib.rs:49,78).  Calls on one context must be serialised
+ * by the caller; different contexts are independent.  wrach_step is
+ * asynchronous on the context's HIP stream; wrach_read and wrach_sync
+ * synchronise it.
+ */
+#ifndef WRACH_H
+#define WRACH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WRACH_ABI_VERSION 1
+
+typedef struct wrach_ctx wrach_ctx;
+
+typedef enum wrach_status {
+    WRACH_OK = 0,
+    WRACH_E_INVALID = -1,        /* null pointer, zero cell size, bad flag */
+    WRACH_E_TOO_MANY_CELLS = -2, /* grid too large (builder.rs:39-42 asserts) */
+    WRACH_E_CAPACITY = -3,       /* more particles than the buffers hold */
+    WRACH_E_NONFINITE = -4,      /* NaN position or velocity in an upload */
+    WRACH_E_DEVICE = -5,         /* HIP runtime error */
+    WRACH_E_BUFFER = -6,         /* caller's output array too small */
+    WRACH_E_STATE = -7           /* device state failed a consistency check */
+} wrach_status;
+
+/* WrachConfig (runners/bevy/src/config_app.rs:10-35) plus extensions.
+ * gravity = (0,0) and dt = 1 give the reference's step bit for bit. */
+typedef struct wrach_config {
+    uint16_t width;                   /* dimensions.0 */
+    uint16_t height;                  /* dimensions.1 */
+    uint16_t cell_size;               /* spatial-bin cell side, >= 1 */
+    uint8_t boundaries_as_dimensions; /* accepted and unused, as in the reference */
+    uint8_t reserved0;
+    float gravity[2];                 /* extension: v += g*dt before integrate */
+    float dt;                         /* extension: p += v*dt */
+    uint32_t flags;                   /* must be 0 */
+    int32_t device;                   /* HIP device ordinal */
+} wrach_config;
+
+/* ShaderWorldSettings, the 32-byte uniform (runners/bevy/src/
+ * config_shader.rs:15-29, shaders/shared/src/lib.rs:19-31). */
+typedef struct wrach_world_settings {
+    float view_dimensions[2];
+    float view_anchor[2];
+    uint32_t grid_dimensions[2];
+    uint32_t cell_size;
+    uint32_t particles_in_frame_count;
+} wrach_world_settings;
+
+/* Device-resident state, for zero-copy consumers (a renderer, a caller that
+ * keeps its own stream).  Inside each cell the particles are a permutation
+ * of the canonical order until wrach_read (or wrach_canonical_device) runs. */
+typedef struct wrach_device_view {
+    const uint32_t *indices;   /* grid cells + 2 entries */
+    const float *positions;    /* 2 floats per particle */
+    const float *velocities;   /* 2 floats per particle */
+    uint32_t n_particles;
+    uint32_t n_indices;
+    void *stream;              /* hipStream_t the frames run on */
+} wrach_device_view;
+
+/* Number of kernels timed by wrach_step_profiled, in launch order:
+ * 0 collide/integrate/key, 1 exclusive scan, 2 stable scatter. */
+#define WRACH_NUM_FRAME_KERNELS 3
+
+/* WrachConfig::default (config_app.rs:26-35): 480x352, cell size 3
+ * (SPATIAL_BIN_CELL_SIZE, shaders/shared/src/lib.rs:34). */
+void wrach_config_default(wrach_config *cfg);
+
+/* WrachAPI::new (runners/api/src/lib.rs:30-45) + PhysicsComputeWorker::build
+ * (runners/bevy/src/compute/builder.rs:24-92): grid from the viewport
+ * (0,0,W,H), buffer capacity max_particles_per_frame
+ * (runners/bevy/src/particle_store.rs:116-133), zeroed device buffers. */
+int wrach_create(const wrach_config *cfg, wrach_ctx **out);
+
+/* Drop of WrachAPI: frees device buffers, stream and graphs.  NULL is a no-op. */
+void wrach_destroy(wrach_ctx *ctx);
+
+/* WrachAPI::add_particles (runners/api/src/lib.rs:78-81) ->
+ * WrachState::add_particles (runners/bevy/src/state.rs:90-101): appends n
+ * particles (pos_xy, vel_xy: 2n floats each) to the host store, re-packs the
+ * whole store (particle_store.rs:54-103, spatial_bin.rs:103-149; particles
+ * outside the grid stay in the store but are not simulated) and uploads it,
+ * replacing the device state.  On error the store is left unchanged. */
+int wrach_add_particles(wrach_ctx *ctx, const float *pos_xy,
+                        const float *vel_xy, uint32_t n);
+
+/* The compute part of WrachAPI::tick / app.update() (runners/api/src/
+ * lib.rs:49-52; passes chained at builder.rs:86-89): nframes frames of
+ * collide + integrate + stable re-bin.  Asynchronous. */
+int wrach_step(wrach_ctx *ctx, uint32_t nframes);
+
+/* The readback of the `tick` system (plugin/build.rs:135-158) and
+ * WrachAPI::read_data (runners/api/src/lib.rs:58-74): synchronises, then
+ * copies the packed triple in canonical order.  Any output pointer may be
+ * NULL.  pos_xy/vel_xy receive *n_particles particles (particle_cap is in
+ * particles); indices receives the grid cells + 2 offsets
+ * [0, start_0, ..., start_{C-1}, N].  Returns WRACH_E_BUFFER if a non-NULL
+ * array is too small. */
+int wrach_read(wrach_ctx *ctx, float *pos_xy, float *vel_xy,
+               size_t particle_cap, uint32_t *indices, size_t indices_cap,
+               uint32_t *n_particles);
+
+/* Buffer sizes of PhysicsComputeWorker::build (builder.rs:26-55):
+ * max_particles = max_particles_per_frame(), n_indices = cells + 2. */
+int wrach_capacity(wrach_ctx *ctx, uint32_t *max_particles,
+                   uint32_t *n_indices);
+
+/* Current ShaderWorldSettings (WrachState.shader_settings, state.rs:22). */
+int wrach_settings(wrach_ctx *ctx, wrach_world_settings *out);
+
+/* Blocks until all submitted frames have finished. */
+int wrach_sync(wrach_ctx *ctx);
+
+/* Device pointers of the live state (no copy, no synchronisation). */
+int wrach_device_state(wrach_ctx *ctx, wrach_device_view *out);
+
+/* Writes the canonical (in-cell source order) copy of the state into
+ * device staging buffers and returns them in *out; asynchronous.  The
+ * pointers stay valid until the next wrach_step / wrach_read. */
+int wrach_canonical_device(wrach_ctx *ctx, wrach_device_view *out);
+
+/* 1 (default): frames are replayed from a captured hipGraph; 0: plain
+ * stream launches. */
+int wrach_set_graph(wrach_ctx *ctx, int enable);
+
+/* Runs nframes frames with plain launches and a hipEvent pair around each
+ * kernel, synchronises, and adds each kernel's summed elapsed time in
+ * milliseconds to kernel_ms[0..WRACH_NUM_FRAME_KERNELS). */
+int wrach_step_profiled(wrach_ctx *ctx, uint32_t nframes, double *kernel_ms);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *wrach_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WRACH_H */

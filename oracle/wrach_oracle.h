@@ -1,0 +1,137 @@
+/*
+ * wrach_oracle.h -- CPU restatement of wrach's per-frame particle step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP path is compared
+ * against; it is never linked into, loaded by or called from the product
+ * library (wrach_amd/csrc).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may use it.
+ *
+ * Every function restates a specific piece of the reference (tombh/wrach);
+ * the citation is on each declaration.  Arithmetic is scalar IEEE-754 f32
+ * with correctly rounded division and square root and no FMA contraction
+ * (the file is compiled with -ffp-contract=off), which is what the
+ * reference's shader crate computes when built natively (Rust never
+ * contracts a*b+c).
+ *
+ * Parity pins (what checks each function independently of its own code):
+ *   wo_cell_coord        golden vectors of runners/bevy/src/spatial_bin.rs:156-183
+ *   wo_active_grid       golden vectors of runners/bevy/src/spatial_bin.rs:185-248
+ *   wo_max_particles     runners/bevy/src/particle_store.rs:231-234 (74),
+ *                        runners/api/src/lib.rs:122-125 (164)
+ *   wo_pack              runners/bevy/src/particle_store.rs:143-229 and
+ *                        runners/bevy/src/compute/03_prefix_sum.rs:196-199;
+ *                        brute force (Python stable sort) on random inputs
+ *   wo_collide_cell      shaders/physics/src/particles.rs:128-146 (golden),
+ *                        closed forms for a separated pair, coincident pair
+ *   wo_finish_particle   closed forms: integrate, wall clamp+reflect order,
+ *                        velocity clamp, exactly-on-wall, NaN-free
+ *   wo_physics           reference GPU tests after 4 ticks
+ *                        (03_prefix_sum.rs:150-260, 04_pack_particle_data.rs:
+ *                        72-140) plus a pure-Python brute-force step on tiny
+ *                        worlds
+ *   wo_rebin             brute force (Python stable sort by cell), invariants
+ *                        (permutation, sortedness, counts)
+ *   wo_step_mt           equality with wo_step (serial) on every tested case
+ */
+#ifndef WRACH_ORACLE_H
+#define WRACH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-cell collision cap: SPATIAL_BIN_CELL_SIZE^2 * CELL_LEEWAY = 3*3*1.0
+ * (shaders/physics/src/cell.rs:21,29-30).  It does not follow the runtime
+ * cell size. */
+#define WO_MAX_PARTICLES_IN_CELL 9
+
+/* Step parameters.  The first eight fields are the reference's 32-byte
+ * WorldSettings uniform (shaders/shared/src/lib.rs:19-31,
+ * runners/bevy/src/config_shader.rs:15-29, assets/shaders/types.wgsl:3-15).
+ * gravity and dt are extensions: gravity == (0,0) and dt == 1 reproduce the
+ * reference bit for bit. */
+typedef struct wo_params {
+    float view_dimensions[2];
+    float view_anchor[2];
+    uint32_t grid_dimensions[2];
+    uint32_t cell_size;
+    uint32_t particles_in_frame_count;
+    float gravity[2];
+    float dt;
+} wo_params;
+
+/* SpatialBin::get_cell_coord (runners/bevy/src/spatial_bin.rs:48-64):
+ * f32::div_euclid by the cell size, then a saturating `as i32`. */
+void wo_cell_coord(float x, float y, float cell_size, int32_t out[2]);
+
+/* SpatialBin::get_active_cells (spatial_bin.rs:68-95): bottom-left cell
+ * coordinate and grid dimensions of viewport (x0, y0, x1, y1). */
+void wo_active_grid(const float viewport[4], float cell_size,
+                    int32_t bottom_left[2], uint32_t grid[2]);
+
+/* ParticleStore::max_particles_per_frame (particle_store.rs:116-133):
+ * n_cells*cs^2 plus 10 * ceil(n_cells*cs^2 / 100).  Returns 0 on u32
+ * overflow. */
+uint32_t wo_max_particles(uint64_t n_cells, uint32_t cell_size);
+
+/* ParticleStore::add_particle + SpatialBin::create_packed_data
+ * (particle_store.rs:54-59, spatial_bin.rs:103-149): particles in insertion
+ * order; cells of the active grid in row-major order; insertion order inside
+ * a cell; particles outside the grid are not packed.  indices gets
+ * [0, 0, c0, c0+c1, ..., N] (grid cells + 2 entries).  Returns the number of
+ * packed particles. */
+uint32_t wo_pack(const float *pos_xy, const float *vel_xy, uint32_t n,
+                 const float viewport[4], float cell_size,
+                 uint32_t *indices, float *pos_out, float *vel_out);
+
+/* Particles::new + pairs + push_close_particles_apart
+ * (shaders/physics/src/particles.rs:31-93): in-place Gauss-Seidel separation
+ * of m <= 9 particles, pairs (i<j) in index order.  pos_xy is updated in
+ * place (2*m floats). */
+void wo_collide_cell(float *pos_xy, uint32_t m);
+
+/* Particle::integrate + enforce_limits (shaders/physics/src/particle.rs:
+ * 40-82), with the gravity/dt extension placed just before integrate. */
+void wo_finish_particle(const wo_params *p, float pos[2], float vel[2]);
+
+/* The physics pass over every cell (shaders/physics/src/lib.rs:41-64,
+ * cell.rs:52-112): collide the first min(n,9) particles of each cell,
+ * integrate/limit all of them, write at the same packed index.  Reads the
+ * full `indices` before anything is cleared (the reference's in-kernel
+ * clear races, cell.rs:116-131). */
+void wo_physics(const wo_params *p, const uint32_t *indices,
+                const float *pos_in, const float *vel_in,
+                float *pos_out, float *vel_out);
+
+/* Cell key of a position written by the physics pass
+ * (assets/shaders/particles_per_cell.wgsl:14-27): floor((x-ax)/cs) etc.
+ * Returns -1 if the position falls outside the grid. */
+int64_t wo_cell_key(const wo_params *p, float x, float y);
+
+/* Re-bin (count + exclusive scan + scatter: particles_per_cell.wgsl,
+ * prefix_sum.wgsl, pack_new_particle_data.wgsl) as a STABLE counting sort by
+ * cell key: inside a cell, particles keep their pos_out index order.
+ * indices gets [0, start_0, ..., start_{C-1}, N].  Returns 0, or -1 if a key
+ * falls outside the grid. */
+int wo_rebin(const wo_params *p, const float *pos_out, const float *vel_out,
+             uint32_t *indices, float *pos_in, float *vel_in);
+
+/* One frame: wo_physics then wo_rebin.  Scratch: 4*N floats.  Returns 0 or
+ * -1 (key outside the grid). */
+int wo_step(const wo_params *p, uint32_t *indices, float *pos, float *vel,
+            float *scratch);
+
+/* The same frame on `nthreads` host threads: cells split into contiguous
+ * ranges for the physics pass, then a parallel stable counting sort
+ * (per-thread windowed histograms, scan, per-thread stable scatter).
+ * Output identical to wo_step.  Scratch: 4*N floats. */
+int wo_step_mt(const wo_params *p, uint32_t *indices, float *pos, float *vel,
+               float *scratch, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WRACH_ORACLE_H */

@@ -1,0 +1,282 @@
+"""HIP path vs the oracle, through the C ABI (needs a GPU).
+
+Bar: cell offsets, per-cell counts and in-cell order are compared exactly;
+positions and velocities are compared bit for bit, which implies the
+stated per-step tolerance (max abs error <= 1e-5 world units, TOL below).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from wrach_amd import Particle, World, WrachAPI, WrachConfig, WrachError, workloads
+from wrach_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def f32(a):
+    return np.asarray(a, np.float32)
+
+
+def assert_same_state(gpu, ref, what=""):
+    gi, gp, gv = gpu
+    ri, rp, rv = ref
+    assert np.array_equal(gi, ri), f"{what}: indices differ at {np.flatnonzero(gi != ri)[:10]}"
+    assert gp.shape == rp.shape and gv.shape == rv.shape, what
+    err = max(float(np.max(np.abs(gp - rp), initial=0.0)), float(np.max(np.abs(gv - rv), initial=0.0)))
+    assert err <= TOL, f"{what}: max abs error {err}"
+    bad = np.flatnonzero(np.any(gp.view(np.uint32) != rp.view(np.uint32), axis=1) |
+                         np.any(gv.view(np.uint32) != rv.view(np.uint32), axis=1))
+    assert bad.size == 0, f"{what}: {bad.size} particles differ in bits, first {bad[:8]}"
+
+
+def oracle_world(oracle, cfg: WrachConfig, pos, vel):
+    w, h = cfg.dimensions
+    idx, p, v = oracle.pack(pos, vel, (0, 0, w, h), cfg.cell_size)
+    P = oracle.params(w, h, cfg.cell_size, len(p), gravity=cfg.gravity, dt=cfg.dt)
+    return P, (idx, p, v)
+
+
+def run_parity(oracle, cfg, pos, vel, steps, every=1, label="", threads=1):
+    say(f"[parity] {label}: N={len(pos)} dims={cfg.dimensions} cs={cfg.cell_size} g={cfg.gravity} steps={steps}")
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        assert_same_state(w.read(), ref, f"{label} upload")
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            w.step(k)
+            ref = oracle.step(P, *ref, nsteps=k, nthreads=threads)
+            done += k
+            assert_same_state(w.read(), ref, f"{label} frame {done}")
+    return ref
+
+
+# ------------------------------------------------------ the reference's tests
+
+@pytest.mark.parametrize("case", GOLD["gpu_frames"], ids=lambda c: c["source"].split(" ")[-1])
+def test_reference_gpu_tests_through_wrachapi(oracle, case):
+    say(f"[ref] {case['source']}")
+    w, h = case["dimensions"]
+    cfg = WrachConfig(dimensions=(w, h), cell_size=case["cell_size"])
+    api = WrachAPI(cfg)
+    parts = [Particle(tuple(p), tuple(v)) for p, v in zip(case["positions"], case["velocities"])]
+    api.add_particles(parts)
+    for _ in range(case["ticks"]):
+        api.tick()
+    state = api.get_simulation_state()
+    cpu_idx, _, _ = oracle.pack(case["positions"], case["velocities"], (0, 0, w, h), cfg.cell_size)
+    if "expected_indices" in case:
+        assert state.packed_data.indices.tolist() == case["expected_indices"]
+    if case.get("indices_equal_cpu_pack"):
+        assert state.packed_data.indices.tolist() == cpu_idx.tolist()
+    if "expected_positions" in case:
+        n = len(case["expected_positions"])
+        got = state.packed_data.positions[:n]
+        # canonical order is exact here; the reference sorts the first cell
+        assert np.array_equal(got.view(np.uint32), f32(case["expected_positions"]).view(np.uint32))
+    if "expected_readback_length" in case:
+        assert len(api.positions) == case["expected_readback_length"]
+        assert len(api.velocities) == case["expected_readback_length"]
+        assert api.positions[0] != (0.0, 0.0)
+        assert api.velocities[0] != (0.0, 0.0)
+    api.close()
+
+
+# ------------------------------------------------------ step parity, small
+
+@pytest.mark.parametrize("gravity", [(0.0, 0.0), (0.0, -0.1)], ids=["reference", "gravity"])
+def test_uniform_16k_every_frame(oracle, gravity):
+    """16,384 particles in 512x512, compared after every one of 100 frames."""
+    wl = workloads.small_uniform(seed=0, gravity=gravity)
+    run_parity(oracle, wl.config, wl.pos, wl.vel, 100, every=1, label=wl.name)
+
+
+@pytest.mark.parametrize("cs", [2, 3, 5])
+def test_cell_sizes(oracle, cs):
+    r = np.random.default_rng(30 + cs)
+    n = 6000
+    pos = f32(r.uniform(0, [200, 120], (n, 2)))
+    vel = f32(r.uniform(-1.3, 1.3, (n, 2)))
+    cfg = WrachConfig(dimensions=(200, 120), cell_size=cs, gravity=(0.0, -0.1))
+    run_parity(oracle, cfg, pos, vel, 30, every=3, label=f"cs{cs}")
+
+
+# ------------------------------------------------------ edge cases
+
+def test_empty_world(oracle):
+    say("[edge] empty world")
+    cfg = WrachConfig(dimensions=(30, 20), cell_size=1)
+    with World(cfg) as w:
+        w.step(5)
+        idx, p, v = w.read()
+        assert idx.tolist() == [0] * (31 * 21 + 2) and len(p) == 0
+        w.add_particles(np.zeros((0, 2)), np.zeros((0, 2)))
+        w.step(2)
+        assert w.read()[0].tolist() == [0] * (31 * 21 + 2)
+
+
+def test_single_and_coincident_particles(oracle):
+    pos = f32([[5.5, 5.5], [7.25, 3.5], [7.25, 3.5], [7.25, 3.5], [0.0, 0.0], [20.0, 10.0]])
+    vel = f32([[0.3, -0.2], [0.0, 0.0], [0.0, 0.0], [0.1, 0.1], [0.0, 0.0], [0.5, 0.5]])
+    cfg = WrachConfig(dimensions=(20, 10), cell_size=1)
+    run_parity(oracle, cfg, pos, vel, 20, label="coincident+walls")
+
+
+@pytest.mark.parametrize("per_cell", [2, 3, 4, 5, 8, 9, 10, 16, 17, 40, 300, 2048, 2049, 5000])
+def test_full_cells(oracle, per_cell):
+    """Register paths (<=16), block-cooperative LDS sort (<=2048) and the
+    merge sort (>2048), all with the 9-particle collision cap."""
+    r = np.random.default_rng(per_cell)
+    cells = [(3, 4), (10, 2), (11, 2), (20, 7)]
+    pos, vel = [], []
+    for cx, cy in cells:
+        pos.append(r.uniform([cx, cy], [cx + 1, cy + 1], (per_cell, 2)))
+        vel.append(r.uniform(-1.2, 1.2, (per_cell, 2)))
+    extra = r.uniform(0, [24, 12], (500, 2))
+    pos = f32(np.concatenate(pos + [extra]))
+    vel = f32(np.concatenate(vel + [r.uniform(-1, 1, (500, 2))]))
+    perm = r.permutation(len(pos))  # insertion order differs from cell order
+    cfg = WrachConfig(dimensions=(24, 12), cell_size=1, gravity=(0.0, -0.1))
+    steps = 6 if per_cell >= 2048 else 12
+    run_parity(oracle, cfg, pos[perm], vel[perm], steps, label=f"full cells x{per_cell}")
+
+
+def test_particles_outside_grid_are_dropped(oracle):
+    pos = f32([[1.0, 1.0], [-0.5, 2.0], [9.0, 9.0], [10.0, 3.0], [3.0, 10.5], [1e30, 1.0]])
+    vel = f32(np.full((6, 2), 0.25))
+    cfg = WrachConfig(dimensions=(10, 10), cell_size=1)
+    ref = run_parity(oracle, cfg, pos, vel, 4, label="off-grid")
+    assert len(ref[1]) == 3
+
+
+def test_velocity_and_wall_extremes(oracle):
+    pos = f32([[0.0, 0.0], [64.0, 32.0], [63.9, 0.1], [0.1, 31.9], [32.0, 16.0], [10.0, 10.0]])
+    vel = f32([[-5.0, -5.0], [7.0, 3.0], [1e9, -1e9], [-0.0, 0.0], [np.inf, -np.inf], [1.0, -1.0]])
+    cfg = WrachConfig(dimensions=(64, 32), cell_size=1, gravity=(0.0, -0.1))
+    run_parity(oracle, cfg, pos, vel, 10, label="extremes")
+
+
+def test_errors_leave_state_usable(oracle):
+    say("[edge] error paths")
+    cfg = WrachConfig(dimensions=(6, 6), cell_size=2)  # capacity 74
+    with World(cfg) as w:
+        assert w.capacity == 74
+        w.add_particles(f32([[1, 1]] * 70), f32([[0, 0]] * 70))
+        with pytest.raises(WrachError) as e:
+            w.add_particles(f32([[2, 2]] * 5), f32([[0, 0]] * 5))
+        assert e.value.code == _lib.WRACH_E_CAPACITY
+        assert w.n_particles == 70
+        with pytest.raises(WrachError) as e:
+            w.add_particles(f32([[np.nan, 1]]), f32([[0, 0]]))
+        assert e.value.code == _lib.WRACH_E_NONFINITE
+        w.step(3)
+        assert w.read()[0][-1] == 70
+
+
+def test_add_particles_repacks_the_store(oracle):
+    """state.rs:90-101: add_particles re-packs the whole host store (which the
+    frames never write back, particle_store.rs:77-85) and replaces the device
+    state."""
+    say("[edge] add after ticks")
+    r = np.random.default_rng(40)
+    a_p, a_v = f32(r.uniform(0, 30, (200, 2))), f32(r.uniform(-1, 1, (200, 2)))
+    b_p, b_v = f32(r.uniform(0, 30, (50, 2))), f32(r.uniform(-1, 1, (50, 2)))
+    cfg = WrachConfig(dimensions=(30, 30), cell_size=1)
+    with World(cfg) as w:
+        w.add_particles(a_p, a_v)
+        w.step(7)
+        w.add_particles(b_p, b_v)
+        _, ref = oracle_world(oracle, cfg, np.concatenate([a_p, b_p]), np.concatenate([a_v, b_v]))
+        assert_same_state(w.read(), ref, "repacked")
+
+
+def test_graph_replay_matches_plain_launches(oracle):
+    """Replays of the captured frame graph, with eager work and readbacks in
+    between, against plain stream launches."""
+    wl = workloads.small_uniform(seed=3)
+    say("[graph] 16 + 1 frame graphs vs plain launches")
+    with World(wl.config) as a, World(wl.config) as b:
+        a.add_particles(wl.pos, wl.vel)
+        b.add_particles(wl.pos, wl.vel)
+        b.set_graph(False)
+        for k in (1, 16, 3, 16, 16, 2, 33):
+            a.step(k)
+            b.step(k)
+            sa, sb = a.read(), b.read()
+            assert_same_state(sa, sb, f"graph after +{k}")
+
+
+def test_profiled_step_matches_and_times(oracle):
+    wl = workloads.small_uniform(seed=4)
+    say("[profile] step_profiled")
+    P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        ms = w.step_profiled(5)
+        assert set(ms) == {"collide", "scan", "scatter"} and all(v > 0 for v in ms.values())
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=5), "profiled")
+
+
+def test_device_views(oracle):
+    wl = workloads.small_uniform(seed=5)
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        w.step(3)
+        v = w.device_state()
+        c = w.device_state(canonical=True)
+        assert v.n_particles == c.n_particles == wl.n
+        assert v.n_indices == c.n_indices == wl.cells + 2
+        assert v.positions and c.positions and v.stream
+
+
+# ------------------------------------------------------ full-size workloads
+
+def test_lattice_1m_full_size(oracle):
+    """1,000,000 particles, 2048x1024, cs 1, gravity: exact for 12 frames."""
+    wl = workloads.lattice_1m()
+    run_parity(oracle, wl.config, wl.pos, wl.vel, 12, every=4, label=wl.name, threads=THREADS)
+
+
+def test_lattice_1m_rollout_and_invariants(oracle):
+    """300 frames: still exact against the oracle, plus the size-independent
+    invariants (particle count, sorted offsets, every particle in its cell,
+    inside the walls, |v| <= 1)."""
+    wl = workloads.lattice_1m()
+    say(f"[rollout] {wl.name} 300 frames")
+    P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        w.step(300)
+        idx, pos, vel = w.read()
+    ref = oracle.step(P, *ref, nsteps=300, nthreads=THREADS)
+    assert_same_state((idx, pos, vel), ref, "lattice 300")
+    W, H = wl.config.dimensions
+    gx = W + 1
+    assert idx[0] == 0 and idx[1] == 0 and idx[-1] == wl.n
+    assert np.all(np.diff(idx.astype(np.int64)) >= 0)
+    keys = np.floor(pos[:, 1]).astype(np.int64) * gx + np.floor(pos[:, 0]).astype(np.int64)
+    cell_of = np.repeat(np.arange(len(idx) - 2), np.diff(idx[1:].astype(np.int64)))
+    assert np.array_equal(keys, cell_of)
+    assert pos.min() >= 0 and pos[:, 0].max() <= W and pos[:, 1].max() <= H
+    assert np.abs(vel).max() <= 1.0
+
+
+def test_blobs_4m_full_size(oracle):
+    """4,000,000 particles in 64 Gaussian blobs, 4096x4096 (16.8M cells, more
+    than the reference's 4,194,304-cell scan limit): exact for 3 frames."""
+    wl = workloads.gaussian_blobs()
+    run_parity(oracle, wl.config, wl.pos, wl.vel, 3, every=1, label=wl.name, threads=THREADS)

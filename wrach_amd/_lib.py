@@ -1,0 +1,129 @@
+"""ctypes binding of libwrach_hip.so (include/wrach.h).
+
+Loading fails loudly: there is no CPU fallback for the step.  Build the
+library first (`python -m wrach_amd.build` or __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libwrach_hip.so")
+
+WRACH_OK = 0
+WRACH_E_INVALID = -1
+WRACH_E_TOO_MANY_CELLS = -2
+WRACH_E_CAPACITY = -3
+WRACH_E_NONFINITE = -4
+WRACH_E_DEVICE = -5
+WRACH_E_BUFFER = -6
+WRACH_E_STATE = -7
+NUM_FRAME_KERNELS = 3
+FRAME_KERNEL_NAMES = ("collide", "scan", "scatter")
+
+# Every symbol include/wrach.h declares.
+EXPORTS = (
+    "wrach_config_default",
+    "wrach_create",
+    "wrach_destroy",
+    "wrach_add_particles",
+    "wrach_step",
+    "wrach_read",
+    "wrach_capacity",
+    "wrach_settings",
+    "wrach_sync",
+    "wrach_device_state",
+    "wrach_canonical_device",
+    "wrach_set_graph",
+    "wrach_step_profiled",
+    "wrach_last_error",
+)
+
+
+class WrachConfigC(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint16),
+        ("height", ctypes.c_uint16),
+        ("cell_size", ctypes.c_uint16),
+        ("boundaries_as_dimensions", ctypes.c_uint8),
+        ("reserved0", ctypes.c_uint8),
+        ("gravity", ctypes.c_float * 2),
+        ("dt", ctypes.c_float),
+        ("flags", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+class WorldSettingsC(ctypes.Structure):
+    _fields_ = [
+        ("view_dimensions", ctypes.c_float * 2),
+        ("view_anchor", ctypes.c_float * 2),
+        ("grid_dimensions", ctypes.c_uint32 * 2),
+        ("cell_size", ctypes.c_uint32),
+        ("particles_in_frame_count", ctypes.c_uint32),
+    ]
+
+
+class DeviceViewC(ctypes.Structure):
+    _fields_ = [
+        ("indices", ctypes.c_void_p),
+        ("positions", ctypes.c_void_p),
+        ("velocities", ctypes.c_void_p),
+        ("n_particles", ctypes.c_uint32),
+        ("n_indices", ctypes.c_uint32),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+assert ctypes.sizeof(WorldSettingsC) == 32
+
+
+class WrachError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"wrach error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m wrach_amd.build` "
+            "(the particle step has no CPU fallback)")
+    L = ctypes.CDLL(path)
+    vp, f32p, u32p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32)
+    L.wrach_config_default.argtypes = [ctypes.POINTER(WrachConfigC)]
+    L.wrach_config_default.restype = None
+    L.wrach_create.argtypes = [ctypes.POINTER(WrachConfigC), ctypes.POINTER(vp)]
+    L.wrach_destroy.argtypes = [vp]
+    L.wrach_destroy.restype = None
+    L.wrach_add_particles.argtypes = [vp, f32p, f32p, ctypes.c_uint32]
+    L.wrach_step.argtypes = [vp, ctypes.c_uint32]
+    L.wrach_read.argtypes = [vp, f32p, f32p, ctypes.c_size_t, u32p, ctypes.c_size_t, u32p]
+    L.wrach_capacity.argtypes = [vp, u32p, u32p]
+    L.wrach_settings.argtypes = [vp, ctypes.POINTER(WorldSettingsC)]
+    L.wrach_sync.argtypes = [vp]
+    L.wrach_device_state.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
+    L.wrach_canonical_device.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
+    L.wrach_set_graph.argtypes = [vp, ctypes.c_int]
+    L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    L.wrach_last_error.argtypes = []
+    L.wrach_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name not in ("wrach_config_default", "wrach_destroy", "wrach_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != WRACH_OK:
+        msg = load().wrach_last_error().decode(errors="replace")
+        raise WrachError(rc, msg)

@@ -1,0 +1,221 @@
+"""Python mirror of wrach's `runners/api` surface over the C ABI.
+
+`WrachAPI`, `WrachConfig`, `Particle` and `WrachState` keep the names,
+argument meaning and error behaviour of runners/api/src/lib.rs:17-88,
+runners/bevy/src/config_app.rs:10-35 and runners/bevy/src/state.rs:17-59, so
+tests read like the reference's own.  `World` is the thin numpy-level handle
+used by the bench and the parity tests.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import WrachError, check
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+@dataclass
+class WrachConfig:
+    """WrachConfig (config_app.rs:10-35); defaults of config_app.rs:26-35.
+
+    gravity and dt are extensions: (0, 0) and 1.0 reproduce the reference."""
+
+    dimensions: tuple = (480, 352)
+    boundaries_as_dimensions: bool = False
+    cell_size: int = 3
+    gravity: tuple = (0.0, 0.0)
+    dt: float = 1.0
+    device: int = 0
+
+    def to_c(self) -> _lib.WrachConfigC:
+        c = _lib.WrachConfigC()
+        _lib.load().wrach_config_default(ctypes.byref(c))
+        c.width, c.height = int(self.dimensions[0]), int(self.dimensions[1])
+        c.cell_size = int(self.cell_size)
+        c.boundaries_as_dimensions = 1 if self.boundaries_as_dimensions else 0
+        c.gravity[0], c.gravity[1] = float(self.gravity[0]), float(self.gravity[1])
+        c.dt = float(self.dt)
+        c.device = int(self.device)
+        return c
+
+
+@dataclass
+class Particle:
+    """Particle (state.rs:41-51)."""
+
+    position: tuple
+    velocity: tuple
+
+
+@dataclass
+class PackedData:
+    """PackedData (spatial_bin.rs:22-33); positions/velocities are capacity
+    long and zero padded, as the staging buffers the reference reads back."""
+
+    indices: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    positions: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.float32))
+    velocities: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.float32))
+
+
+@dataclass
+class ShaderWorldSettings:
+    """ShaderWorldSettings (config_shader.rs:15-29)."""
+
+    view_dimensions: tuple
+    view_anchor: tuple
+    grid_dimensions: tuple
+    cell_size: int
+    particles_in_frame_count: int
+
+
+@dataclass
+class WrachState:
+    """The parts of WrachState (state.rs:17-33) a caller reads."""
+
+    config: WrachConfig
+    shader_settings: ShaderWorldSettings
+    packed_data: PackedData
+
+
+class World:
+    """One simulated world on one GPU (a wrach_ctx)."""
+
+    def __init__(self, config: WrachConfig):
+        L = _lib.load()
+        self._L = L
+        self.config = config
+        h = ctypes.c_void_p()
+        cfg = config.to_c()
+        check(L.wrach_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        cap, nidx = ctypes.c_uint32(), ctypes.c_uint32()
+        check(L.wrach_capacity(h, ctypes.byref(cap), ctypes.byref(nidx)))
+        self.capacity = cap.value
+        self.n_indices = nidx.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.wrach_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_particles(self, pos, vel) -> None:
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 2)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 2)
+        if pos.shape != vel.shape:
+            raise ValueError("positions and velocities differ in length")
+        check(self._L.wrach_add_particles(self._h, pos.ctypes.data_as(_f32p),
+                                          vel.ctypes.data_as(_f32p), len(pos)))
+
+    def step(self, nframes: int = 1) -> None:
+        check(self._L.wrach_step(self._h, int(nframes)))
+
+    def sync(self) -> None:
+        check(self._L.wrach_sync(self._h))
+
+    def set_graph(self, enable: bool) -> None:
+        check(self._L.wrach_set_graph(self._h, 1 if enable else 0))
+
+    def settings(self) -> ShaderWorldSettings:
+        s = _lib.WorldSettingsC()
+        check(self._L.wrach_settings(self._h, ctypes.byref(s)))
+        return ShaderWorldSettings(tuple(s.view_dimensions), tuple(s.view_anchor),
+                                   tuple(s.grid_dimensions), s.cell_size,
+                                   s.particles_in_frame_count)
+
+    @property
+    def n_particles(self) -> int:
+        return self.settings().particles_in_frame_count
+
+    def read(self):
+        """Canonical packed triple (indices, positions[N,2], velocities[N,2])."""
+        n = self.n_particles
+        pos = np.zeros((max(n, 1), 2), np.float32)
+        vel = np.zeros((max(n, 1), 2), np.float32)
+        idx = np.zeros(self.n_indices, np.uint32)
+        got = ctypes.c_uint32()
+        check(self._L.wrach_read(self._h, pos.ctypes.data_as(_f32p), vel.ctypes.data_as(_f32p),
+                                 pos.shape[0], idx.ctypes.data_as(_u32p), len(idx),
+                                 ctypes.byref(got)))
+        return idx, pos[:got.value], vel[:got.value]
+
+    def step_profiled(self, nframes: int):
+        """Per-kernel summed milliseconds over nframes (HIP events)."""
+        ms = (ctypes.c_double * _lib.NUM_FRAME_KERNELS)()
+        check(self._L.wrach_step_profiled(self._h, int(nframes), ms))
+        return dict(zip(_lib.FRAME_KERNEL_NAMES, list(ms)))
+
+    def device_state(self, canonical: bool = False) -> _lib.DeviceViewC:
+        v = _lib.DeviceViewC()
+        fn = self._L.wrach_canonical_device if canonical else self._L.wrach_device_state
+        check(fn(self._h, ctypes.byref(v)))
+        return v
+
+
+class WrachAPI:
+    """WrachAPI (runners/api/src/lib.rs:17-88)."""
+
+    def __init__(self, config: WrachConfig | None = None):
+        self.config = config if config is not None else WrachConfig()
+        self.world = World(self.config)
+        self.positions: list = []
+        self.velocities: list = []
+        self._packed = PackedData()
+
+    def tick(self) -> None:
+        """One frame, then read_data (lib.rs:49-52)."""
+        self.world.step(1)
+        self._readback()
+        self.read_data()
+
+    def _readback(self) -> None:
+        # plugin/build.rs:135-158: the three staging buffers, capacity long.
+        idx, pos, vel = self.world.read()
+        cap = self.world.capacity
+        p = np.zeros((cap, 2), np.float32)
+        v = np.zeros((cap, 2), np.float32)
+        p[:len(pos)] = pos
+        v[:len(vel)] = vel
+        self._packed = PackedData(idx, p, v)
+
+    def read_data(self) -> None:
+        """lib.rs:58-74: positions/velocities as (x, y) tuples."""
+        self.positions = [(float(x), float(y)) for x, y in self._packed.positions]
+        self.velocities = [(float(x), float(y)) for x, y in self._packed.velocities]
+
+    def add_particles(self, particles) -> None:
+        """lib.rs:78-81 -> state.rs:90-101."""
+        if not particles:
+            pos = np.zeros((0, 2), np.float32)
+            vel = np.zeros((0, 2), np.float32)
+        else:
+            pos = np.array([p.position for p in particles], np.float32)
+            vel = np.array([p.velocity for p in particles], np.float32)
+        self.world.add_particles(pos, vel)
+
+    def get_simulation_state(self) -> WrachState:
+        return WrachState(self.config, self.world.settings(), self._packed)
+
+    def close(self) -> None:
+        self.world.close()
+
+
+__all__ = ["WrachAPI", "WrachConfig", "Particle", "PackedData", "WrachState",
+           "ShaderWorldSettings", "World", "WrachError"]

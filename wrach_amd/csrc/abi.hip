@@ -1,0 +1,481 @@
+// Host side of libwrach_hip.so: the C ABI declared in include/wrach.h.
+//
+// The context owns the host particle store (the reference's ParticleStore,
+// runners/bevy/src/particle_store.rs:14-27), the device buffers of the
+// compute worker (runners/bevy/src/compute/builder.rs:70-84), one HIP stream
+// and the captured frame graphs.
+#include "frame.h"
+#include "wrach.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define WRACH_HIP(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(WRACH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Rust's `f as i32` (saturating, NaN -> 0).
+int32_t sat_i32(float f) {
+    if (std::isnan(f)) return 0;
+    if (f >= 2147483648.0f) return std::numeric_limits<int32_t>::max();
+    if (f <= -2147483648.0f) return std::numeric_limits<int32_t>::min();
+    return (int32_t)f;
+}
+
+// f32::div_euclid for a positive divisor.
+float div_euclid(float a, float b) {
+    const float q = std::trunc(a / b);
+    return std::fmod(a, b) < 0.0f ? q - 1.0f : q;
+}
+
+// SpatialBin::get_cell_coord (runners/bevy/src/spatial_bin.rs:48-64).
+inline void cell_coord(float x, float y, float cs, int32_t &cx, int32_t &cy) {
+    cx = sat_i32(div_euclid(x, cs));
+    cy = sat_i32(div_euclid(y, cs));
+}
+
+// ParticleStore::max_particles_per_frame (particle_store.rs:116-133);
+// 0 if it does not fit u32 (where the reference's `expect` would panic).
+uint32_t max_particles_per_frame(uint64_t cells, uint32_t cs) {
+    const uint64_t normal = cells * cs * cs;
+    const uint64_t total = normal + 10 * ((normal + 99) / 100);
+    return total > 0xffffffffull ? 0 : (uint32_t)total;
+}
+
+template <typename T>
+hipError_t dmalloc(T **p, size_t count) {
+    return hipMalloc((void **)p, count * sizeof(T) + 256);
+}
+
+// Grid, in cells, of the viewport (0, 0, W, H): SpatialBin::get_active_cells
+// (spatial_bin.rs:68-95) with bottom-left (0,0), so gx = floor(W/cs) + 1.
+constexpr uint64_t kMaxCells = (1ull << 31) - 8;
+
+}  // namespace
+
+struct wrach_ctx {
+    wrach_config cfg{};
+    wrach_world_settings settings{};
+    wrach::FrameParams fp{};
+    wrach::FrameBuffers buf{};
+    uint32_t capacity = 0;
+    uint64_t ncell = 0;
+    void *idx_alloc = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph_one = nullptr;
+    hipGraphExec_t graph_many = nullptr;
+    bool use_graph = true;
+    bool canonical_valid = false;
+    // Host store: every particle ever added, in insertion order.  Cells of
+    // the hashmap store keep insertion order, so a stable sort by cell of
+    // this list is the packed layout (particle_store.rs:54-59).
+    std::vector<float> store_pos, store_vel;
+};
+
+namespace {
+
+constexpr uint32_t kFramesPerGraph = 16;
+
+void free_graphs(wrach_ctx *c) {
+    if (c->graph_one) (void)hipGraphExecDestroy(c->graph_one);
+    if (c->graph_many) (void)hipGraphExecDestroy(c->graph_many);
+    c->graph_one = c->graph_many = nullptr;
+}
+
+int enqueue_frame(wrach_ctx *c) {
+    WRACH_HIP(wrach::launch_collide(c->fp, c->buf, c->stream));
+    WRACH_HIP(wrach::launch_scan(c->fp, c->buf, c->stream));
+    WRACH_HIP(wrach::launch_scatter(c->fp, c->buf, c->stream));
+    return WRACH_OK;
+}
+
+int capture(wrach_ctx *c, uint32_t frames, hipGraphExec_t *out) {
+    hipGraph_t g = nullptr;
+    WRACH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = WRACH_OK;
+    for (uint32_t f = 0; f < frames && rc == WRACH_OK; ++f) rc = enqueue_frame(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != WRACH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(WRACH_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(WRACH_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return WRACH_OK;
+}
+
+void release(wrach_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_graphs(c);
+    (void)hipFree(c->idx_alloc);
+    (void)hipFree(c->buf.pos);
+    (void)hipFree(c->buf.vel);
+    (void)hipFree(c->buf.src);
+    (void)hipFree(c->buf.pos_out);
+    (void)hipFree(c->buf.vel_out);
+    (void)hipFree(c->buf.kslot);
+    (void)hipFree(c->buf.sortbuf);
+    (void)hipFree(c->buf.cnt);
+    (void)hipFree(c->buf.tile_state);
+    (void)hipFree(c->buf.ticket);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// The packer: ParticleStore::create_packed_data (particle_store.rs:90-103)
+// -> SpatialBin::create_packed_data (spatial_bin.rs:103-149).  Stable
+// counting sort of the store by row-major cell of the active grid; particles
+// outside the grid are skipped.  indices = [0, 0, c0, c0+c1, ..., N].
+uint32_t pack_store(const wrach_ctx *c, std::vector<uint32_t> &indices,
+                    std::vector<float> &pos, std::vector<float> &vel) {
+    const size_t total = c->store_pos.size() / 2;
+    const uint32_t gx = c->settings.grid_dimensions[0], gy = c->settings.grid_dimensions[1];
+    const float cs = (float)c->cfg.cell_size;
+    std::vector<int64_t> cell(total);
+    indices.assign(c->ncell + 2, 0);
+    for (size_t i = 0; i < total; ++i) {
+        int32_t cx, cy;
+        cell_coord(c->store_pos[2 * i], c->store_pos[2 * i + 1], cs, cx, cy);
+        if (cx < 0 || cy < 0 || (uint32_t)cx >= gx || (uint32_t)cy >= gy) {
+            cell[i] = -1;
+            continue;
+        }
+        cell[i] = (int64_t)cy * gx + cx;
+        indices[cell[i] + 2] += 1;
+    }
+    uint64_t run = 0;
+    for (uint64_t k = 2; k < c->ncell + 2; ++k) {
+        run += indices[k];
+        indices[k] = (uint32_t)run;  // end of cell k-2 == start of cell k-1
+    }
+    if (run > 0xffffffffull) return 0xffffffffu;
+    const uint32_t n = (uint32_t)run;
+    pos.resize(2 * (size_t)n);
+    vel.resize(2 * (size_t)n);
+    std::vector<uint32_t> cursor(indices.begin() + 1, indices.end() - 1);  // start_k
+    for (size_t i = 0; i < total; ++i) {
+        if (cell[i] < 0) continue;
+        const uint32_t d = cursor[cell[i]]++;
+        pos[2 * (size_t)d] = c->store_pos[2 * i];
+        pos[2 * (size_t)d + 1] = c->store_pos[2 * i + 1];
+        vel[2 * (size_t)d] = c->store_vel[2 * i];
+        vel[2 * (size_t)d + 1] = c->store_vel[2 * i + 1];
+    }
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *wrach_last_error(void) { return g_err.c_str(); }
+
+void wrach_config_default(wrach_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->width = 480;
+    cfg->height = 352;
+    cfg->cell_size = 3;
+    cfg->boundaries_as_dimensions = 0;
+    cfg->gravity[0] = cfg->gravity[1] = 0.0f;
+    cfg->dt = 1.0f;
+    cfg->device = 0;
+}
+
+int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
+    if (!cfg || !out) return fail(WRACH_E_INVALID, "wrach_create: null argument");
+    *out = nullptr;
+    if (cfg->cell_size == 0) return fail(WRACH_E_INVALID, "cell_size must be >= 1");
+    if (cfg->flags != 0) return fail(WRACH_E_INVALID, "unknown flags");
+    if (!std::isfinite(cfg->gravity[0]) || !std::isfinite(cfg->gravity[1]) || !std::isfinite(cfg->dt))
+        return fail(WRACH_E_INVALID, "gravity and dt must be finite");
+    int ndev = 0;
+    WRACH_HIP(hipGetDeviceCount(&ndev));
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(WRACH_E_INVALID, "no such HIP device");
+    WRACH_HIP(hipSetDevice(cfg->device));
+
+    const float cs = (float)cfg->cell_size;
+    int32_t tx, ty;
+    cell_coord((float)cfg->width, (float)cfg->height, cs, tx, ty);
+    const uint64_t gx = (uint64_t)tx + 1, gy = (uint64_t)ty + 1;
+    const uint64_t ncell = gx * gy;
+    if (ncell > kMaxCells)
+        return fail(WRACH_E_TOO_MANY_CELLS, "grid has more cells than the 32-bit offsets allow");
+    const uint32_t cap = max_particles_per_frame(ncell, cfg->cell_size);
+    if (cap == 0) return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity does not fit in u32");
+
+    wrach_ctx *c = new wrach_ctx();
+    c->cfg = *cfg;
+    c->ncell = ncell;
+    c->capacity = cap;
+    c->settings.view_dimensions[0] = (float)cfg->width;
+    c->settings.view_dimensions[1] = (float)cfg->height;
+    c->settings.view_anchor[0] = 0.0f;
+    c->settings.view_anchor[1] = 0.0f;
+    c->settings.grid_dimensions[0] = (uint32_t)gx;
+    c->settings.grid_dimensions[1] = (uint32_t)gy;
+    c->settings.cell_size = cfg->cell_size;
+    c->settings.particles_in_frame_count = 0;
+
+    wrach::FrameParams &fp = c->fp;
+    fp.ax = 0.0f;
+    fp.ay = 0.0f;
+    fp.x1 = fp.ax + c->settings.view_dimensions[0];
+    fp.y1 = fp.ay + c->settings.view_dimensions[1];
+    fp.cs = cs;
+    fp.gravx = cfg->gravity[0];
+    fp.gravy = cfg->gravity[1];
+    fp.dt = cfg->dt;
+    fp.gx = (uint32_t)gx;
+    fp.gy = (uint32_t)gy;
+    fp.ncell = (uint32_t)ncell;
+    fp.n = 0;
+
+    wrach::FrameBuffers &b = c->buf;
+    b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell);
+    auto bail = [&](int rc) { release(c); return rc; };
+#define WRACH_HIP_C(expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return bail(fail(WRACH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    WRACH_HIP_C(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const size_t nidx = ncell + 2;
+    WRACH_HIP_C(hipMalloc(&c->idx_alloc, (nidx + 64) * sizeof(uint32_t)));
+    b.idx = reinterpret_cast<uint32_t *>(c->idx_alloc) + 3;  // &idx[1] on a 16-byte boundary
+    WRACH_HIP_C(dmalloc(&b.pos, cap));
+    WRACH_HIP_C(dmalloc(&b.vel, cap));
+    WRACH_HIP_C(dmalloc(&b.src, cap));
+    WRACH_HIP_C(dmalloc(&b.pos_out, cap));
+    WRACH_HIP_C(dmalloc(&b.vel_out, cap));
+    WRACH_HIP_C(dmalloc(&b.kslot, cap));
+    WRACH_HIP_C(dmalloc(&b.sortbuf, cap));
+    WRACH_HIP_C(dmalloc(&b.cnt, ncell + 16));
+    WRACH_HIP_C(dmalloc(&b.tile_state, b.scan_tiles));
+    WRACH_HIP_C(dmalloc(&b.ticket, 1));
+    WRACH_HIP_C(hipMemsetAsync(c->idx_alloc, 0, (nidx + 64) * sizeof(uint32_t), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.pos, 0, cap * sizeof(float2), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.vel, 0, cap * sizeof(float2), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.pos_out, 0, cap * sizeof(float2), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.vel_out, 0, cap * sizeof(float2), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.cnt, 0, (ncell + 16) * sizeof(uint32_t), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.tile_state, 0, b.scan_tiles * sizeof(unsigned long long), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.ticket, 0, sizeof(unsigned long long), c->stream));
+    WRACH_HIP_C(hipStreamSynchronize(c->stream));
+#undef WRACH_HIP_C
+    *out = c;
+    g_err.clear();
+    return WRACH_OK;
+}
+
+void wrach_destroy(wrach_ctx *ctx) { release(ctx); }
+
+int wrach_add_particles(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint32_t n) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (n > 0 && (!pos_xy || !vel_xy)) return fail(WRACH_E_INVALID, "null particle arrays");
+    for (uint32_t i = 0; i < 2 * n; ++i)
+        if (std::isnan(pos_xy[i]) || std::isnan(vel_xy[i]))
+            return fail(WRACH_E_NONFINITE, "NaN in particle " + std::to_string(i / 2));
+    const size_t old = c->store_pos.size();
+    c->store_pos.insert(c->store_pos.end(), pos_xy, pos_xy + 2 * (size_t)n);
+    c->store_vel.insert(c->store_vel.end(), vel_xy, vel_xy + 2 * (size_t)n);
+    std::vector<uint32_t> indices;
+    std::vector<float> pos, vel;
+    const uint32_t packed = pack_store(c, indices, pos, vel);
+    if (packed > c->capacity) {
+        c->store_pos.resize(old);
+        c->store_vel.resize(old);
+        return fail(WRACH_E_CAPACITY, "particles in frame (" + std::to_string(packed) +
+                                          ") exceed the buffer capacity (" +
+                                          std::to_string(c->capacity) + ")");
+    }
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_HIP(hipMemcpyAsync(c->buf.idx, indices.data(), indices.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, c->stream));
+    if (packed) {
+        WRACH_HIP(hipMemcpyAsync(c->buf.pos, pos.data(), pos.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+        WRACH_HIP(hipMemcpyAsync(c->buf.vel, vel.data(), vel.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    WRACH_HIP(wrach::launch_iota(c->buf.src, packed, c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (c->fp.n != packed) free_graphs(c);
+    c->fp.n = packed;
+    c->settings.particles_in_frame_count = packed;
+    c->canonical_valid = false;
+    g_err.clear();
+    return WRACH_OK;
+}
+
+int wrach_step(wrach_ctx *c, uint32_t nframes) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (nframes == 0 || c->fp.n == 0) return WRACH_OK;  // an empty world does not change
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    c->canonical_valid = false;
+    if (!c->use_graph) {
+        for (uint32_t f = 0; f < nframes; ++f) {
+            const int rc = enqueue_frame(c);
+            if (rc) return rc;
+        }
+        return WRACH_OK;
+    }
+    if (nframes >= kFramesPerGraph && !c->graph_many) {
+        const int rc = capture(c, kFramesPerGraph, &c->graph_many);
+        if (rc) return rc;
+    }
+    if (nframes % kFramesPerGraph && !c->graph_one) {
+        const int rc = capture(c, 1, &c->graph_one);
+        if (rc) return rc;
+    }
+    for (uint32_t f = 0; f < nframes / kFramesPerGraph; ++f)
+        WRACH_HIP(hipGraphLaunch(c->graph_many, c->stream));
+    for (uint32_t f = 0; f < nframes % kFramesPerGraph; ++f)
+        WRACH_HIP(hipGraphLaunch(c->graph_one, c->stream));
+    return WRACH_OK;
+}
+
+int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
+    if (!c || !kernel_ms) return fail(WRACH_E_INVALID, "null argument");
+    if (nframes == 0 || c->fp.n == 0) return WRACH_OK;
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    c->canonical_valid = false;
+    constexpr int K = WRACH_NUM_FRAME_KERNELS;
+    std::vector<hipEvent_t> ev(2 * K * (size_t)nframes, nullptr);
+    for (auto &e : ev) WRACH_HIP(hipEventCreate(&e));
+    int rc = WRACH_OK;
+    for (uint32_t f = 0; f < nframes && rc == WRACH_OK; ++f) {
+        hipEvent_t *fe = &ev[2 * K * (size_t)f];
+        hipError_t e = hipSuccess;
+        e = hipEventRecord(fe[0], c->stream);
+        if (e == hipSuccess) e = wrach::launch_collide(c->fp, c->buf, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(fe[1], c->stream);
+        if (e == hipSuccess) e = hipEventRecord(fe[2], c->stream);
+        if (e == hipSuccess) e = wrach::launch_scan(c->fp, c->buf, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(fe[3], c->stream);
+        if (e == hipSuccess) e = hipEventRecord(fe[4], c->stream);
+        if (e == hipSuccess) e = wrach::launch_scatter(c->fp, c->buf, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(fe[5], c->stream);
+        if (e != hipSuccess) rc = fail(WRACH_E_DEVICE, std::string("profiled frame: ") + hipGetErrorString(e));
+    }
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (rc == WRACH_OK && se != hipSuccess)
+        rc = fail(WRACH_E_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+    if (rc == WRACH_OK) {
+        for (uint32_t f = 0; f < nframes; ++f) {
+            for (int k = 0; k < K; ++k) {
+                float ms = 0.0f;
+                hipError_t e = hipEventElapsedTime(&ms, ev[2 * K * (size_t)f + 2 * k],
+                                                   ev[2 * K * (size_t)f + 2 * k + 1]);
+                if (e != hipSuccess) {
+                    rc = fail(WRACH_E_DEVICE, std::string("hipEventElapsedTime: ") + hipGetErrorString(e));
+                    break;
+                }
+                kernel_ms[k] += ms;
+            }
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int wrach_sync(wrach_ctx *c) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    return WRACH_OK;
+}
+
+int wrach_set_graph(wrach_ctx *c, int enable) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    c->use_graph = enable != 0;
+    return WRACH_OK;
+}
+
+int wrach_capacity(wrach_ctx *c, uint32_t *max_particles, uint32_t *n_indices) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (max_particles) *max_particles = c->capacity;
+    if (n_indices) *n_indices = (uint32_t)(c->ncell + 2);
+    return WRACH_OK;
+}
+
+int wrach_settings(wrach_ctx *c, wrach_world_settings *out) {
+    if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
+    *out = c->settings;
+    return WRACH_OK;
+}
+
+int wrach_device_state(wrach_ctx *c, wrach_device_view *out) {
+    if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
+    out->indices = c->buf.idx;
+    out->positions = reinterpret_cast<const float *>(c->buf.pos);
+    out->velocities = reinterpret_cast<const float *>(c->buf.vel);
+    out->n_particles = c->fp.n;
+    out->n_indices = (uint32_t)(c->ncell + 2);
+    out->stream = c->stream;
+    return WRACH_OK;
+}
+
+int wrach_canonical_device(wrach_ctx *c, wrach_device_view *out) {
+    if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    if (!c->canonical_valid && c->fp.n > 0) {
+        WRACH_HIP(wrach::launch_canonical(c->fp, c->buf, c->stream));
+        c->canonical_valid = true;
+    }
+    out->indices = c->buf.idx;
+    out->positions = reinterpret_cast<const float *>(c->fp.n ? c->buf.pos_out : c->buf.pos);
+    out->velocities = reinterpret_cast<const float *>(c->fp.n ? c->buf.vel_out : c->buf.vel);
+    out->n_particles = c->fp.n;
+    out->n_indices = (uint32_t)(c->ncell + 2);
+    out->stream = c->stream;
+    return WRACH_OK;
+}
+
+int wrach_read(wrach_ctx *c, float *pos_xy, float *vel_xy, size_t particle_cap,
+               uint32_t *indices, size_t indices_cap, uint32_t *n_particles) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    const uint32_t n = c->fp.n;
+    if ((pos_xy || vel_xy) && particle_cap < n)
+        return fail(WRACH_E_BUFFER, "particle arrays hold " + std::to_string(particle_cap) +
+                                        " particles, the frame has " + std::to_string(n));
+    if (indices && indices_cap < c->ncell + 2)
+        return fail(WRACH_E_BUFFER, "indices array too small");
+    wrach_device_view v{};
+    int rc = wrach_canonical_device(c, &v);
+    if (rc) return rc;
+    if (indices)
+        WRACH_HIP(hipMemcpyAsync(indices, v.indices, (c->ncell + 2) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (pos_xy && n)
+        WRACH_HIP(hipMemcpyAsync(pos_xy, v.positions, (size_t)n * 2 * sizeof(float),
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (vel_xy && n)
+        WRACH_HIP(hipMemcpyAsync(vel_xy, v.velocities, (size_t)n * 2 * sizeof(float),
+                                 hipMemcpyDeviceToHost, c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (n_particles) *n_particles = n;
+    return WRACH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// Internal interface between the host side of the library (abi.hip) and the
+// frame kernels (kernels.hip).  Not installed; the public ABI is wrach.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wrach {
+
+// Per-frame constants, passed by value to every kernel.  x1/y1 are
+// view_anchor + view_dimensions evaluated in f32, the viewport corner of
+// Particle::enforce_boundaries (shaders/physics/src/particle.rs:46-52).
+struct FrameParams {
+    float ax, ay;        // view_anchor
+    float x1, y1;        // view_anchor + view_dimensions
+    float cs;            // cell size as f32 (particles_per_cell.wgsl:18)
+    float gravx, gravy;  // extension; 0 skips the add
+    float dt;            // extension; 1 integrates p += v exactly
+    uint32_t gx, gy;     // grid_dimensions
+    uint32_t ncell;      // gx * gy
+    uint32_t n;          // particles_in_frame_count
+};
+
+// Device buffers of one world.  `idx` has ncell + 2 entries and is placed 3
+// words past a 256-byte boundary so that &idx[1] is 16-byte aligned (the
+// scan writes idx[1..] with 128-bit stores).
+struct FrameBuffers {
+    uint32_t *idx;        // [0, start_0, ..., start_{C-1}, N]
+    float2 *pos, *vel;    // cell-sorted state; in-cell order is by `src`
+    uint32_t *src;        // canonical index of each particle in the previous frame
+    float2 *pos_out, *vel_out;  // post-collide state in canonical order
+    uint2 *kslot;         // (new cell key, slot inside the new cell)
+    unsigned long long *sortbuf;  // merge-sort ping-pong for very full cells
+    uint32_t *cnt;        // per-cell counters, zero between frames
+    unsigned long long *tile_state;  // scan look-back words
+    unsigned long long *ticket;      // scan tile ticket (monotonic)
+    uint32_t scan_tiles;
+};
+
+constexpr int kCollideBlock = 256;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;            // per thread
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kScatterBlock = 256;
+constexpr int kMaxInCell = 9;            // cell.rs:29-30
+constexpr int kRegCell = 16;             // cells up to this size sort in registers
+constexpr int kLdsSort = 2048;           // keys per LDS bitonic sort
+
+inline uint32_t scan_tiles_for(uint32_t ncell) {
+    return (ncell + kScanTile - 1) / kScanTile;
+}
+
+// Launchers (kernels.hip).  Each enqueues on `stream` and returns the launch
+// status.
+hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream);
+
+}  // namespace wrach

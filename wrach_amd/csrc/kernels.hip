@@ -1,0 +1,594 @@
+// Frame kernels of the particle step, written for gfx950 (wave64).
+//
+// One frame is three launches:
+//   collide_kernel  one lane per cell: restore the cell's canonical order
+//                   (sort by `src`), separate the first 9 particles
+//                   (Gauss-Seidel, shaders/physics/src/particles.rs:62-93),
+//                   integrate and limit every particle (particle.rs:40-82,
+//                   cell.rs:79-95), write pos_out/vel_out at the canonical
+//                   index, compute the new cell key
+//                   (particles_per_cell.wgsl:14-27) and take a slot in the
+//                   new cell with one atomic add.
+//   scan_kernel     single-pass exclusive scan of the per-cell counts with
+//                   decoupled look-back; writes [0, start_0.., N] and clears
+//                   the counts for the next frame (prefix_sum.wgsl +
+//                   cell.rs:116-131).
+//   scatter_kernel  one lane per particle: dest = start[key] + slot
+//                   (pack_new_particle_data.wgsl:39-44); records the source
+//                   index so the next collide pass can rebuild the stable
+//                   order that the reference's CPU packer defines
+//                   (spatial_bin.rs:122-142).
+// Floating point: the library is compiled with -ffp-contract=off, so every
+// a*b+c is a rounded multiply then a rounded add, and f32 division and
+// sqrtf are the correctly rounded sequences -- the native Rust semantics of
+// the reference's shader crate.
+#include "frame.h"
+
+namespace wrach {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Per-particle arithmetic.
+
+// Particle::integrate + enforce_boundaries + enforce_velocity
+// (shaders/physics/src/particle.rs:40-82) with the gravity/dt extension
+// placed just before the integration.  A zero gravity component is skipped so
+// that -0.0 velocities keep their sign, as in the reference.
+__device__ __forceinline__ void finish_particle(const FrameParams &p, float &px, float &py,
+                                                float &vx, float &vy) {
+    if (p.gravx != 0.0f) vx = vx + p.gravx * p.dt;
+    if (p.gravy != 0.0f) vy = vy + p.gravy * p.dt;
+    if (p.dt == 1.0f) {
+        px = px + vx;
+        py = py + vy;
+    } else {
+        px = px + vx * p.dt;
+        py = py + vy * p.dt;
+    }
+    if (px > p.x1) { px = p.x1; vx = vx * -1.0f; }
+    if (px < p.ax) { px = p.ax; vx = vx * -1.0f; }
+    if (py > p.y1) { py = p.y1; vy = vy * -1.0f; }
+    if (py < p.ay) { py = p.ay; vy = vy * -1.0f; }
+    if (vx < -1.0f) vx = -1.0f;
+    if (vx > 1.0f) vx = 1.0f;
+    if (vy < -1.0f) vy = -1.0f;
+    if (vy > 1.0f) vy = 1.0f;
+}
+
+// Particles::pairs body for one pair (particles.rs:65-92): glam distance,
+// skip beyond MIN_DISTANCE = 1, 0 -> 0.0001, force (0.5*(1-d))/d, equal and
+// opposite displacement.
+__device__ __forceinline__ void collide_pair(float &xi, float &yi, float &xj, float &yj) {
+    const float dx = xi - xj;
+    const float dy = yi - yj;
+    float sq = dx * dx;
+    sq = sq + dy * dy;
+    float d = sqrtf(sq);
+    if (d > 1.0f) return;
+    if (d == 0.0f) d = 0.0001f;
+    const float f = (0.5f * (1.0f - d)) / d;
+    const float ex = (xj - xi) * f;
+    const float ey = (yj - yi) * f;
+    xi = xi - ex;
+    yi = yi - ey;
+    xj = xj + ex;
+    yj = yj + ey;
+}
+
+// Cell key of particles_per_cell.wgsl:14-27: floor((x - ax) / cs) per axis,
+// row-major.  The clamp only matters for a position outside the walls, which
+// the step never produces; it keeps every key inside the count array.
+__device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, float y) {
+    float fx = floorf((x - p.ax) / p.cs);
+    float fy = floorf((y - p.ay) / p.cs);
+    fx = fminf(fmaxf(fx, 0.0f), (float)(p.gx - 1));
+    fy = fminf(fmaxf(fy, 0.0f), (float)(p.gy - 1));
+    return (uint32_t)fy * p.gx + (uint32_t)fx;
+}
+
+// Writes one finished particle at canonical index `at`, and claims its slot
+// in the new cell.
+__device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b, uint32_t at,
+                                     float px, float py, float vx, float vy) {
+    b.pos_out[at] = make_float2(px, py);
+    b.vel_out[at] = make_float2(vx, vy);
+    const uint32_t k = cell_key(p, px, py);
+    const uint32_t slot = atomicAdd(&b.cnt[k], 1u);
+    b.kslot[at] = make_uint2(k, slot);
+}
+
+// ---------------------------------------------------------------------------
+// Batcher odd-even merge sort networks, generated at compile time so that
+// every register-array index is a constant after unrolling.
+
+template <int N>
+struct Network {
+    static constexpr int count() {
+        int c = 0;
+        for (int p = 1; p < N; p += p)
+            for (int k = p; k > 0; k /= 2)
+                for (int j = k % p; j + k < N; j += k + k)
+                    for (int i = 0; i < k; ++i)
+                        if (i + j + k < N && (i + j) / (p + p) == (i + j + k) / (p + p)) ++c;
+        return c;
+    }
+    static constexpr int size = count();
+    int lo[size];
+    int hi[size];
+    constexpr Network() : lo(), hi() {
+        int c = 0;
+        for (int p = 1; p < N; p += p)
+            for (int k = p; k > 0; k /= 2)
+                for (int j = k % p; j + k < N; j += k + k)
+                    for (int i = 0; i < k; ++i)
+                        if (i + j + k < N && (i + j) / (p + p) == (i + j + k) / (p + p)) {
+                            lo[c] = i + j;
+                            hi[c] = i + j + k;
+                            ++c;
+                        }
+    }
+};
+static_assert(Network<2>::size == 1 && Network<4>::size == 5 && Network<8>::size == 19 &&
+                  Network<16>::size == 63,
+              "Batcher network sizes");
+
+template <int N>
+__device__ __forceinline__ void sort_by_src(uint32_t (&key)[N], float (&px)[N], float (&py)[N],
+                                            float (&vx)[N], float (&vy)[N]) {
+    constexpr Network<N> net{};
+#pragma unroll
+    for (int q = 0; q < Network<N>::size; ++q) {
+        const int a = net.lo[q], c = net.hi[q];
+        const bool sw = key[a] > key[c];
+        const uint32_t k0 = key[a], k1 = key[c];
+        const float x0 = px[a], x1 = px[c], y0 = py[a], y1 = py[c];
+        const float u0 = vx[a], u1 = vx[c], w0 = vy[a], w1 = vy[c];
+        key[a] = sw ? k1 : k0;  key[c] = sw ? k0 : k1;
+        px[a] = sw ? x1 : x0;   px[c] = sw ? x0 : x1;
+        py[a] = sw ? y1 : y0;   py[c] = sw ? y0 : y1;
+        vx[a] = sw ? u1 : u0;   vx[c] = sw ? u0 : u1;
+        vy[a] = sw ? w1 : w0;   vy[c] = sw ? w0 : w1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cells handled by one lane.
+
+template <bool STEP>
+__device__ __forceinline__ void one_particle(const FrameParams &p, const FrameBuffers &b,
+                                             uint32_t s) {
+    const float2 q = b.pos[s];
+    const float2 v = b.vel[s];
+    if (STEP) {
+        float px = q.x, py = q.y, vx = v.x, vy = v.y;
+        finish_particle(p, px, py, vx, vy);
+        emit(p, b, s, px, py, vx, vy);
+    } else {
+        b.pos_out[s] = q;
+        b.vel_out[s] = v;
+    }
+}
+
+// 2..NB particles: load, sort by source index, collide the first min(n,9)
+// in index order (particles.rs:62-81), finish all (overflow ones without
+// collisions, cell.rs:79-95).
+template <int NB, bool STEP>
+__device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuffers &b,
+                                           uint32_t s, uint32_t n) {
+    uint32_t key[NB];
+    float px[NB], py[NB], vx[NB], vy[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j < (int)n) {
+            const float2 q = b.pos[s + j];
+            const float2 v = b.vel[s + j];
+            px[j] = q.x; py[j] = q.y; vx[j] = v.x; vy[j] = v.y;
+            key[j] = b.src[s + j];
+        } else {
+            px[j] = py[j] = vx[j] = vy[j] = 0.0f;
+            key[j] = 0xffffffffu;
+        }
+    }
+    sort_by_src<NB>(key, px, py, vx, vy);
+    if (STEP) {
+        constexpr int M = NB < kMaxInCell ? NB : kMaxInCell;
+        const int m = (int)(n < (uint32_t)kMaxInCell ? n : (uint32_t)kMaxInCell);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = i + 1; j < M; ++j)
+                if (j < m) collide_pair(px[i], py[i], px[j], py[j]);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < (int)n) {
+                finish_particle(p, px[j], py[j], vx[j], vy[j]);
+                emit(p, b, s + j, px[j], py[j], vx[j], vy[j]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < (int)n) {
+                b.pos_out[s + j] = make_float2(px[j], py[j]);
+                b.vel_out[s + j] = make_float2(vx[j], vy[j]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cells with more than kRegCell particles: the whole block cooperates.
+
+// Ascending bitonic sort of P (a power of two, <= kLdsSort) 64-bit keys in
+// LDS.  Callers synchronise before the call; the last stage ends with a
+// barrier.
+__device__ void bitonic_lds(unsigned long long *sk, uint32_t P) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < P; i += kCollideBlock) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const unsigned long long x = sk[i], y = sk[l];
+                    const bool asc = (i & k) == 0;
+                    if ((x > y) == asc) {
+                        sk[i] = y;
+                        sk[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Merge of two sorted runs of distinct keys into z, by the whole block (merge
+// path: each thread finds its split with a binary search, then merges its
+// share of the output sequentially).
+__device__ void merge_block(const unsigned long long *x, uint32_t nx,
+                            const unsigned long long *y, uint32_t ny,
+                            unsigned long long *z) {
+    const uint32_t tid = threadIdx.x;
+    const uint64_t total = (uint64_t)nx + ny;
+    const uint32_t d0 = (uint32_t)(total * tid / kCollideBlock);
+    const uint32_t d1 = (uint32_t)(total * (tid + 1) / kCollideBlock);
+    uint32_t lo = d0 > ny ? d0 - ny : 0;
+    uint32_t hi = d0 < nx ? d0 : nx;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (x[mid] < y[d0 - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t i = lo, j = d0 - lo;
+    for (uint32_t d = d0; d < d1; ++d) {
+        const bool take_x = j >= ny || (i < nx && x[i] < y[j]);
+        z[d] = take_x ? x[i++] : y[j++];
+    }
+}
+
+// Sorts (src << 32 | local index) for a cell of n > kLdsSort particles:
+// LDS-sorted chunks, then block merges ping-ponging between the cell's own
+// ranges of kslot and sortbuf (both unused by any other cell).  Returns the
+// buffer holding the result.
+__device__ const unsigned long long *sort_large_cell(const FrameBuffers &b, uint32_t s, uint32_t n,
+                                                     unsigned long long *sk) {
+    const uint32_t tid = threadIdx.x;
+    unsigned long long *a = reinterpret_cast<unsigned long long *>(b.kslot) + s;
+    unsigned long long *t = b.sortbuf + s;
+    for (uint32_t base = 0; base < n; base += kLdsSort) {
+        const uint32_t len = min((uint32_t)kLdsSort, n - base);
+        for (uint32_t i = tid; i < (uint32_t)kLdsSort; i += kCollideBlock)
+            sk[i] = i < len ? ((unsigned long long)b.src[s + base + i] << 32) | (base + i) : ~0ull;
+        __syncthreads();
+        bitonic_lds(sk, kLdsSort);
+        for (uint32_t i = tid; i < len; i += kCollideBlock) a[base + i] = sk[i];
+        __syncthreads();
+    }
+    unsigned long long *from = a, *to = t;
+    for (uint64_t w = kLdsSort; w < n; w *= 2) {
+        for (uint64_t lo = 0; lo < n; lo += 2 * w) {
+            const uint64_t mid = lo + w < n ? lo + w : n;
+            const uint64_t hi = lo + 2 * w < n ? lo + 2 * w : n;
+            merge_block(from + lo, (uint32_t)(mid - lo), from + mid, (uint32_t)(hi - mid), to + lo);
+        }
+        __syncthreads();
+        unsigned long long *sw = from;
+        from = to;
+        to = sw;
+    }
+    return from;
+}
+
+template <bool STEP>
+__device__ void big_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s, uint32_t n,
+                         unsigned long long *sk) {
+    const uint32_t tid = threadIdx.x;
+    const unsigned long long *perm;
+    if (n <= (uint32_t)kLdsSort) {
+        uint32_t P = 2;
+        while (P < n) P <<= 1;
+        for (uint32_t i = tid; i < P; i += kCollideBlock)
+            sk[i] = i < n ? ((unsigned long long)b.src[s + i] << 32) | i : ~0ull;
+        __syncthreads();
+        bitonic_lds(sk, P);
+        perm = sk;
+    } else {
+        perm = sort_large_cell(b, s, n, sk);
+    }
+    if (STEP) {
+        // Ranks 0..8 collide in one lane; perm[r] and kslot[s+r] are only
+        // touched by the thread that owns rank r.
+        if (tid == 0) {
+            float px[kMaxInCell], py[kMaxInCell], vx[kMaxInCell], vy[kMaxInCell];
+#pragma unroll
+            for (int q = 0; q < kMaxInCell; ++q) {
+                const uint32_t a = (uint32_t)perm[q];
+                const float2 pq = b.pos[s + a];
+                const float2 vq = b.vel[s + a];
+                px[q] = pq.x; py[q] = pq.y; vx[q] = vq.x; vy[q] = vq.y;
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxInCell; ++i)
+#pragma unroll
+                for (int j = i + 1; j < kMaxInCell; ++j) collide_pair(px[i], py[i], px[j], py[j]);
+#pragma unroll
+            for (int q = 0; q < kMaxInCell; ++q) {
+                finish_particle(p, px[q], py[q], vx[q], vy[q]);
+                emit(p, b, s + q, px[q], py[q], vx[q], vy[q]);
+            }
+        }
+        for (uint32_t r = kMaxInCell + tid; r < n; r += kCollideBlock) {
+            const uint32_t a = (uint32_t)perm[r];
+            const float2 pq = b.pos[s + a];
+            const float2 vq = b.vel[s + a];
+            float px = pq.x, py = pq.y, vx = vq.x, vy = vq.y;
+            finish_particle(p, px, py, vx, vy);
+            emit(p, b, s + r, px, py, vx, vy);
+        }
+    } else {
+        for (uint32_t r = tid; r < n; r += kCollideBlock) {
+            const uint32_t a = (uint32_t)perm[r];
+            b.pos_out[s + r] = b.pos[s + a];
+            b.vel_out[s + r] = b.vel[s + a];
+        }
+    }
+    __syncthreads();
+}
+
+// STEP = true: the collide pass.  STEP = false: canonical copy of the state
+// into pos_out/vel_out (readback), same ordering code, no arithmetic.
+template <bool STEP>
+__global__ __launch_bounds__(kCollideBlock) void cell_kernel(FrameParams p, FrameBuffers b) {
+    __shared__ unsigned long long sk[kLdsSort];
+    __shared__ uint32_t big_s[kCollideBlock];
+    __shared__ uint32_t big_n[kCollideBlock];
+    __shared__ uint32_t nbig;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t c = blockIdx.x * kCollideBlock + tid;
+    if (tid == 0) nbig = 0;
+    __syncthreads();
+    if (c < p.ncell) {
+        const uint32_t s = b.idx[c + 1];
+        uint32_t n = b.idx[c + 2] - s;
+        // Guard for a corrupt offset array: never read past the particles.
+        if (s > p.n) n = 0;
+        else if (n > p.n - s) n = p.n - s;
+        if (n == 1) {
+            one_particle<STEP>(p, b, s);
+        } else if (n == 2) {
+            small_cell<2, STEP>(p, b, s, n);
+        } else if (n <= 4 && n > 0) {
+            small_cell<4, STEP>(p, b, s, n);
+        } else if (n <= 8 && n > 0) {
+            small_cell<8, STEP>(p, b, s, n);
+        } else if (n <= (uint32_t)kRegCell && n > 0) {
+            small_cell<kRegCell, STEP>(p, b, s, n);
+        } else if (n > 0) {
+            const uint32_t q = atomicAdd(&nbig, 1u);
+            big_s[q] = s;
+            big_n[q] = n;
+        }
+    }
+    __syncthreads();
+    const uint32_t nb = nbig;
+    for (uint32_t q = 0; q < nb; ++q) big_cell<STEP>(p, b, big_s[q], big_n[q], sk);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of the counts: one tile of kScanTile cells per block, tile
+// order taken from a monotonic ticket so that every earlier tile has started
+// (forward progress of the look-back), status words tagged with the frame's
+// epoch so they never need clearing.
+
+constexpr unsigned long long kFlagAggregate = 1ull;
+constexpr unsigned long long kFlagInclusive = 2ull;
+constexpr uint32_t kEpochMask = 0x3fffffffu;
+
+__device__ __forceinline__ unsigned long long status_word(unsigned long long flag, uint32_t epoch,
+                                                          uint32_t value) {
+    return (flag << 62) | ((unsigned long long)(epoch & kEpochMask) << 32) | value;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, FrameBuffers b) {
+    __shared__ uint32_t s_tile, s_epoch, s_prefix;
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        const unsigned long long t = atomicAdd(b.ticket, 1ull);
+        s_tile = (uint32_t)(t % b.scan_tiles);
+        s_epoch = (uint32_t)((t / b.scan_tiles) & kEpochMask);
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile, epoch = s_epoch;
+    const uint32_t base = tile * kScanTile + tid * kScanItems;
+
+    // Load 16 counts and clear them for the next frame.
+    uint32_t v[kScanItems];
+    if (base + kScanItems <= p.ncell) {
+        uint4 *q = reinterpret_cast<uint4 *>(b.cnt + base);
+#pragma unroll
+        for (int i = 0; i < kScanItems / 4; ++i) {
+            const uint4 w = q[i];
+            v[4 * i] = w.x; v[4 * i + 1] = w.y; v[4 * i + 2] = w.z; v[4 * i + 3] = w.w;
+        }
+#pragma unroll
+        for (int i = 0; i < kScanItems / 4; ++i) q[i] = make_uint4(0, 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            if (base + i < p.ncell) {
+                v[i] = b.cnt[base + i];
+                b.cnt[base + i] = 0;
+            } else {
+                v[i] = 0;
+            }
+        }
+    }
+    uint32_t mine = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) mine += v[i];
+    const uint32_t incl = wave_inclusive_sum(mine);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, agg = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        const uint32_t ws = s_wave[w];
+        if ((uint32_t)w < wave) wave_off += ws;
+        agg += ws;
+    }
+
+    if (wave == 0) {
+        uint32_t excl = 0;
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&b.tile_state[0], status_word(kFlagInclusive, epoch, agg),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&b.tile_state[tile], status_word(kFlagAggregate, epoch, agg),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
+            while (true) {
+                const int64_t jj = j - (int64_t)lane;
+                const unsigned long long w =
+                    jj >= 0 ? __hip_atomic_load(&b.tile_state[jj], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : status_word(kFlagInclusive, epoch, 0);
+                const unsigned long long flag = w >> 62;
+                const bool ok = flag != 0 && (uint32_t)((w >> 32) & kEpochMask) == epoch;
+                const unsigned long long okm = __ballot(ok);
+                const unsigned long long incm = __ballot(ok && flag == kFlagInclusive);
+                const uint32_t val = (uint32_t)w;
+                if (incm != 0) {
+                    const int first = __builtin_ctzll(incm);
+                    const unsigned long long need = first == 63 ? ~0ull : ((1ull << (first + 1)) - 1);
+                    if ((okm & need) == need) {
+                        excl += wave_sum((int)lane <= first ? val : 0u);
+                        break;
+                    }
+                } else if (okm == ~0ull) {
+                    excl += wave_sum(val);
+                    j -= 64;
+                    continue;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (lane == 0)
+                __hip_atomic_store(&b.tile_state[tile],
+                                   status_word(kFlagInclusive, epoch, excl + agg),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_prefix = excl;
+    }
+    __syncthreads();
+
+    // idx[1 + k] = start_k; idx[0] = 0; idx[C + 1] = N.
+    uint32_t run = s_prefix + wave_off + incl - mine;
+    uint32_t out[kScanItems];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        out[i] = run;
+        run += v[i];
+    }
+    uint32_t *dst = b.idx + 1 + base;
+    if (base + kScanItems <= p.ncell) {
+        uint4 *q = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+        for (int i = 0; i < kScanItems / 4; ++i)
+            q[i] = make_uint4(out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i)
+            if (base + i < p.ncell) dst[i] = out[i];
+    }
+    if (tile == 0 && tid == 0) b.idx[0] = 0;
+    if (tile == b.scan_tiles - 1 && tid == 0) b.idx[p.ncell + 1] = s_prefix + agg;
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, FrameBuffers b) {
+    const uint32_t i = blockIdx.x * kScatterBlock + threadIdx.x;
+    if (i >= p.n) return;
+    const uint2 ks = b.kslot[i];
+    const uint32_t dest = b.idx[ks.x + 1] + ks.y;
+    b.pos[dest] = b.pos_out[i];
+    b.vel[dest] = b.vel_out[i];
+    b.src[dest] = i;
+}
+
+__global__ void iota_kernel(uint32_t *dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = i;
+}
+
+}  // namespace
+
+hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
+    const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
+    hipLaunchKernelGGL(cell_kernel<true>, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
+    const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
+    hipLaunchKernelGGL(cell_kernel<false>, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
+    hipLaunchKernelGGL(scan_kernel, dim3(b.scan_tiles), dim3(kScanThreads), 0, stream, p, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
+    const uint32_t grid = (p.n + kScatterBlock - 1) / kScatterBlock;
+    hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kScatterBlock), 0, stream, p, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace wrach
