@@ -30,14 +30,22 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 METRIC = "particle-steps/s & %HBM roofline @1M"
 
 
-def algorithmic_bytes(n: int, cells: int) -> dict:
-    """Bytes each kernel must move per launch.  Per particle: collide reads
-    position+velocity (16) and writes them (16) plus the cell key (4); the
-    scatter reads key (4) + position+velocity (16) and writes them (16).
-    Per cell (C + 2 offsets): collide reads one offset (4); the scan reads a
-    count (4), writes an offset (4) and clears the count (4)."""
+def algorithmic_bytes(n: int, cells: int, n_full: int) -> dict:
+    """Bytes each kernel must move per launch.  Per particle: the collide
+    pass reads position+velocity (16) and writes them (16) plus the cell key
+    (4) -- in collide_kernel for cells of <= 16 particles, in the full-cell
+    kernel for the n_full particles of fuller cells; the scatter reads key
+    (4) + position+velocity (16) and writes them (16).  Per cell (C + 2
+    offsets): collide_kernel reads one offset (4); the scan reads a count
+    (4), writes an offset (4) and clears the count (4)."""
     c2 = cells + 2
-    return {"collide": 36 * n + 4 * c2, "scan": 12 * c2, "scatter": 36 * n}
+    return {"collide": 36 * (n - n_full) + 4 * c2, "full_cells": 36 * n_full,
+            "scan": 12 * c2, "scatter": 36 * n}
+
+
+def particles_in_full_cells(idx: np.ndarray) -> int:
+    cnt = np.diff(idx[1:].astype(np.int64))
+    return int(cnt[cnt > 16].sum())
 
 
 def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
@@ -94,9 +102,11 @@ def main() -> int:
     print(f"[bench] timed {args.steps} frames in {elapsed * 1e3:.2f} ms", file=sys.stderr, flush=True)
 
     # Per-kernel durations: HIP events around each launch, plain launches.
+    full_before = particles_in_full_cells(world.read()[0])
     ms = world.step_profiled(args.profile_steps)
+    full_after = particles_in_full_cells(world.read()[0])
     avg_us = {k: 1e3 * v / args.profile_steps for k, v in ms.items()}
-    bytes_ = algorithmic_bytes(n, cells)
+    bytes_ = algorithmic_bytes(n, cells, (full_before + full_after) // 2)
     dom = max(avg_us, key=avg_us.get)
     achieved = bytes_[dom] / (avg_us[dom] * 1e-6) / 1e9
     step_bytes = sum(bytes_.values())
@@ -148,6 +158,7 @@ def main() -> int:
             "traffic": traffic,
             "algorithmic_bytes_per_launch": bytes_[dom],
             "kernel_avg_us": avg_us,
+            "particles_in_full_cells": [full_before, full_after],
             "step_algorithmic_bytes": step_bytes,
             "step_achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
             "step_frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,

@@ -80,8 +80,9 @@ typedef struct wrach_device_view {
 } wrach_device_view;
 
 /* Number of kernels timed by wrach_step_profiled, in launch order:
- * 0 collide/integrate/key, 1 exclusive scan, 2 stable scatter. */
-#define WRACH_NUM_FRAME_KERNELS 3
+ * 0 collide/integrate/key (cells of <= 16 particles), 1 full cells,
+ * 2 exclusive scan, 3 stable scatter. */
+#define WRACH_NUM_FRAME_KERNELS 4
 
 /* WrachConfig::default (config_app.rs:26-35): 480x352, cell size 3
  * (SPATIAL_BIN_CELL_SIZE, shaders/shared/src/lib.rs:34). */

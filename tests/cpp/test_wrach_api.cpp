@@ -1,0 +1,84 @@
+// The reference's API-level tests, written against the C++ host mirror
+// (include/wrach_api.hpp).  Built and run by tests/test_cpp_api.py.
+#include <cstdio>
+#include <vector>
+
+#include "wrach_api.hpp"
+
+static int failures = 0;
+#define CHECK(cond)                                                   \
+    do {                                                              \
+        if (!(cond)) {                                                \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+            ++failures;                                               \
+        }                                                             \
+    } while (0)
+
+// runners/api/src/lib.rs:101-126 test_api_returns_data
+static void api_returns_data() {
+    wrach::WrachConfig cfg;
+    cfg.dimensions = {10, 10};
+    cfg.cell_size = 3;
+    wrach::WrachAPI wrach(cfg);
+    std::vector<wrach::Particle> particles(3, wrach::Particle{{5.0f, 5.0f}, {0.5f, 0.5f}});
+    wrach.add_particles(particles);
+    for (int i = 0; i < 5; ++i) wrach.tick();
+    CHECK(wrach.positions.size() == 164);
+    CHECK(wrach.positions[0] != std::make_pair(0.0f, 0.0f));
+    CHECK(wrach.velocities.size() == 164);
+    CHECK(wrach.velocities[0] != std::make_pair(0.0f, 0.0f));
+}
+
+// runners/bevy/src/compute/03_prefix_sum.rs:150-207 prefix_sum_for_small_arrays
+static void prefix_sum_for_small_arrays() {
+    wrach::WrachConfig cfg;
+    cfg.dimensions = {10, 10};
+    cfg.cell_size = 5;
+    wrach::WrachAPI wrach(cfg);
+    wrach.add_particles({{{0.1f, 0.1f}, {0, 0}}, {{2.0f, 2.0f}, {0, 0}},
+                         {{5.0f, 5.0f}, {0, 0}}, {{10.0f, 10.0f}, {0, 0}}});
+    for (int i = 0; i < 4; ++i) wrach.tick();
+    const std::vector<uint32_t> want = {0, 0, 2, 2, 2, 2, 3, 3, 3, 3, 4};
+    CHECK(wrach.get_simulation_state().packed_data.indices == want);
+}
+
+// runners/bevy/src/compute/04_pack_particle_data.rs:72-139 packed_data
+static void packed_data() {
+    wrach::WrachConfig cfg;
+    cfg.dimensions = {10, 10};
+    cfg.cell_size = 3;
+    wrach::WrachAPI wrach(cfg);
+    wrach.add_particles({{{0.1f, 0.1f}, {0, 0}}, {{2.5f, 2.5f}, {0, 0}},
+                         {{5.0f, 5.0f}, {0, 0}}, {{10.0f, 10.0f}, {0, 0}}});
+    for (int i = 0; i < 4; ++i) wrach.tick();
+    const auto &pos = wrach.get_simulation_state().packed_data.positions;
+    CHECK((pos[0] == wrach::Vec2{0.1f, 0.1f}));
+    CHECK((pos[1] == wrach::Vec2{2.5f, 2.5f}));
+    CHECK((pos[2] == wrach::Vec2{5.0f, 5.0f}));
+    CHECK((pos[3] == wrach::Vec2{10.0f, 10.0f}));
+}
+
+// Where the reference would panic on a full buffer, the mirror throws.
+static void capacity_error() {
+    wrach::WrachConfig cfg;
+    cfg.dimensions = {6, 6};
+    cfg.cell_size = 2;
+    wrach::WrachAPI wrach(cfg);
+    bool threw = false;
+    try {
+        wrach.add_particles(std::vector<wrach::Particle>(75, wrach::Particle{{1, 1}, {0, 0}}));
+    } catch (const wrach::Error &e) {
+        threw = e.code() == WRACH_E_CAPACITY;
+    }
+    CHECK(threw);
+    CHECK(wrach.get_simulation_state().shader_settings.particles_in_frame_count == 0);
+}
+
+int main() {
+    api_returns_data();
+    prefix_sum_for_small_arrays();
+    packed_data();
+    capacity_error();
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);
+    return failures ? 1 : 0;
+}

@@ -146,11 +146,13 @@ def test_full_cells(oracle, per_cell):
     for cx, cy in cells:
         pos.append(r.uniform([cx, cy], [cx + 1, cy + 1], (per_cell, 2)))
         vel.append(r.uniform(-1.2, 1.2, (per_cell, 2)))
-    extra = r.uniform(0, [24, 12], (500, 2))
+    # cell size 1 gives ~1.1 particles of capacity per cell
+    # (particle_store.rs:116-133), so the world is sized for 4 x 5000 + 500.
+    extra = r.uniform(0, [160, 120], (500, 2))
     pos = f32(np.concatenate(pos + [extra]))
     vel = f32(np.concatenate(vel + [r.uniform(-1, 1, (500, 2))]))
     perm = r.permutation(len(pos))  # insertion order differs from cell order
-    cfg = WrachConfig(dimensions=(24, 12), cell_size=1, gravity=(0.0, -0.1))
+    cfg = WrachConfig(dimensions=(160, 120), cell_size=1, gravity=(0.0, -0.1))
     steps = 6 if per_cell >= 2048 else 12
     run_parity(oracle, cfg, pos[perm], vel[perm], steps, label=f"full cells x{per_cell}")
 
@@ -160,7 +162,7 @@ def test_particles_outside_grid_are_dropped(oracle):
     vel = f32(np.full((6, 2), 0.25))
     cfg = WrachConfig(dimensions=(10, 10), cell_size=1)
     ref = run_parity(oracle, cfg, pos, vel, 4, label="off-grid")
-    assert len(ref[1]) == 3
+    assert len(ref[1]) == 4  # x = 10 and y = 10.5 still bin into the 11x11 grid
 
 
 def test_velocity_and_wall_extremes(oracle):
@@ -227,7 +229,8 @@ def test_profiled_step_matches_and_times(oracle):
     with World(wl.config) as w:
         w.add_particles(wl.pos, wl.vel)
         ms = w.step_profiled(5)
-        assert set(ms) == {"collide", "scan", "scatter"} and all(v > 0 for v in ms.values())
+        assert set(ms) == {"collide", "full_cells", "scan", "scatter"}
+        assert all(v > 0 for v in ms.values())
         assert_same_state(w.read(), oracle.step(P, *ref, nsteps=5), "profiled")
 
 

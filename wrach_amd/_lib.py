@@ -19,8 +19,8 @@ WRACH_E_NONFINITE = -4
 WRACH_E_DEVICE = -5
 WRACH_E_BUFFER = -6
 WRACH_E_STATE = -7
-NUM_FRAME_KERNELS = 3
-FRAME_KERNEL_NAMES = ("collide", "scan", "scatter")
+NUM_FRAME_KERNELS = 4
+FRAME_KERNEL_NAMES = ("collide", "full_cells", "scan", "scatter")
 
 # Every symbol include/wrach.h declares.
 EXPORTS = (

@@ -100,6 +100,7 @@ void free_graphs(wrach_ctx *c) {
 
 int enqueue_frame(wrach_ctx *c) {
     WRACH_HIP(wrach::launch_collide(c->fp, c->buf, c->stream));
+    WRACH_HIP(wrach::launch_full_cells(c->fp, c->buf, c->stream));
     WRACH_HIP(wrach::launch_scan(c->fp, c->buf, c->stream));
     WRACH_HIP(wrach::launch_scatter(c->fp, c->buf, c->stream));
     return WRACH_OK;
@@ -136,7 +137,10 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.sortbuf);
     (void)hipFree(c->buf.cnt);
     (void)hipFree(c->buf.tile_state);
-    (void)hipFree(c->buf.ticket);
+    (void)hipFree(c->buf.epoch);
+    (void)hipFree(c->buf.full_wave);
+    (void)hipFree(c->buf.full_block);
+    (void)hipFree(c->buf.full_count);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -222,6 +226,9 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
         return fail(WRACH_E_TOO_MANY_CELLS, "grid has more cells than the 32-bit offsets allow");
     const uint32_t cap = max_particles_per_frame(ncell, cfg->cell_size);
     if (cap == 0) return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity does not fit in u32");
+    // Bit 31 of a particle's source index marks the start of a run.
+    if (cap > 0x7fffffffu)
+        return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity exceeds 2^31 - 1");
 
     wrach_ctx *c = new wrach_ctx();
     c->cfg = *cfg;
@@ -252,6 +259,13 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
 
     wrach::FrameBuffers &b = c->buf;
     b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell);
+    // A cell goes on a full-cell list when it holds more than kRegCell
+    // particles, so each list needs at most capacity / (kRegCell + 1) entries.
+    b.full_cap = cap / (wrach::kRegCell + 1) + 64;
+    {
+        const uint32_t want = (b.full_cap + wrach::kWavesPerBlock - 1) / wrach::kWavesPerBlock;
+        b.full_grid = want < wrach::kFullGridMax ? want : wrach::kFullGridMax;
+    }
     auto bail = [&](int rc) { release(c); return rc; };
 #define WRACH_HIP_C(expr)                                                                   \
     do {                                                                                    \
@@ -272,7 +286,10 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.sortbuf, cap));
     WRACH_HIP_C(dmalloc(&b.cnt, ncell + 16));
     WRACH_HIP_C(dmalloc(&b.tile_state, b.scan_tiles));
-    WRACH_HIP_C(dmalloc(&b.ticket, 1));
+    WRACH_HIP_C(dmalloc(&b.epoch, 1));
+    WRACH_HIP_C(dmalloc(&b.full_wave, b.full_cap));
+    WRACH_HIP_C(dmalloc(&b.full_block, b.full_cap));
+    WRACH_HIP_C(dmalloc(&b.full_count, 2));
     WRACH_HIP_C(hipMemsetAsync(c->idx_alloc, 0, (nidx + 64) * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.pos, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.vel, 0, cap * sizeof(float2), c->stream));
@@ -280,7 +297,12 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     WRACH_HIP_C(hipMemsetAsync(b.vel_out, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.cnt, 0, (ncell + 16) * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.tile_state, 0, b.scan_tiles * sizeof(unsigned long long), c->stream));
-    WRACH_HIP_C(hipMemsetAsync(b.ticket, 0, sizeof(unsigned long long), c->stream));
+    {
+        static const uint32_t first_epoch = 1;  // status words start at 0, never valid
+        WRACH_HIP_C(hipMemcpyAsync(b.epoch, &first_epoch, sizeof first_epoch, hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    WRACH_HIP_C(hipMemsetAsync(b.full_count, 0, 2 * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipStreamSynchronize(c->stream));
 #undef WRACH_HIP_C
     *out = c;
@@ -367,16 +389,15 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
     int rc = WRACH_OK;
     for (uint32_t f = 0; f < nframes && rc == WRACH_OK; ++f) {
         hipEvent_t *fe = &ev[2 * K * (size_t)f];
+        hipError_t (*const launch[K])(const wrach::FrameParams &, const wrach::FrameBuffers &,
+                                      hipStream_t) = {wrach::launch_collide, wrach::launch_full_cells,
+                                                      wrach::launch_scan, wrach::launch_scatter};
         hipError_t e = hipSuccess;
-        e = hipEventRecord(fe[0], c->stream);
-        if (e == hipSuccess) e = wrach::launch_collide(c->fp, c->buf, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(fe[1], c->stream);
-        if (e == hipSuccess) e = hipEventRecord(fe[2], c->stream);
-        if (e == hipSuccess) e = wrach::launch_scan(c->fp, c->buf, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(fe[3], c->stream);
-        if (e == hipSuccess) e = hipEventRecord(fe[4], c->stream);
-        if (e == hipSuccess) e = wrach::launch_scatter(c->fp, c->buf, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(fe[5], c->stream);
+        for (int k = 0; k < K && e == hipSuccess; ++k) {
+            e = hipEventRecord(fe[2 * k], c->stream);
+            if (e == hipSuccess) e = launch[k](c->fp, c->buf, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(fe[2 * k + 1], c->stream);
+        }
         if (e != hipSuccess) rc = fail(WRACH_E_DEVICE, std::string("profiled frame: ") + hipGetErrorString(e));
     }
     hipError_t se = hipStreamSynchronize(c->stream);

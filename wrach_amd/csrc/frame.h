@@ -33,18 +33,26 @@ struct FrameBuffers {
     unsigned long long *sortbuf;  // merge-sort ping-pong for very full cells
     uint32_t *cnt;        // per-cell counters, zero between frames
     unsigned long long *tile_state;  // scan look-back words
-    unsigned long long *ticket;      // scan tile ticket (monotonic)
+    uint32_t *epoch;      // frame counter tagging the scan's status words; the scatter advances it
+    uint4 *full_wave;     // cells of kRegCell < n <= kWaveSort particles: (s, n, cell, 0)
+    uint4 *full_block;    // cells of more than kWaveSort particles
+    uint32_t *full_count; // [0] entries in full_wave, [1] in full_block; reset by the scan
+    uint32_t full_cap;    // capacity of each list
     uint32_t scan_tiles;
+    uint32_t full_grid;   // blocks of the full-cell kernel
 };
 
 constexpr int kCollideBlock = 256;
-constexpr int kScanThreads = 256;
+constexpr int kScanThreads = 512;
 constexpr int kScanItems = 16;            // per thread
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kScatterBlock = 256;
 constexpr int kMaxInCell = 9;            // cell.rs:29-30
 constexpr int kRegCell = 16;             // cells up to this size sort in registers
-constexpr int kLdsSort = 2048;           // keys per LDS bitonic sort
+constexpr int kLdsSort = 2048;           // keys per LDS bitonic sort (block path)
+constexpr int kWaveSort = 1024;          // keys per wave-level LDS bitonic sort
+constexpr int kWavesPerBlock = kCollideBlock / 64;
+constexpr uint32_t kFullGridMax = 1024;  // blocks of the full-cell kernel
 
 inline uint32_t scan_tiles_for(uint32_t ncell) {
     return (ncell + kScanTile - 1) / kScanTile;
@@ -53,6 +61,7 @@ inline uint32_t scan_tiles_for(uint32_t ncell) {
 // Launchers (kernels.hip).  Each enqueues on `stream` and returns the launch
 // status.
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+hipError_t launch_full_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);

@@ -1,23 +1,36 @@
 // Frame kernels of the particle step, written for gfx950 (wave64).
 //
-// One frame is three launches:
-//   collide_kernel  one lane per cell: restore the cell's canonical order
-//                   (sort by `src`), separate the first 9 particles
-//                   (Gauss-Seidel, shaders/physics/src/particles.rs:62-93),
-//                   integrate and limit every particle (particle.rs:40-82,
-//                   cell.rs:79-95), write pos_out/vel_out at the canonical
-//                   index, compute the new cell key
-//                   (particles_per_cell.wgsl:14-27) and take a slot in the
-//                   new cell with one atomic add.
-//   scan_kernel     single-pass exclusive scan of the per-cell counts with
-//                   decoupled look-back; writes [0, start_0.., N] and clears
-//                   the counts for the next frame (prefix_sum.wgsl +
-//                   cell.rs:116-131).
-//   scatter_kernel  one lane per particle: dest = start[key] + slot
-//                   (pack_new_particle_data.wgsl:39-44); records the source
-//                   index so the next collide pass can rebuild the stable
-//                   order that the reference's CPU packer defines
-//                   (spatial_bin.rs:122-142).
+// One frame is four launches:
+//   collide_kernel     four cells per lane (one 128-bit offset load): cells
+//                      of <= 8 particles are handled in registers -- restore
+//                      the canonical order (sort by `src`), separate the first
+//                      9 particles (Gauss-Seidel, shaders/physics/src/
+//                      particles.rs:62-93), integrate and limit every particle
+//                      (particle.rs:40-82, cell.rs:79-95), write pos_out/
+//                      vel_out at the canonical index, compute the new cell key
+//                      (particles_per_cell.wgsl:14-27) and claim a slot in the
+//                      new cell with one atomic.  Fuller cells are queued.
+//   full_cells_kernel  one wavefront per queued cell (<= kWaveSort particles),
+//                      one block per larger cell.
+//   scan_kernel        single-pass exclusive scan of the per-cell counts with
+//                      decoupled look-back; writes [0, start_0.., N] and clears
+//                      the counts for the next frame (prefix_sum.wgsl +
+//                      cell.rs:116-131).
+//   scatter_kernel     one lane per particle: dest = start[key] + slot
+//                      (pack_new_particle_data.wgsl:39-44); records the source
+//                      index so the next collide pass can rebuild the stable
+//                      order that the reference's CPU packer defines
+//                      (spatial_bin.rs:122-142).
+//
+// In-cell order.  Slots inside a new cell come from atomics, so the stored
+// order is a permutation of the canonical one (ascending source index).  A
+// source cell that sends several particles in one direction reserves them as
+// one contiguous run with one atomic, in canonical order; the scatter flags
+// the first particle of every run in bit 31 of `src`.  A cell's segment is
+// therefore a handful of ascending runs whose source ranges do not
+// interleave, and ordering the runs by their first source index restores the
+// canonical order in O(n) (a full sort remains the fallback).
+//
 // Floating point: the library is compiled with -ffp-contract=off, so every
 // a*b+c is a rounded multiply then a rounded add, and f32 division and
 // sqrtf are the correctly rounded sequences -- the native Rust semantics of
@@ -26,6 +39,10 @@
 
 namespace wrach {
 namespace {
+
+constexpr uint32_t kRunFlag = 0x80000000u;  // in src and in kslot.y: first of a run
+constexpr uint32_t kSrcMask = 0x7fffffffu;
+constexpr int kCellsPerLane = 4;
 
 // ---------------------------------------------------------------------------
 // Per-particle arithmetic.
@@ -75,26 +92,46 @@ __device__ __forceinline__ void collide_pair(float &xi, float &yi, float &xj, fl
     yj = yj + ey;
 }
 
-// Cell key of particles_per_cell.wgsl:14-27: floor((x - ax) / cs) per axis,
-// row-major.  The clamp only matters for a position outside the walls, which
+// Cell coordinates of particles_per_cell.wgsl:14-27: floor((x - ax) / cs)
+// per axis.  The clamp only matters for a position outside the walls, which
 // the step never produces; it keeps every key inside the count array.
-__device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, float y) {
+__device__ __forceinline__ void cell_xy(const FrameParams &p, float x, float y, uint32_t &kx,
+                                        uint32_t &ky) {
     float fx = floorf((x - p.ax) / p.cs);
     float fy = floorf((y - p.ay) / p.cs);
     fx = fminf(fmaxf(fx, 0.0f), (float)(p.gx - 1));
     fy = fminf(fmaxf(fy, 0.0f), (float)(p.gy - 1));
-    return (uint32_t)fy * p.gx + (uint32_t)fx;
+    kx = (uint32_t)fx;
+    ky = (uint32_t)fy;
 }
 
-// Writes one finished particle at canonical index `at`, and claims its slot
-// in the new cell.
+__device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, float y) {
+    uint32_t kx, ky;
+    cell_xy(p, x, y, kx, ky);
+    return ky * p.gx + kx;
+}
+
+// Writes one finished particle at canonical index `at` and claims a slot of
+// its own (a run of one) in the new cell.
 __device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b, uint32_t at,
                                      float px, float py, float vx, float vy) {
     b.pos_out[at] = make_float2(px, py);
     b.vel_out[at] = make_float2(vx, vy);
     const uint32_t k = cell_key(p, px, py);
     const uint32_t slot = atomicAdd(&b.cnt[k], 1u);
-    b.kslot[at] = make_uint2(k, slot);
+    b.kslot[at] = make_uint2(k, slot | kRunFlag);
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << (threadIdx.x & 63)) - 1;
+}
+
+// Orders LDS accesses of one wavefront: its LDS instructions execute in
+// program order, so a compiler fence is all a wave-private region needs.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ---------------------------------------------------------------------------
@@ -183,7 +220,7 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
             const float2 q = b.pos[s + j];
             const float2 v = b.vel[s + j];
             px[j] = q.x; py[j] = q.y; vx[j] = v.x; vy[j] = v.y;
-            key[j] = b.src[s + j];
+            key[j] = b.src[s + j] & kSrcMask;
         } else {
             px[j] = py[j] = vx[j] = vy[j] = 0.0f;
             key[j] = 0xffffffffu;
@@ -217,7 +254,7 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
 }
 
 // ---------------------------------------------------------------------------
-// Cells with more than kRegCell particles: the whole block cooperates.
+// Cells of more than kWaveSort particles: the whole block cooperates.
 
 // Ascending bitonic sort of P (a power of two, <= kLdsSort) 64-bit keys in
 // LDS.  Callers synchronise before the call; the last stage ends with a
@@ -278,7 +315,8 @@ __device__ const unsigned long long *sort_large_cell(const FrameBuffers &b, uint
     for (uint32_t base = 0; base < n; base += kLdsSort) {
         const uint32_t len = min((uint32_t)kLdsSort, n - base);
         for (uint32_t i = tid; i < (uint32_t)kLdsSort; i += kCollideBlock)
-            sk[i] = i < len ? ((unsigned long long)b.src[s + base + i] << 32) | (base + i) : ~0ull;
+            sk[i] = i < len ? ((unsigned long long)(b.src[s + base + i] & kSrcMask) << 32) | (base + i)
+                            : ~0ull;
         __syncthreads();
         bitonic_lds(sk, kLdsSort);
         for (uint32_t i = tid; i < len; i += kCollideBlock) a[base + i] = sk[i];
@@ -308,7 +346,7 @@ __device__ void big_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s
         uint32_t P = 2;
         while (P < n) P <<= 1;
         for (uint32_t i = tid; i < P; i += kCollideBlock)
-            sk[i] = i < n ? ((unsigned long long)b.src[s + i] << 32) | i : ~0ull;
+            sk[i] = i < n ? ((unsigned long long)(b.src[s + i] & kSrcMask) << 32) | i : ~0ull;
         __syncthreads();
         bitonic_lds(sk, P);
         perm = sk;
@@ -355,10 +393,384 @@ __device__ void big_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s
     __syncthreads();
 }
 
-// STEP = true: the collide pass.  STEP = false: canonical copy of the state
-// into pos_out/vel_out (readback), same ordering code, no arithmetic.
-template <bool STEP>
-__global__ __launch_bounds__(kCollideBlock) void cell_kernel(FrameParams p, FrameBuffers b) {
+// ---------------------------------------------------------------------------
+// The collide pass.
+
+// Per-cell offsets with a guard for a corrupt offset array: never read past
+// the particles.
+__device__ __forceinline__ void clamp_range(const FrameParams &p, uint32_t s, uint32_t &n) {
+    if (s > p.n) n = 0;
+    else if (n > p.n - s) n = p.n - s;
+}
+
+// Four consecutive cells per lane.  Single-particle cells are batched so the
+// lane's atomics are in flight together; cells of 2..8 particles follow in
+// registers; fuller cells are queued for full_cells_kernel with one global
+// atomic per block and list.
+__global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
+    __shared__ uint32_t s_cnt[2], s_base[2];
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t c0 = (blockIdx.x * kCollideBlock + threadIdx.x) * kCellsPerLane;
+    uint32_t o[kCellsPerLane + 1];
+    if (c0 + kCellsPerLane <= p.ncell) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(b.idx + 1 + c0);  // 16-byte aligned
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+        o[4] = b.idx[c0 + 5];
+    } else {
+        // idx has ncell + 2 entries; entry c + 1 exists for every c <= ncell.
+#pragma unroll
+        for (int j = 0; j <= kCellsPerLane; ++j)
+            o[j] = c0 + j <= p.ncell ? b.idx[c0 + 1 + j] : (j ? o[j - 1] : 0u);
+    }
+    uint32_t st[kCellsPerLane], nn[kCellsPerLane];
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        st[j] = o[j];
+        nn[j] = c0 + j < p.ncell ? o[j + 1] - o[j] : 0u;
+        clamp_range(p, st[j], nn[j]);
+    }
+
+    // Single-particle cells.
+    float px[kCellsPerLane], py[kCellsPerLane], vx[kCellsPerLane], vy[kCellsPerLane];
+    uint32_t key[kCellsPerLane], slot[kCellsPerLane];
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        key[j] = 0;
+        slot[j] = 0;
+        px[j] = py[j] = vx[j] = vy[j] = 0.0f;
+        if (nn[j] == 1) {
+            const float2 q = b.pos[st[j]];
+            const float2 w = b.vel[st[j]];
+            px[j] = q.x; py[j] = q.y; vx[j] = w.x; vy[j] = w.y;
+            finish_particle(p, px[j], py[j], vx[j], vy[j]);
+            key[j] = cell_key(p, px[j], py[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j)
+        if (nn[j] == 1) slot[j] = atomicAdd(&b.cnt[key[j]], 1u);
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        if (nn[j] == 1) {
+            b.pos_out[st[j]] = make_float2(px[j], py[j]);
+            b.vel_out[st[j]] = make_float2(vx[j], vy[j]);
+            b.kslot[st[j]] = make_uint2(key[j], slot[j] | kRunFlag);
+        }
+    }
+
+    // Cells of 2..8 particles.
+#pragma unroll 1
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        const uint32_t s = j == 0 ? st[0] : j == 1 ? st[1] : j == 2 ? st[2] : st[3];
+        const uint32_t n = j == 0 ? nn[0] : j == 1 ? nn[1] : j == 2 ? nn[2] : nn[3];
+        if (n == 2) small_cell<2, true>(p, b, s, n);
+        else if (n > 2 && n <= 4) small_cell<4, true>(p, b, s, n);
+        else if (n > 4 && n <= 8) small_cell<8, true>(p, b, s, n);
+    }
+
+    // Fuller cells: LDS offsets inside the block, one global atomic per list.
+    uint32_t want_w = 0, want_b = 0;
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        want_w += nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort;
+        want_b += nn[j] > (uint32_t)kWaveSort;
+    }
+    uint32_t at_w = want_w ? atomicAdd(&s_cnt[0], want_w) : 0u;
+    uint32_t at_b = want_b ? atomicAdd(&s_cnt[1], want_b) : 0u;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const uint32_t k = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = k ? atomicAdd(&b.full_count[threadIdx.x], k) : 0u;
+    }
+    __syncthreads();
+    if (want_w | want_b) {
+#pragma unroll
+        for (int j = 0; j < kCellsPerLane; ++j) {
+            const uint4 ent = make_uint4(st[j], nn[j], c0 + j, 0);
+            if (nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort) {
+                const uint32_t at = s_base[0] + at_w++;
+                if (at < b.full_cap) b.full_wave[at] = ent;
+            } else if (nn[j] > (uint32_t)kWaveSort) {
+                const uint32_t at = s_base[1] + at_b++;
+                if (at < b.full_cap) b.full_block[at] = ent;
+            }
+        }
+    }
+}
+
+// Neighbour direction of a new cell relative to the old one: 0..8 for the
+// 3x3 neighbourhood (row-major from (-1,-1)), 9 for anything farther.
+__device__ __forceinline__ uint32_t direction(const FrameParams &p, float x, float y, uint32_t cx,
+                                              uint32_t cy, uint32_t &key) {
+    uint32_t kx, ky;
+    cell_xy(p, x, y, kx, ky);
+    key = ky * p.gx + kx;
+    const int dx = (int)kx - (int)cx, dy = (int)ky - (int)cy;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1) return 9;
+    return (uint32_t)((dy + 1) * 3 + (dx + 1));
+}
+
+// Wave-private LDS of full_cells_kernel.
+struct WaveLds {
+    unsigned long long sort[kWaveSort];  // sort keys, or (run path) the permutation
+    uint32_t head_key[64];               // first source index of each run
+    uint32_t head_pos[64];               // stored position of each run's start
+    uint32_t run_off[64];                // canonical rank of each run's start
+    float4 fin[kMaxInCell];              // ranks 0..8 after the collisions
+};
+
+// Gauss-Seidel on the nine lowest ranks by lane 0 (particles.rs:62-92);
+// results, finished, go to L.fin.
+__device__ __forceinline__ void collide_nine(const FrameParams &p, const FrameBuffers &b,
+                                             uint32_t s, const uint32_t *perm, float4 *fin) {
+    float px[kMaxInCell], py[kMaxInCell], vx[kMaxInCell], vy[kMaxInCell];
+#pragma unroll
+    for (int q = 0; q < kMaxInCell; ++q) {
+        const uint32_t a = perm[q];
+        const float2 pq = b.pos[s + a];
+        const float2 vq = b.vel[s + a];
+        px[q] = pq.x; py[q] = pq.y; vx[q] = vq.x; vy[q] = vq.y;
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxInCell; ++i)
+#pragma unroll
+        for (int j = i + 1; j < kMaxInCell; ++j) collide_pair(px[i], py[i], px[j], py[j]);
+#pragma unroll
+    for (int q = 0; q < kMaxInCell; ++q) {
+        finish_particle(p, px[q], py[q], vx[q], vy[q]);
+        fin[q] = make_float4(px[q], py[q], vx[q], vy[q]);
+    }
+}
+
+// One cell of 8 < n <= kWaveSort particles, processed by one wavefront.
+//  1. Canonical order: runs (flagged in src) ordered by their first source
+//     index -> perm[rank] = stored position; more than 64 runs fall back to
+//     a bitonic sort of (src, position).
+//  2. Lane 0: Gauss-Seidel on ranks 0..8, overlapped with the first batch of
+//     gathers.
+//  3. Every rank finished and written at its canonical index; particles
+//     counted per neighbour direction.
+//  4. One atomic per occupied direction reserves a contiguous, canonically
+//     ordered run in the new cell; a farther mover takes its own slot.
+// Loops work in batches of four 64-rank chunks so that four gathers per lane
+// are in flight at once.
+constexpr int kBatch = 4;
+
+__device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s, uint32_t n,
+                          uint32_t c, WaveLds &L) {
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long lt = lanemask_lt();
+    const unsigned long long le = lt | (1ull << lane);
+    uint32_t *perm32 = reinterpret_cast<uint32_t *>(L.sort);
+
+    // 1a. Run starts: flagged particles, and position 0.
+    uint32_t runs = 0;
+    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const uint32_t i = b0 + 64 * q + lane;
+            v[q] = i < n ? b.src[s + i] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const uint32_t i = b0 + 64 * q + lane;
+            const bool start = i < n && (i == 0 || (v[q] & kRunFlag));
+            const unsigned long long m = __ballot(start);
+            if (start) {
+                const uint32_t rid = runs + (uint32_t)__popcll(m & lt);
+                if (rid < 64) {
+                    L.head_key[rid] = v[q] & kSrcMask;
+                    L.head_pos[rid] = i;
+                }
+            }
+            runs += (uint32_t)__popcll(m);
+        }
+    }
+    wave_sync();
+    if (runs <= 64) {
+        // 1b. Rank of each run = particles in runs with a smaller first source.
+        uint32_t hk = 0xffffffffu, len = 0;
+        if (lane < runs) {
+            hk = L.head_key[lane];
+            const uint32_t end = lane + 1 < runs ? L.head_pos[lane + 1] : n;
+            len = end - L.head_pos[lane];
+        }
+        uint32_t off = 0;
+        for (uint32_t m = 0; m < runs; ++m) {
+            const uint32_t km = __shfl(hk, (int)m, 64);
+            const uint32_t lm = __shfl(len, (int)m, 64);
+            if (km < hk) off += lm;
+        }
+        if (lane < runs) L.run_off[lane] = off;
+        wave_sync();
+        // 1c. perm[rank] = stored position.
+        uint32_t seen = 0;
+        for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
+            uint32_t v[kBatch];
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const uint32_t i = b0 + 64 * q + lane;
+                v[q] = i < n ? b.src[s + i] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const uint32_t i = b0 + 64 * q + lane;
+                const bool start = i < n && (i == 0 || (v[q] & kRunFlag));
+                const unsigned long long m = __ballot(start);
+                if (i < n) {
+                    const uint32_t rid = seen + (uint32_t)__popcll(m & le) - 1;
+                    perm32[L.run_off[rid] + (i - L.head_pos[rid])] = i;
+                }
+                seen += (uint32_t)__popcll(m);
+            }
+        }
+    } else {
+        uint32_t P = 64;
+        while (P < n) P <<= 1;
+        for (uint32_t i = lane; i < P; i += 64)
+            L.sort[i] = i < n ? ((unsigned long long)(b.src[s + i] & kSrcMask) << 32) | i : ~0ull;
+        wave_sync();
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = lane; i < (P >> 1); i += 64) {
+                    const uint32_t a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                    const uint32_t d = a | j;
+                    const unsigned long long x = L.sort[a], y = L.sort[d];
+                    if ((x > y) == ((a & k) == 0)) {
+                        L.sort[a] = y;
+                        L.sort[d] = x;
+                    }
+                }
+                wave_sync();
+            }
+        }
+        // Compact the low words in place, one chunk of 64 at a time: chunk q
+        // writes bytes [256q, 256q+256), which chunk q or an earlier one has
+        // already read.
+        for (uint32_t base = 0; base < n; base += 64) {
+            const uint32_t i = base + lane;
+            const uint32_t a = i < n ? (uint32_t)L.sort[i] : 0u;
+            wave_sync();
+            if (i < n) perm32[i] = a;
+            wave_sync();
+        }
+    }
+    wave_sync();
+
+    // 2 + 3. Finish and write every rank; count particles per direction.
+    const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+    uint32_t count[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) count[d] = 0;
+    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
+        float2 lp[kBatch], lv[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const uint32_t r = b0 + 64 * q + lane;
+            lp[q] = lv[q] = make_float2(0.0f, 0.0f);
+            if (r >= (uint32_t)kMaxInCell && r < n) {
+                const uint32_t a = perm32[r];
+                lp[q] = b.pos[s + a];
+                lv[q] = b.vel[s + a];
+            }
+        }
+        if (b0 == 0) {
+            if (lane == 0) collide_nine(p, b, s, perm32, L.fin);
+            wave_sync();
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const uint32_t r = b0 + 64 * q + lane;
+            uint32_t dir = 10, key;
+            if (r < n) {
+                float px, py, vx, vy;
+                if (r < (uint32_t)kMaxInCell) {
+                    const float4 f = L.fin[r];
+                    px = f.x; py = f.y; vx = f.z; vy = f.w;
+                } else {
+                    px = lp[q].x; py = lp[q].y; vx = lv[q].x; vy = lv[q].y;
+                    finish_particle(p, px, py, vx, vy);
+                }
+                b.pos_out[s + r] = make_float2(px, py);
+                b.vel_out[s + r] = make_float2(vx, vy);
+                dir = direction(p, px, py, cx, cy, key);
+            }
+#pragma unroll
+            for (int d = 0; d < 9; ++d)
+                count[d] += (uint32_t)__popcll(__ballot(dir == (uint32_t)d));
+        }
+    }
+
+    // 4. One atomic per occupied direction (lanes 0..8 together), then the
+    //    same lanes re-read their own ranks and write the slots.
+    uint32_t mine = 0;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) mine = lane == (uint32_t)d ? count[d] : mine;
+    uint32_t got = 0;
+    if (lane < 9 && mine > 0) {
+        const uint32_t tx = cx + lane % 3 - 1, ty = cy + lane / 3 - 1;
+        got = atomicAdd(&b.cnt[ty * p.gx + tx], mine);
+    }
+    uint32_t first[9], done[9];
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        first[d] = __shfl(got, d, 64);
+        done[d] = 0;
+    }
+    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
+        float2 q2[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const uint32_t r = b0 + 64 * q + lane;
+            q2[q] = r < n ? b.pos_out[s + r] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const uint32_t r = b0 + 64 * q + lane;
+            uint32_t dir = 10, key = 0, slot = 0;
+            if (r < n) dir = direction(p, q2[q].x, q2[q].y, cx, cy, key);
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                const unsigned long long m = __ballot(dir == (uint32_t)d);
+                if (dir == (uint32_t)d) {
+                    const uint32_t at = done[d] + (uint32_t)__popcll(m & lt);
+                    slot = (first[d] + at) | (at == 0 ? kRunFlag : 0u);
+                }
+                done[d] += (uint32_t)__popcll(m);
+            }
+            if (dir == 9) slot = atomicAdd(&b.cnt[key], 1u) | kRunFlag;
+            if (r < n) b.kslot[s + r] = make_uint2(key, slot);
+        }
+    }
+}
+
+// Cells queued by collide_kernel.  Each wavefront takes cells of the wave
+// list; then each block takes cells of the block list (over kWaveSort
+// particles: block-wide LDS sort, or merge sort past kLdsSort).
+__global__ __launch_bounds__(kCollideBlock) void full_cells_kernel(FrameParams p, FrameBuffers b) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    static_assert(sizeof(WaveLds) * kWavesPerBlock >= kLdsSort * sizeof(unsigned long long),
+                  "the block path reuses the wave regions");
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nw = min(b.full_count[0], b.full_cap);
+    for (uint32_t e = blockIdx.x * kWavesPerBlock + wave; e < nw; e += gridDim.x * kWavesPerBlock) {
+        const uint4 cell = b.full_wave[e];
+        wave_cell(p, b, cell.x, cell.y, cell.z, lds[wave]);
+    }
+    const uint32_t nb = min(b.full_count[1], b.full_cap);
+    if (nb == 0) return;
+    __syncthreads();
+    for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
+        const uint4 cell = b.full_block[e];
+        big_cell<true>(p, b, cell.x, cell.y, reinterpret_cast<unsigned long long *>(lds));
+    }
+}
+
+// Canonical copy of the state into pos_out/vel_out (readback): the same
+// ordering code as the step, no arithmetic.
+__global__ __launch_bounds__(kCollideBlock) void canonical_kernel(FrameParams p, FrameBuffers b) {
     __shared__ unsigned long long sk[kLdsSort];
     __shared__ uint32_t big_s[kCollideBlock];
     __shared__ uint32_t big_n[kCollideBlock];
@@ -370,20 +782,18 @@ __global__ __launch_bounds__(kCollideBlock) void cell_kernel(FrameParams p, Fram
     if (c < p.ncell) {
         const uint32_t s = b.idx[c + 1];
         uint32_t n = b.idx[c + 2] - s;
-        // Guard for a corrupt offset array: never read past the particles.
-        if (s > p.n) n = 0;
-        else if (n > p.n - s) n = p.n - s;
+        clamp_range(p, s, n);
         if (n == 1) {
-            one_particle<STEP>(p, b, s);
+            one_particle<false>(p, b, s);
         } else if (n == 2) {
-            small_cell<2, STEP>(p, b, s, n);
-        } else if (n <= 4 && n > 0) {
-            small_cell<4, STEP>(p, b, s, n);
-        } else if (n <= 8 && n > 0) {
-            small_cell<8, STEP>(p, b, s, n);
-        } else if (n <= (uint32_t)kRegCell && n > 0) {
-            small_cell<kRegCell, STEP>(p, b, s, n);
-        } else if (n > 0) {
+            small_cell<2, false>(p, b, s, n);
+        } else if (n > 2 && n <= 4) {
+            small_cell<4, false>(p, b, s, n);
+        } else if (n > 4 && n <= 8) {
+            small_cell<8, false>(p, b, s, n);
+        } else if (n > 8 && n <= (uint32_t)kRegCell) {
+            small_cell<kRegCell, false>(p, b, s, n);
+        } else if (n > (uint32_t)kRegCell) {
             const uint32_t q = atomicAdd(&nbig, 1u);
             big_s[q] = s;
             big_n[q] = n;
@@ -391,18 +801,17 @@ __global__ __launch_bounds__(kCollideBlock) void cell_kernel(FrameParams p, Fram
     }
     __syncthreads();
     const uint32_t nb = nbig;
-    for (uint32_t q = 0; q < nb; ++q) big_cell<STEP>(p, b, big_s[q], big_n[q], sk);
+    for (uint32_t q = 0; q < nb; ++q) big_cell<false>(p, b, big_s[q], big_n[q], sk);
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of the counts: one tile of kScanTile cells per block, tile
-// order taken from a monotonic ticket so that every earlier tile has started
-// (forward progress of the look-back), status words tagged with the frame's
-// epoch so they never need clearing.
+// Exclusive scan of the counts: one tile of kScanTile cells per block; status
+// words are tagged with the frame's epoch so they never need clearing.
 
 constexpr unsigned long long kFlagAggregate = 1ull;
 constexpr unsigned long long kFlagInclusive = 2ull;
 constexpr uint32_t kEpochMask = 0x3fffffffu;
+constexpr int kLookWindows = 4;  // 64-tile windows read per look-back round
 
 __device__ __forceinline__ unsigned long long status_word(unsigned long long flag, uint32_t epoch,
                                                           uint32_t value) {
@@ -425,18 +834,62 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Exclusive prefix of `tile` from its predecessors' status words, by one
+// wavefront.  Lane l of window w reads tile j - 64w - l.
+__device__ uint32_t look_back(const FrameBuffers &b, uint32_t tile, uint32_t epoch) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t excl = 0;
+    int64_t j = (int64_t)tile - 1;
+    while (true) {
+        unsigned long long w[kLookWindows];
+#pragma unroll
+        for (int q = 0; q < kLookWindows; ++q) {
+            const int64_t jj = j - 64 * q - (int64_t)lane;
+            w[q] = jj >= 0 ? __hip_atomic_load(&b.tile_state[jj], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : status_word(kFlagInclusive, epoch, 0);
+        }
+        bool finished = false, stalled = false;
+#pragma unroll
+        for (int q = 0; q < kLookWindows; ++q) {
+            if (!finished && !stalled) {
+                const unsigned long long flag = w[q] >> 62;
+                const bool ok = flag != 0 && (uint32_t)((w[q] >> 32) & kEpochMask) == epoch;
+                const unsigned long long okm = __ballot(ok);
+                const unsigned long long incm = __ballot(ok && flag == kFlagInclusive);
+                const uint32_t val = (uint32_t)w[q];
+                if (incm != 0) {
+                    const int first = __builtin_ctzll(incm);
+                    const unsigned long long need =
+                        first == 63 ? ~0ull : ((1ull << (first + 1)) - 1);
+                    if ((okm & need) == need) {
+                        excl += wave_sum((int)lane <= first ? val : 0u);
+                        finished = true;
+                    } else {
+                        stalled = true;
+                    }
+                } else if (okm == ~0ull) {
+                    excl += wave_sum(val);
+                    j -= 64;
+                } else {
+                    stalled = true;
+                }
+            }
+        }
+        if (finished) return excl;
+        if (stalled) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ uint32_t s_tile, s_epoch, s_prefix;
+    __shared__ uint32_t s_prefix;
     __shared__ uint32_t s_wave[kScanThreads / 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) {
-        const unsigned long long t = atomicAdd(b.ticket, 1ull);
-        s_tile = (uint32_t)(t % b.scan_tiles);
-        s_epoch = (uint32_t)((t / b.scan_tiles) & kEpochMask);
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile, epoch = s_epoch;
+    // Tiles follow block order: a block waits only on lower-numbered blocks,
+    // which were dispatched before it.
+    const uint32_t tile = blockIdx.x;
+    const uint32_t epoch = *b.epoch & kEpochMask;
     const uint32_t base = tile * kScanTile + tid * kScanItems;
 
     // Load 16 counts and clear them for the next frame.
@@ -485,32 +938,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
             if (lane == 0)
                 __hip_atomic_store(&b.tile_state[tile], status_word(kFlagAggregate, epoch, agg),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t j = (int64_t)tile - 1;
-            while (true) {
-                const int64_t jj = j - (int64_t)lane;
-                const unsigned long long w =
-                    jj >= 0 ? __hip_atomic_load(&b.tile_state[jj], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT)
-                            : status_word(kFlagInclusive, epoch, 0);
-                const unsigned long long flag = w >> 62;
-                const bool ok = flag != 0 && (uint32_t)((w >> 32) & kEpochMask) == epoch;
-                const unsigned long long okm = __ballot(ok);
-                const unsigned long long incm = __ballot(ok && flag == kFlagInclusive);
-                const uint32_t val = (uint32_t)w;
-                if (incm != 0) {
-                    const int first = __builtin_ctzll(incm);
-                    const unsigned long long need = first == 63 ? ~0ull : ((1ull << (first + 1)) - 1);
-                    if ((okm & need) == need) {
-                        excl += wave_sum((int)lane <= first ? val : 0u);
-                        break;
-                    }
-                } else if (okm == ~0ull) {
-                    excl += wave_sum(val);
-                    j -= 64;
-                    continue;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
+            excl = look_back(b, tile, epoch);
             if (lane == 0)
                 __hip_atomic_store(&b.tile_state[tile],
                                    status_word(kFlagInclusive, epoch, excl + agg),
@@ -539,7 +967,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
         for (int i = 0; i < kScanItems; ++i)
             if (base + i < p.ncell) dst[i] = out[i];
     }
-    if (tile == 0 && tid == 0) b.idx[0] = 0;
+    if (tile == 0 && tid == 0) {
+        b.idx[0] = 0;
+        b.full_count[0] = 0;  // the full-cell kernel of this frame has finished
+        b.full_count[1] = 0;
+    }
     if (tile == b.scan_tiles - 1 && tid == 0) b.idx[p.ncell + 1] = s_prefix + agg;
 }
 
@@ -547,12 +979,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
 
 __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, FrameBuffers b) {
     const uint32_t i = blockIdx.x * kScatterBlock + threadIdx.x;
+    if (i == 0) *b.epoch = *b.epoch + 1;  // the frame's scan has finished
     if (i >= p.n) return;
     const uint2 ks = b.kslot[i];
-    const uint32_t dest = b.idx[ks.x + 1] + ks.y;
+    const uint32_t dest = b.idx[ks.x + 1] + (ks.y & kSrcMask);
     b.pos[dest] = b.pos_out[i];
     b.vel[dest] = b.vel_out[i];
-    b.src[dest] = i;
+    b.src[dest] = i | (ks.y & kRunFlag);
 }
 
 __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
@@ -563,14 +996,20 @@ __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
 }  // namespace
 
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
-    hipLaunchKernelGGL(cell_kernel<true>, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
+    const uint32_t per_block = kCollideBlock * kCellsPerLane;
+    const uint32_t grid = (p.ncell + per_block - 1) / per_block;
+    hipLaunchKernelGGL(collide_kernel, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_full_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
+    hipLaunchKernelGGL(full_cells_kernel, dim3(b.full_grid), dim3(kCollideBlock), 0, stream, p, b);
     return hipGetLastError();
 }
 
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
     const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
-    hipLaunchKernelGGL(cell_kernel<false>, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
+    hipLaunchKernelGGL(canonical_kernel, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
     return hipGetLastError();
 }
 
