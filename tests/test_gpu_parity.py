@@ -172,6 +172,21 @@ def test_velocity_and_wall_extremes(oracle):
     run_parity(oracle, cfg, pos, vel, 10, label="extremes")
 
 
+@pytest.mark.parametrize("per_cell", [5, 20])
+def test_far_movers_across_rows(oracle, per_cell):
+    """Particles leaving a cell at column 0 for the last column one row down
+    (key - cell == -1, the same difference as the left neighbour), mixed
+    with ordinary neighbours, from a register-path cell (5 particles) and a
+    wave-path cell (20)."""
+    r = np.random.default_rng(77 + per_cell)
+    W, H = 40, 20
+    pos = f32(r.uniform([0.0, 5.0], [1.0, 6.0], (per_cell, 2)))
+    vel = f32(r.uniform(-1, 1, (per_cell, 2)))
+    vel[::3] = (W + 5.0, -1.0)
+    cfg = WrachConfig(dimensions=(W, H), cell_size=1)
+    run_parity(oracle, cfg, pos, vel, 6, label=f"far movers x{per_cell}")
+
+
 def test_errors_leave_state_usable(oracle):
     say("[edge] error paths")
     cfg = WrachConfig(dimensions=(6, 6), cell_size=2)  # capacity 74

@@ -1,15 +1,16 @@
 // Frame kernels of the particle step, written for gfx950 (wave64).
 //
 // One frame is four launches:
-//   collide_kernel     four cells per lane (one 128-bit offset load): cells
-//                      of <= 8 particles are handled in registers -- restore
+//   collide_kernel     one cell per lane: cells of <= 8 particles are
+//                      handled in registers -- restore
 //                      the canonical order (sort by `src`), separate the first
 //                      9 particles (Gauss-Seidel, shaders/physics/src/
 //                      particles.rs:62-93), integrate and limit every particle
 //                      (particle.rs:40-82, cell.rs:79-95), write pos_out/
 //                      vel_out at the canonical index, compute the new cell key
-//                      (particles_per_cell.wgsl:14-27) and claim a slot in the
-//                      new cell with one atomic.  Fuller cells are queued.
+//                      (particles_per_cell.wgsl:14-27) and claim slots in the
+//                      new cells with one atomic per neighbour direction.
+//                      Fuller cells are queued.
 //   full_cells_kernel  one wavefront per queued cell (<= kWaveSort particles),
 //                      one block per larger cell.
 //   scan_kernel        single-pass exclusive scan of the per-cell counts with
@@ -42,7 +43,6 @@ namespace {
 
 constexpr uint32_t kRunFlag = 0x80000000u;  // in src and in kslot.y: first of a run
 constexpr uint32_t kSrcMask = 0x7fffffffu;
-constexpr int kCellsPerLane = 4;
 
 // ---------------------------------------------------------------------------
 // Per-particle arithmetic.
@@ -120,6 +120,18 @@ __device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b
     const uint32_t k = cell_key(p, px, py);
     const uint32_t slot = atomicAdd(&b.cnt[k], 1u);
     b.kslot[at] = make_uint2(k, slot | kRunFlag);
+}
+
+// Neighbour direction of a new cell relative to the old one: 0..8 for the
+// 3x3 neighbourhood (row-major from (-1,-1)), 9 for anything farther.
+__device__ __forceinline__ uint32_t direction(const FrameParams &p, float x, float y, uint32_t cx,
+                                              uint32_t cy, uint32_t &key) {
+    uint32_t kx, ky;
+    cell_xy(p, x, y, kx, ky);
+    key = ky * p.gx + kx;
+    const int dx = (int)kx - (int)cx, dy = (int)ky - (int)cy;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1) return 9;
+    return (uint32_t)((dy + 1) * 3 + (dx + 1));
 }
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
@@ -211,7 +223,7 @@ __device__ __forceinline__ void one_particle(const FrameParams &p, const FrameBu
 // collisions, cell.rs:79-95).
 template <int NB, bool STEP>
 __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuffers &b,
-                                           uint32_t s, uint32_t n) {
+                                           uint32_t c, uint32_t s, uint32_t n) {
     uint32_t key[NB];
     float px[NB], py[NB], vx[NB], vy[NB];
 #pragma unroll
@@ -235,11 +247,49 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
 #pragma unroll
             for (int j = i + 1; j < M; ++j)
                 if (j < m) collide_pair(px[i], py[i], px[j], py[j]);
+        // Finish every particle; direction of its new cell from this one.
+        const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+        uint32_t dir[NB], nkey[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            dir[j] = 10;
+            nkey[j] = 0;
+            if (j < (int)n) {
+                finish_particle(p, px[j], py[j], vx[j], vy[j]);
+                dir[j] = direction(p, px[j], py[j], cx, cy, nkey[j]);
+            }
+        }
+        // One run per occupied direction: count, then all atomics at once.
+        uint32_t count[9], first[9], far[NB];
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            count[d] = 0;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) count[d] += dir[j] == (uint32_t)d;
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            first[d] = 0;
+            if (count[d]) {
+                const uint32_t tx = cx + d % 3 - 1, ty = cy + d / 3 - 1;
+                first[d] = atomicAdd(&b.cnt[ty * p.gx + tx], count[d]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) far[j] = dir[j] == 9 ? atomicAdd(&b.cnt[nkey[j]], 1u) : 0u;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             if (j < (int)n) {
-                finish_particle(p, px[j], py[j], vx[j], vy[j]);
-                emit(p, b, s + j, px[j], py[j], vx[j], vy[j]);
+                uint32_t rank = 0, base = 0;
+#pragma unroll
+                for (int i = 0; i < j; ++i) rank += dir[i] == dir[j];
+#pragma unroll
+                for (int d = 0; d < 9; ++d) base = dir[j] == (uint32_t)d ? first[d] : base;
+                const uint32_t slot = dir[j] == 9 ? (far[j] | kRunFlag)
+                                                  : ((base + rank) | (rank == 0 ? kRunFlag : 0u));
+                b.pos_out[s + j] = make_float2(px[j], py[j]);
+                b.vel_out[s + j] = make_float2(vx[j], vy[j]);
+                b.kslot[s + j] = make_uint2(nkey[j], slot);
             }
         }
     } else {
@@ -403,121 +453,60 @@ __device__ __forceinline__ void clamp_range(const FrameParams &p, uint32_t s, ui
     else if (n > p.n - s) n = p.n - s;
 }
 
-// Four consecutive cells per lane.  Single-particle cells are batched so the
-// lane's atomics are in flight together; cells of 2..8 particles follow in
-// registers; fuller cells are queued for full_cells_kernel with one global
-// atomic per block and list.
+// One cell per lane, so no lane waits on more than one cell's round trips:
+// cells of 1..8 particles are handled in registers, with every slot atomic
+// of the cell in flight at once; fuller cells are queued for
+// full_cells_kernel with one global atomic per block and list.
 __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
     __shared__ uint32_t s_cnt[2], s_base[2];
     if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t c0 = (blockIdx.x * kCollideBlock + threadIdx.x) * kCellsPerLane;
-    uint32_t o[kCellsPerLane + 1];
-    if (c0 + kCellsPerLane <= p.ncell) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(b.idx + 1 + c0);  // 16-byte aligned
-        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-        o[4] = b.idx[c0 + 5];
-    } else {
-        // idx has ncell + 2 entries; entry c + 1 exists for every c <= ncell.
-#pragma unroll
-        for (int j = 0; j <= kCellsPerLane; ++j)
-            o[j] = c0 + j <= p.ncell ? b.idx[c0 + 1 + j] : (j ? o[j - 1] : 0u);
+    const uint32_t c = blockIdx.x * kCollideBlock + threadIdx.x;
+    uint32_t s = 0, n = 0;
+    if (c < p.ncell) {
+        s = b.idx[c + 1];
+        n = b.idx[c + 2] - s;
+        clamp_range(p, s, n);
     }
-    uint32_t st[kCellsPerLane], nn[kCellsPerLane];
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        st[j] = o[j];
-        nn[j] = c0 + j < p.ncell ? o[j + 1] - o[j] : 0u;
-        clamp_range(p, st[j], nn[j]);
+    if (n == 1) {
+        one_particle<true>(p, b, s);
+    } else if (n == 2) {
+        small_cell<2, true>(p, b, c, s, n);
+    } else if (n > 2 && n <= 4) {
+        small_cell<4, true>(p, b, c, s, n);
+    } else if (n > 4 && n <= 8) {
+        small_cell<8, true>(p, b, c, s, n);
     }
-
-    // Single-particle cells.
-    float px[kCellsPerLane], py[kCellsPerLane], vx[kCellsPerLane], vy[kCellsPerLane];
-    uint32_t key[kCellsPerLane], slot[kCellsPerLane];
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        key[j] = 0;
-        slot[j] = 0;
-        px[j] = py[j] = vx[j] = vy[j] = 0.0f;
-        if (nn[j] == 1) {
-            const float2 q = b.pos[st[j]];
-            const float2 w = b.vel[st[j]];
-            px[j] = q.x; py[j] = q.y; vx[j] = w.x; vy[j] = w.y;
-            finish_particle(p, px[j], py[j], vx[j], vy[j]);
-            key[j] = cell_key(p, px[j], py[j]);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j)
-        if (nn[j] == 1) slot[j] = atomicAdd(&b.cnt[key[j]], 1u);
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        if (nn[j] == 1) {
-            b.pos_out[st[j]] = make_float2(px[j], py[j]);
-            b.vel_out[st[j]] = make_float2(vx[j], vy[j]);
-            b.kslot[st[j]] = make_uint2(key[j], slot[j] | kRunFlag);
-        }
-    }
-
-    // Cells of 2..8 particles.
-#pragma unroll 1
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        const uint32_t s = j == 0 ? st[0] : j == 1 ? st[1] : j == 2 ? st[2] : st[3];
-        const uint32_t n = j == 0 ? nn[0] : j == 1 ? nn[1] : j == 2 ? nn[2] : nn[3];
-        if (n == 2) small_cell<2, true>(p, b, s, n);
-        else if (n > 2 && n <= 4) small_cell<4, true>(p, b, s, n);
-        else if (n > 4 && n <= 8) small_cell<8, true>(p, b, s, n);
-    }
-
     // Fuller cells: LDS offsets inside the block, one global atomic per list.
-    uint32_t want_w = 0, want_b = 0;
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        want_w += nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort;
-        want_b += nn[j] > (uint32_t)kWaveSort;
-    }
-    uint32_t at_w = want_w ? atomicAdd(&s_cnt[0], want_w) : 0u;
-    uint32_t at_b = want_b ? atomicAdd(&s_cnt[1], want_b) : 0u;
+    const bool to_wave = n > 8 && n <= (uint32_t)kWaveSort;
+    const bool to_block = n > (uint32_t)kWaveSort;
+    const uint32_t at_w = to_wave ? atomicAdd(&s_cnt[0], 1u) : 0u;
+    const uint32_t at_b = to_block ? atomicAdd(&s_cnt[1], 1u) : 0u;
     __syncthreads();
     if (threadIdx.x < 2) {
         const uint32_t k = s_cnt[threadIdx.x];
         s_base[threadIdx.x] = k ? atomicAdd(&b.full_count[threadIdx.x], k) : 0u;
     }
     __syncthreads();
-    if (want_w | want_b) {
-#pragma unroll
-        for (int j = 0; j < kCellsPerLane; ++j) {
-            const uint4 ent = make_uint4(st[j], nn[j], c0 + j, 0);
-            if (nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort) {
-                const uint32_t at = s_base[0] + at_w++;
-                if (at < b.full_cap) b.full_wave[at] = ent;
-            } else if (nn[j] > (uint32_t)kWaveSort) {
-                const uint32_t at = s_base[1] + at_b++;
-                if (at < b.full_cap) b.full_block[at] = ent;
-            }
-        }
-    }
+    const uint4 ent = make_uint4(s, n, c, 0);
+    if (to_wave && s_base[0] + at_w < b.full_cap) b.full_wave[s_base[0] + at_w] = ent;
+    if (to_block && s_base[1] + at_b < b.full_cap) b.full_block[s_base[1] + at_b] = ent;
 }
 
-// Neighbour direction of a new cell relative to the old one: 0..8 for the
-// 3x3 neighbourhood (row-major from (-1,-1)), 9 for anything farther.
-__device__ __forceinline__ uint32_t direction(const FrameParams &p, float x, float y, uint32_t cx,
-                                              uint32_t cy, uint32_t &key) {
-    uint32_t kx, ky;
-    cell_xy(p, x, y, kx, ky);
-    key = ky * p.gx + kx;
-    const int dx = (int)kx - (int)cx, dy = (int)ky - (int)cy;
-    if (dx < -1 || dx > 1 || dy < -1 || dy > 1) return 9;
-    return (uint32_t)((dy + 1) * 3 + (dx + 1));
-}
-
-// Wave-private LDS of full_cells_kernel.
+// Wave-private LDS of full_cells_kernel.  The bitonic fallback uses `sort`;
+// the run path uses `perm` and `aux` (source indices, then new keys).
 struct WaveLds {
-    unsigned long long sort[kWaveSort];  // sort keys, or (run path) the permutation
-    uint32_t head_key[64];               // first source index of each run
-    uint32_t head_pos[64];               // stored position of each run's start
-    uint32_t run_off[64];                // canonical rank of each run's start
-    float4 fin[kMaxInCell];              // ranks 0..8 after the collisions
+    union {
+        unsigned long long sort[kWaveSort];
+        struct {
+            uint32_t perm[kWaveSort];  // perm[rank] = stored position
+            uint32_t aux[kWaveSort];   // src by position, then (direction | bit 31) or far key by rank
+        } r;
+    };
+    uint32_t head_key[64];  // first source index of each run
+    uint32_t head_pos[64];  // stored position of each run's start
+    uint32_t run_off[64];   // canonical rank of each run's start
+    float4 fin[kMaxInCell]; // ranks 0..8 after the collisions
 };
 
 // Gauss-Seidel on the nine lowest ranks by lane 0 (particles.rs:62-92);
@@ -562,9 +551,9 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
     const uint32_t lane = threadIdx.x & 63;
     const unsigned long long lt = lanemask_lt();
     const unsigned long long le = lt | (1ull << lane);
-    uint32_t *perm32 = reinterpret_cast<uint32_t *>(L.sort);
+    uint32_t *perm = L.r.perm;
 
-    // 1a. Run starts: flagged particles, and position 0.
+    // 1a. Run starts: flagged particles, and position 0.  src kept in LDS.
     uint32_t runs = 0;
     for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
         uint32_t v[kBatch];
@@ -578,6 +567,7 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
             const uint32_t i = b0 + 64 * q + lane;
             const bool start = i < n && (i == 0 || (v[q] & kRunFlag));
             const unsigned long long m = __ballot(start);
+            if (i < n) L.r.aux[i] = v[q];
             if (start) {
                 const uint32_t rid = runs + (uint32_t)__popcll(m & lt);
                 if (rid < 64) {
@@ -607,24 +597,16 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
         wave_sync();
         // 1c. perm[rank] = stored position.
         uint32_t seen = 0;
-        for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
-            uint32_t v[kBatch];
-#pragma unroll
-            for (int q = 0; q < kBatch; ++q) {
-                const uint32_t i = b0 + 64 * q + lane;
-                v[q] = i < n ? b.src[s + i] : 0u;
+        for (uint32_t base = 0; base < n; base += 64) {
+            const uint32_t i = base + lane;
+            const uint32_t v = i < n ? L.r.aux[i] : 0u;
+            const bool start = i < n && (i == 0 || (v & kRunFlag));
+            const unsigned long long m = __ballot(start);
+            if (i < n) {
+                const uint32_t rid = seen + (uint32_t)__popcll(m & le) - 1;
+                perm[L.run_off[rid] + (i - L.head_pos[rid])] = i;
             }
-#pragma unroll
-            for (int q = 0; q < kBatch; ++q) {
-                const uint32_t i = b0 + 64 * q + lane;
-                const bool start = i < n && (i == 0 || (v[q] & kRunFlag));
-                const unsigned long long m = __ballot(start);
-                if (i < n) {
-                    const uint32_t rid = seen + (uint32_t)__popcll(m & le) - 1;
-                    perm32[L.run_off[rid] + (i - L.head_pos[rid])] = i;
-                }
-                seen += (uint32_t)__popcll(m);
-            }
+            seen += (uint32_t)__popcll(m);
         }
     } else {
         uint32_t P = 64;
@@ -646,20 +628,21 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
                 wave_sync();
             }
         }
-        // Compact the low words in place, one chunk of 64 at a time: chunk q
-        // writes bytes [256q, 256q+256), which chunk q or an earlier one has
-        // already read.
+        // Compact the low words into perm in place, one chunk of 64 at a
+        // time: chunk q writes bytes [256q, 256q+256), which chunk q or an
+        // earlier one has already read.
         for (uint32_t base = 0; base < n; base += 64) {
             const uint32_t i = base + lane;
             const uint32_t a = i < n ? (uint32_t)L.sort[i] : 0u;
             wave_sync();
-            if (i < n) perm32[i] = a;
+            if (i < n) perm[i] = a;
             wave_sync();
         }
     }
     wave_sync();
 
-    // 2 + 3. Finish and write every rank; count particles per direction.
+    // 2 + 3. Finish and write every rank; count particles per direction;
+    //        keep each rank's new key in aux.
     const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
     uint32_t count[9];
 #pragma unroll
@@ -671,13 +654,13 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
             const uint32_t r = b0 + 64 * q + lane;
             lp[q] = lv[q] = make_float2(0.0f, 0.0f);
             if (r >= (uint32_t)kMaxInCell && r < n) {
-                const uint32_t a = perm32[r];
+                const uint32_t a = perm[r];
                 lp[q] = b.pos[s + a];
                 lv[q] = b.vel[s + a];
             }
         }
         if (b0 == 0) {
-            if (lane == 0) collide_nine(p, b, s, perm32, L.fin);
+            if (lane == 0) collide_nine(p, b, s, perm, L.fin);
             wave_sync();
         }
 #pragma unroll
@@ -696,6 +679,8 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
                 b.pos_out[s + r] = make_float2(px, py);
                 b.vel_out[s + r] = make_float2(vx, vy);
                 dir = direction(p, px, py, cx, cy, key);
+                // near: the direction code with bit 31 set; far: the key
+                L.r.aux[r] = dir < 9 ? (kRunFlag | dir) : key;
             }
 #pragma unroll
             for (int d = 0; d < 9; ++d)
@@ -704,7 +689,7 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
     }
 
     // 4. One atomic per occupied direction (lanes 0..8 together), then the
-    //    same lanes re-read their own ranks and write the slots.
+    //    slots, in rank order within each direction.
     uint32_t mine = 0;
 #pragma unroll
     for (int d = 0; d < 9; ++d) mine = lane == (uint32_t)d ? count[d] : mine;
@@ -719,30 +704,31 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
         first[d] = __shfl(got, d, 64);
         done[d] = 0;
     }
-    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
-        float2 q2[kBatch];
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const uint32_t r = b0 + 64 * q + lane;
-            q2[q] = r < n ? b.pos_out[s + r] : make_float2(0.0f, 0.0f);
-        }
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const uint32_t r = b0 + 64 * q + lane;
-            uint32_t dir = 10, key = 0, slot = 0;
-            if (r < n) dir = direction(p, q2[q].x, q2[q].y, cx, cy, key);
-#pragma unroll
-            for (int d = 0; d < 9; ++d) {
-                const unsigned long long m = __ballot(dir == (uint32_t)d);
-                if (dir == (uint32_t)d) {
-                    const uint32_t at = done[d] + (uint32_t)__popcll(m & lt);
-                    slot = (first[d] + at) | (at == 0 ? kRunFlag : 0u);
-                }
-                done[d] += (uint32_t)__popcll(m);
+    wave_sync();
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t r = base + lane;
+        uint32_t dir = 10, key = 0, slot = 0;
+        if (r < n) {
+            const uint32_t v = L.r.aux[r];
+            if (v & kRunFlag) {
+                dir = v & 15u;
+                key = (cy + dir / 3 - 1) * p.gx + (cx + dir % 3 - 1);
+            } else {
+                dir = 9;
+                key = v;
             }
-            if (dir == 9) slot = atomicAdd(&b.cnt[key], 1u) | kRunFlag;
-            if (r < n) b.kslot[s + r] = make_uint2(key, slot);
         }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const unsigned long long m = __ballot(dir == (uint32_t)d);
+            if (dir == (uint32_t)d) {
+                const uint32_t at = done[d] + (uint32_t)__popcll(m & lt);
+                slot = (first[d] + at) | (at == 0 ? kRunFlag : 0u);
+            }
+            done[d] += (uint32_t)__popcll(m);
+        }
+        if (dir == 9) slot = atomicAdd(&b.cnt[key], 1u) | kRunFlag;
+        if (r < n) b.kslot[s + r] = make_uint2(key, slot);
     }
 }
 
@@ -786,13 +772,13 @@ __global__ __launch_bounds__(kCollideBlock) void canonical_kernel(FrameParams p,
         if (n == 1) {
             one_particle<false>(p, b, s);
         } else if (n == 2) {
-            small_cell<2, false>(p, b, s, n);
+            small_cell<2, false>(p, b, c, s, n);
         } else if (n > 2 && n <= 4) {
-            small_cell<4, false>(p, b, s, n);
+            small_cell<4, false>(p, b, c, s, n);
         } else if (n > 4 && n <= 8) {
-            small_cell<8, false>(p, b, s, n);
+            small_cell<8, false>(p, b, c, s, n);
         } else if (n > 8 && n <= (uint32_t)kRegCell) {
-            small_cell<kRegCell, false>(p, b, s, n);
+            small_cell<kRegCell, false>(p, b, c, s, n);
         } else if (n > (uint32_t)kRegCell) {
             const uint32_t q = atomicAdd(&nbig, 1u);
             big_s[q] = s;
@@ -996,8 +982,7 @@ __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
 }  // namespace
 
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    const uint32_t per_block = kCollideBlock * kCellsPerLane;
-    const uint32_t grid = (p.ncell + per_block - 1) / per_block;
+    const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
     hipLaunchKernelGGL(collide_kernel, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
     return hipGetLastError();
 }
