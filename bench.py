@@ -67,6 +67,20 @@ def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
                       f"oracle wo_step_mt on {threads} host threads, {dt:.2f} s"}
 
 
+def copy_probe(nbytes: int) -> int:
+    """A plain device-to-device copy of nbytes, for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters (run alone, under the profiler).  Uses
+    torch in a process that never loads libwrach_hip.so."""
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +90,11 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the world would shard by row strips with a migrant exchange; not built).",

@@ -34,10 +34,13 @@ struct FrameBuffers {
     uint32_t *cnt;        // per-cell counters, zero between frames
     unsigned long long *tile_state;  // scan look-back words
     uint32_t *epoch;      // frame counter tagging the scan's status words; the scatter advances it
-    uint4 *full_wave;     // cells of kRegCell < n <= kWaveSort particles: (s, n, cell, 0)
+    uint4 *full_small;    // cells of 2..8 particles: (s, n, cell, 0)
+    uint4 *full_wave;     // cells of 9..kWaveSort particles
     uint4 *full_block;    // cells of more than kWaveSort particles
-    uint32_t *full_count; // [0] entries in full_wave, [1] in full_block; reset by the scan
-    uint32_t full_cap;    // capacity of each list
+    uint32_t *full_count; // entries in full_wave [0], full_block [1], full_small [2];
+                          // reset by the scan
+    uint32_t full_cap;    // capacity of full_wave and full_block
+    uint32_t small_cap;   // capacity of full_small
     uint32_t scan_tiles;
     uint32_t full_grid;   // blocks of the full-cell kernel
 };

@@ -1,18 +1,20 @@
 // Frame kernels of the particle step, written for gfx950 (wave64).
 //
 // One frame is four launches:
-//   collide_kernel     one cell per lane: cells of <= 8 particles are
-//                      handled in registers -- restore
-//                      the canonical order (sort by `src`), separate the first
-//                      9 particles (Gauss-Seidel, shaders/physics/src/
-//                      particles.rs:62-93), integrate and limit every particle
-//                      (particle.rs:40-82, cell.rs:79-95), write pos_out/
-//                      vel_out at the canonical index, compute the new cell key
-//                      (particles_per_cell.wgsl:14-27) and claim slots in the
-//                      new cells with one atomic per neighbour direction.
-//                      Fuller cells are queued.
-//   full_cells_kernel  one wavefront per queued cell (<= kWaveSort particles),
-//                      one block per larger cell.
+//   collide_kernel     four cells per lane (one 128-bit offset load):
+//                      single-particle cells are finished here -- integrate
+//                      and limit (shaders/physics/src/particle.rs:40-82),
+//                      write pos_out/vel_out, compute the new cell key
+//                      (particles_per_cell.wgsl:14-27), claim a slot in the
+//                      new cell with one atomic.  Fuller cells are queued.
+//   full_cells_kernel  the queued cells: one lane per cell of 2..8 particles,
+//                      one wavefront per cell of 9..kWaveSort, one block per
+//                      larger cell.  Each restores the canonical order (by
+//                      `src`), separates the first 9 particles (Gauss-Seidel,
+//                      particles.rs:62-93), finishes every particle (the
+//                      overflow ones without collisions, cell.rs:79-95),
+//                      writes pos_out/vel_out at the canonical index and
+//                      claims slots with one atomic per neighbour direction.
 //   scan_kernel        single-pass exclusive scan of the per-cell counts with
 //                      decoupled look-back; writes [0, start_0.., N] and clears
 //                      the counts for the next frame (prefix_sum.wgsl +
@@ -97,8 +99,14 @@ __device__ __forceinline__ void collide_pair(float &xi, float &yi, float &xj, fl
 // the step never produces; it keeps every key inside the count array.
 __device__ __forceinline__ void cell_xy(const FrameParams &p, float x, float y, uint32_t &kx,
                                         uint32_t &ky) {
-    float fx = floorf((x - p.ax) / p.cs);
-    float fy = floorf((y - p.ay) / p.cs);
+    float fx, fy;
+    if (p.cs == 1.0f) {  // x / 1 is exact: skip the division sequence
+        fx = floorf(x - p.ax);
+        fy = floorf(y - p.ay);
+    } else {
+        fx = floorf((x - p.ax) / p.cs);
+        fy = floorf((y - p.ay) / p.cs);
+    }
     fx = fminf(fmaxf(fx, 0.0f), (float)(p.gx - 1));
     fy = fminf(fmaxf(fy, 0.0f), (float)(p.gy - 1));
     kx = (uint32_t)fx;
@@ -453,44 +461,99 @@ __device__ __forceinline__ void clamp_range(const FrameParams &p, uint32_t s, ui
     else if (n > p.n - s) n = p.n - s;
 }
 
-// One cell per lane, so no lane waits on more than one cell's round trips:
-// cells of 1..8 particles are handled in registers, with every slot atomic
-// of the cell in flight at once; fuller cells are queued for
-// full_cells_kernel with one global atomic per block and list.
+// Four consecutive cells per lane (one 128-bit offset load).  Single-particle
+// cells are finished here, their slot atomics in flight together; every
+// fuller cell is queued for full_cells_kernel, which spreads them over the
+// whole GPU (after a few hundred frames under gravity they crowd into a few
+// rows, which would otherwise land on a few CUs).  One global atomic per
+// block and list.
+constexpr int kCellsPerLane = 4;
+
 __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ uint32_t s_cnt[2], s_base[2];
-    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __shared__ uint32_t s_cnt[3], s_base[3];
+    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t c = blockIdx.x * kCollideBlock + threadIdx.x;
-    uint32_t s = 0, n = 0;
-    if (c < p.ncell) {
-        s = b.idx[c + 1];
-        n = b.idx[c + 2] - s;
-        clamp_range(p, s, n);
+    const uint32_t c0 = (blockIdx.x * kCollideBlock + threadIdx.x) * kCellsPerLane;
+    uint32_t o[kCellsPerLane + 1];
+    if (c0 + kCellsPerLane <= p.ncell) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(b.idx + 1 + c0);  // 16-byte aligned
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+        o[4] = b.idx[c0 + 5];
+    } else {
+        // idx has ncell + 2 entries; entry c + 1 exists for every c <= ncell.
+#pragma unroll
+        for (int j = 0; j <= kCellsPerLane; ++j)
+            o[j] = c0 + j <= p.ncell ? b.idx[c0 + 1 + j] : (j ? o[j - 1] : 0u);
     }
-    if (n == 1) {
-        one_particle<true>(p, b, s);
-    } else if (n == 2) {
-        small_cell<2, true>(p, b, c, s, n);
-    } else if (n > 2 && n <= 4) {
-        small_cell<4, true>(p, b, c, s, n);
-    } else if (n > 4 && n <= 8) {
-        small_cell<8, true>(p, b, c, s, n);
+    uint32_t st[kCellsPerLane], nn[kCellsPerLane];
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        st[j] = o[j];
+        nn[j] = c0 + j < p.ncell ? o[j + 1] - o[j] : 0u;
+        clamp_range(p, st[j], nn[j]);
     }
-    // Fuller cells: LDS offsets inside the block, one global atomic per list.
-    const bool to_wave = n > 8 && n <= (uint32_t)kWaveSort;
-    const bool to_block = n > (uint32_t)kWaveSort;
-    const uint32_t at_w = to_wave ? atomicAdd(&s_cnt[0], 1u) : 0u;
-    const uint32_t at_b = to_block ? atomicAdd(&s_cnt[1], 1u) : 0u;
+
+    // Single-particle cells.
+    float px[kCellsPerLane], py[kCellsPerLane], vx[kCellsPerLane], vy[kCellsPerLane];
+    uint32_t key[kCellsPerLane], slot[kCellsPerLane];
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        key[j] = 0;
+        slot[j] = 0;
+        px[j] = py[j] = vx[j] = vy[j] = 0.0f;
+        if (nn[j] == 1) {
+            const float2 q = b.pos[st[j]];
+            const float2 w = b.vel[st[j]];
+            px[j] = q.x; py[j] = q.y; vx[j] = w.x; vy[j] = w.y;
+            finish_particle(p, px[j], py[j], vx[j], vy[j]);
+            key[j] = cell_key(p, px[j], py[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j)
+        if (nn[j] == 1) slot[j] = atomicAdd(&b.cnt[key[j]], 1u);
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        if (nn[j] == 1) {
+            b.pos_out[st[j]] = make_float2(px[j], py[j]);
+            b.vel_out[st[j]] = make_float2(vx[j], vy[j]);
+            b.kslot[st[j]] = make_uint2(key[j], slot[j] | kRunFlag);
+        }
+    }
+
+    // Fuller cells: list 2 (2..8), 0 (9..kWaveSort), 1 (more).
+    uint32_t want[3] = {0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        want[2] += nn[j] >= 2 && nn[j] <= 8;
+        want[0] += nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort;
+        want[1] += nn[j] > (uint32_t)kWaveSort;
+    }
+    uint32_t at[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) at[l] = want[l] ? atomicAdd(&s_cnt[l], want[l]) : 0u;
     __syncthreads();
-    if (threadIdx.x < 2) {
+    if (threadIdx.x < 3) {
         const uint32_t k = s_cnt[threadIdx.x];
         s_base[threadIdx.x] = k ? atomicAdd(&b.full_count[threadIdx.x], k) : 0u;
     }
     __syncthreads();
-    const uint4 ent = make_uint4(s, n, c, 0);
-    if (to_wave && s_base[0] + at_w < b.full_cap) b.full_wave[s_base[0] + at_w] = ent;
-    if (to_block && s_base[1] + at_b < b.full_cap) b.full_block[s_base[1] + at_b] = ent;
+    if (want[0] | want[1] | want[2]) {
+#pragma unroll
+        for (int j = 0; j < kCellsPerLane; ++j) {
+            const uint4 ent = make_uint4(st[j], nn[j], c0 + j, 0);
+            if (nn[j] >= 2 && nn[j] <= 8) {
+                const uint32_t a = s_base[2] + at[2]++;
+                if (a < b.small_cap) b.full_small[a] = ent;
+            } else if (nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort) {
+                const uint32_t a = s_base[0] + at[0]++;
+                if (a < b.full_cap) b.full_wave[a] = ent;
+            } else if (nn[j] > (uint32_t)kWaveSort) {
+                const uint32_t a = s_base[1] + at[1]++;
+                if (a < b.full_cap) b.full_block[a] = ent;
+            }
+        }
+    }
 }
 
 // Wave-private LDS of full_cells_kernel.  The bitonic fallback uses `sort`;
@@ -500,35 +563,59 @@ struct WaveLds {
         unsigned long long sort[kWaveSort];
         struct {
             uint32_t perm[kWaveSort];  // perm[rank] = stored position
-            uint32_t aux[kWaveSort];   // src by position, then (direction | bit 31) or far key by rank
+            uint32_t aux[kWaveSort];   // src by position, then per rank: direction and
+                                       // rank in it (bit 31 set), or a far mover's key
         } r;
     };
     uint32_t head_key[64];  // first source index of each run
     uint32_t head_pos[64];  // stored position of each run's start
     uint32_t run_off[64];   // canonical rank of each run's start
-    float4 fin[kMaxInCell]; // ranks 0..8 after the collisions
 };
 
-// Gauss-Seidel on the nine lowest ranks by lane 0 (particles.rs:62-92);
-// results, finished, go to L.fin.
-__device__ __forceinline__ void collide_nine(const FrameParams &p, const FrameBuffers &b,
-                                             uint32_t s, const uint32_t *perm, float4 *fin) {
-    float px[kMaxInCell], py[kMaxInCell], vx[kMaxInCell], vy[kMaxInCell];
+// Gauss-Seidel on ranks 0..8 across lanes 0..8 of a wavefront, lane l
+// holding particle l.  In the in-order loop (particles.rs:62-81) pair (i,j)
+// can run at step i + j: the pairs of one step touch disjoint particles and
+// every particle still sees its updates in the loop's order, so 15 steps give
+// the loop's result bit for bit (36 pairs in sequence otherwise).  The lane
+// holding i computes the pair as collide_pair does; the lane holding j
+// receives the displacement.  All lanes execute the shuffles.
+__device__ __forceinline__ void collide_nine_lanes(float &x, float &y) {
+    const int lane = (int)(threadIdx.x & 63);
 #pragma unroll
-    for (int q = 0; q < kMaxInCell; ++q) {
-        const uint32_t a = perm[q];
-        const float2 pq = b.pos[s + a];
-        const float2 vq = b.vel[s + a];
-        px[q] = pq.x; py[q] = pq.y; vx[q] = vq.x; vy[q] = vq.y;
-    }
-#pragma unroll
-    for (int i = 0; i < kMaxInCell; ++i)
-#pragma unroll
-        for (int j = i + 1; j < kMaxInCell; ++j) collide_pair(px[i], py[i], px[j], py[j]);
-#pragma unroll
-    for (int q = 0; q < kMaxInCell; ++q) {
-        finish_particle(p, px[q], py[q], vx[q], vy[q]);
-        fin[q] = make_float4(px[q], py[q], vx[q], vy[q]);
+    for (int t = 1; t <= 2 * kMaxInCell - 3; ++t) {
+        const int partner = t - lane;
+        const bool left = lane < partner && partner < kMaxInCell;
+        const bool right = partner >= 0 && partner < lane && lane < kMaxInCell;
+        const int from = (left || right) ? partner : lane;
+        const float qx = __shfl(x, from, 64);
+        const float qy = __shfl(y, from, 64);
+        float ex = 0.0f, ey = 0.0f;
+        int hit = 0;
+        if (left) {
+            const float dx = x - qx;
+            const float dy = y - qy;
+            float sq = dx * dx;
+            sq = sq + dy * dy;
+            float d = sqrtf(sq);
+            if (!(d > 1.0f)) {
+                if (d == 0.0f) d = 0.0001f;
+                const float f = (0.5f * (1.0f - d)) / d;
+                ex = (qx - x) * f;
+                ey = (qy - y) * f;
+                hit = 1;
+            }
+        }
+        const float rx = __shfl(ex, from, 64);
+        const float ry = __shfl(ey, from, 64);
+        const int rhit = __shfl(hit, from, 64);
+        if (left && hit) {
+            x = x - ex;
+            y = y - ey;
+        }
+        if (right && rhit) {
+            x = x + rx;
+            y = y + ry;
+        }
     }
 }
 
@@ -536,10 +623,9 @@ __device__ __forceinline__ void collide_nine(const FrameParams &p, const FrameBu
 //  1. Canonical order: runs (flagged in src) ordered by their first source
 //     index -> perm[rank] = stored position; more than 64 runs fall back to
 //     a bitonic sort of (src, position).
-//  2. Lane 0: Gauss-Seidel on ranks 0..8, overlapped with the first batch of
-//     gathers.
-//  3. Every rank finished and written at its canonical index; particles
-//     counted per neighbour direction.
+//  2. Gauss-Seidel on ranks 0..8 across lanes 0..8 (collide_nine_lanes).
+//  3. Every rank finished and written at its canonical index; its rank
+//     among the cell's particles going the same way kept in LDS.
 //  4. One atomic per occupied direction reserves a contiguous, canonically
 //     ordered run in the new cell; a farther mover takes its own slot.
 // Loops work in batches of four 64-rank chunks so that four gathers per lane
@@ -554,16 +640,17 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
     uint32_t *perm = L.r.perm;
 
     // 1a. Run starts: flagged particles, and position 0.  src kept in LDS.
+    constexpr int kSrcBatch = 2 * kBatch;
     uint32_t runs = 0;
-    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
-        uint32_t v[kBatch];
+    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kSrcBatch) {
+        uint32_t v[kSrcBatch];
 #pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
+        for (int q = 0; q < kSrcBatch; ++q) {
             const uint32_t i = b0 + 64 * q + lane;
             v[q] = i < n ? b.src[s + i] : 0u;
         }
 #pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
+        for (int q = 0; q < kSrcBatch; ++q) {
             const uint32_t i = b0 + 64 * q + lane;
             const bool start = i < n && (i == 0 || (v[q] & kRunFlag));
             const unsigned long long m = __ballot(start);
@@ -653,38 +740,35 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
         for (int q = 0; q < kBatch; ++q) {
             const uint32_t r = b0 + 64 * q + lane;
             lp[q] = lv[q] = make_float2(0.0f, 0.0f);
-            if (r >= (uint32_t)kMaxInCell && r < n) {
+            if (r < n) {
                 const uint32_t a = perm[r];
                 lp[q] = b.pos[s + a];
                 lv[q] = b.vel[s + a];
             }
         }
-        if (b0 == 0) {
-            if (lane == 0) collide_nine(p, b, s, perm, L.fin);
-            wave_sync();
-        }
+        if (b0 == 0) collide_nine_lanes(lp[0].x, lp[0].y);  // ranks 0..8 (n > 8)
 #pragma unroll
         for (int q = 0; q < kBatch; ++q) {
             const uint32_t r = b0 + 64 * q + lane;
             uint32_t dir = 10, key;
             if (r < n) {
-                float px, py, vx, vy;
-                if (r < (uint32_t)kMaxInCell) {
-                    const float4 f = L.fin[r];
-                    px = f.x; py = f.y; vx = f.z; vy = f.w;
-                } else {
-                    px = lp[q].x; py = lp[q].y; vx = lv[q].x; vy = lv[q].y;
-                    finish_particle(p, px, py, vx, vy);
-                }
+                float px = lp[q].x, py = lp[q].y, vx = lv[q].x, vy = lv[q].y;
+                finish_particle(p, px, py, vx, vy);
                 b.pos_out[s + r] = make_float2(px, py);
                 b.vel_out[s + r] = make_float2(vx, vy);
                 dir = direction(p, px, py, cx, cy, key);
-                // near: the direction code with bit 31 set; far: the key
-                L.r.aux[r] = dir < 9 ? (kRunFlag | dir) : key;
             }
+            // rank of this particle among the cell's particles going the
+            // same way, in canonical order
+            uint32_t rank = 0;
 #pragma unroll
-            for (int d = 0; d < 9; ++d)
-                count[d] += (uint32_t)__popcll(__ballot(dir == (uint32_t)d));
+            for (int d = 0; d < 9; ++d) {
+                const unsigned long long m = __ballot(dir == (uint32_t)d);
+                if (dir == (uint32_t)d) rank = count[d] + (uint32_t)__popcll(m & lt);
+                count[d] += (uint32_t)__popcll(m);
+            }
+            // near: bit 31 | direction << 27 | rank; far: the key
+            if (r < n) L.r.aux[r] = dir < 9 ? (kRunFlag | (dir << 27) | rank) : key;
         }
     }
 
@@ -698,52 +782,53 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
         const uint32_t tx = cx + lane % 3 - 1, ty = cy + lane / 3 - 1;
         got = atomicAdd(&b.cnt[ty * p.gx + tx], mine);
     }
-    uint32_t first[9], done[9];
-#pragma unroll
-    for (int d = 0; d < 9; ++d) {
-        first[d] = __shfl(got, d, 64);
-        done[d] = 0;
-    }
+    if (lane < 9) L.head_key[lane] = got;  // run bases; the run heads are no longer needed
     wave_sync();
     for (uint32_t base = 0; base < n; base += 64) {
         const uint32_t r = base + lane;
-        uint32_t dir = 10, key = 0, slot = 0;
         if (r < n) {
             const uint32_t v = L.r.aux[r];
+            uint32_t key, slot;
             if (v & kRunFlag) {
-                dir = v & 15u;
+                const uint32_t dir = (v >> 27) & 15u, rank = v & 0x07ffffffu;
                 key = (cy + dir / 3 - 1) * p.gx + (cx + dir % 3 - 1);
+                slot = (L.head_key[dir] + rank) | (rank == 0 ? kRunFlag : 0u);
             } else {
-                dir = 9;
                 key = v;
+                slot = atomicAdd(&b.cnt[key], 1u) | kRunFlag;
             }
+            b.kslot[s + r] = make_uint2(key, slot);
         }
-#pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            const unsigned long long m = __ballot(dir == (uint32_t)d);
-            if (dir == (uint32_t)d) {
-                const uint32_t at = done[d] + (uint32_t)__popcll(m & lt);
-                slot = (first[d] + at) | (at == 0 ? kRunFlag : 0u);
-            }
-            done[d] += (uint32_t)__popcll(m);
-        }
-        if (dir == 9) slot = atomicAdd(&b.cnt[key], 1u) | kRunFlag;
-        if (r < n) b.kslot[s + r] = make_uint2(key, slot);
     }
 }
 
-// Cells queued by collide_kernel.  Each wavefront takes cells of the wave
-// list; then each block takes cells of the block list (over kWaveSort
-// particles: block-wide LDS sort, or merge sort past kLdsSort).
+// Cells queued by collide_kernel: wave-list cells (one wavefront each),
+// cells of 2..8 particles (one lane each, registers, one atomic per
+// neighbour direction), then block-list cells (over kWaveSort particles:
+// block-wide LDS sort, or merge sort past kLdsSort).
 __global__ __launch_bounds__(kCollideBlock) void full_cells_kernel(FrameParams p, FrameBuffers b) {
     __shared__ WaveLds lds[kWavesPerBlock];
     static_assert(sizeof(WaveLds) * kWavesPerBlock >= kLdsSort * sizeof(unsigned long long),
                   "the block path reuses the wave regions");
+    // Wave-list cells (the long per-cell chains) go to waves from the bottom
+    // of the grid; small cells to lanes counted from the top, so a wave
+    // holding a full cell takes small cells only once the full list has run
+    // out of waves.
     const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t gwave = blockIdx.x * kWavesPerBlock + wave;
+    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     const uint32_t nw = min(b.full_count[0], b.full_cap);
-    for (uint32_t e = blockIdx.x * kWavesPerBlock + wave; e < nw; e += gridDim.x * kWavesPerBlock) {
+    for (uint32_t e = gwave; e < nw; e += nwaves) {
         const uint4 cell = b.full_wave[e];
         wave_cell(p, b, cell.x, cell.y, cell.z, lds[wave]);
+    }
+    const uint32_t ns = min(b.full_count[2], b.small_cap);
+    const uint32_t nlanes = nwaves * 64;
+    for (uint32_t e = (nwaves - 1 - gwave) * 64 + (threadIdx.x & 63); e < ns; e += nlanes) {
+        const uint4 cell = b.full_small[e];
+        if (cell.y == 2) small_cell<2, true>(p, b, cell.z, cell.x, cell.y);
+        else if (cell.y <= 4) small_cell<4, true>(p, b, cell.z, cell.x, cell.y);
+        else small_cell<8, true>(p, b, cell.z, cell.x, cell.y);
     }
     const uint32_t nb = min(b.full_count[1], b.full_cap);
     if (nb == 0) return;
@@ -957,6 +1042,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
         b.idx[0] = 0;
         b.full_count[0] = 0;  // the full-cell kernel of this frame has finished
         b.full_count[1] = 0;
+        b.full_count[2] = 0;
     }
     if (tile == b.scan_tiles - 1 && tid == 0) b.idx[p.ncell + 1] = s_prefix + agg;
 }
@@ -982,7 +1068,8 @@ __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
 }  // namespace
 
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
+    const uint32_t per_block = kCollideBlock * kCellsPerLane;
+    const uint32_t grid = (p.ncell + per_block - 1) / per_block;
     hipLaunchKernelGGL(collide_kernel, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
     return hipGetLastError();
 }
