@@ -576,9 +576,10 @@ struct WaveLds {
 // holding particle l.  In the in-order loop (particles.rs:62-81) pair (i,j)
 // can run at step i + j: the pairs of one step touch disjoint particles and
 // every particle still sees its updates in the loop's order, so 15 steps give
-// the loop's result bit for bit (36 pairs in sequence otherwise).  The lane
-// holding i computes the pair as collide_pair does; the lane holding j
-// receives the displacement.  All lanes execute the shuffles.
+// the loop's result bit for bit (36 pairs in sequence otherwise).  Both lanes
+// of a pair fetch the other's position and evaluate collide_pair's
+// arithmetic on the same operands; lane i applies -dv, lane j +dv.  All
+// lanes execute the shuffles.
 __device__ __forceinline__ void collide_nine_lanes(float &x, float &y) {
     const int lane = (int)(threadIdx.x & 63);
 #pragma unroll
@@ -589,32 +590,27 @@ __device__ __forceinline__ void collide_nine_lanes(float &x, float &y) {
         const int from = (left || right) ? partner : lane;
         const float qx = __shfl(x, from, 64);
         const float qy = __shfl(y, from, 64);
-        float ex = 0.0f, ey = 0.0f;
-        int hit = 0;
-        if (left) {
-            const float dx = x - qx;
-            const float dy = y - qy;
+        if (left || right) {
+            const float xi = left ? x : qx, yi = left ? y : qy;
+            const float xj = left ? qx : x, yj = left ? qy : y;
+            const float dx = xi - xj;
+            const float dy = yi - yj;
             float sq = dx * dx;
             sq = sq + dy * dy;
             float d = sqrtf(sq);
             if (!(d > 1.0f)) {
                 if (d == 0.0f) d = 0.0001f;
                 const float f = (0.5f * (1.0f - d)) / d;
-                ex = (qx - x) * f;
-                ey = (qy - y) * f;
-                hit = 1;
+                const float ex = (xj - xi) * f;
+                const float ey = (yj - yi) * f;
+                if (left) {
+                    x = x - ex;
+                    y = y - ey;
+                } else {
+                    x = x + ex;
+                    y = y + ey;
+                }
             }
-        }
-        const float rx = __shfl(ex, from, 64);
-        const float ry = __shfl(ey, from, 64);
-        const int rhit = __shfl(hit, from, 64);
-        if (left && hit) {
-            x = x - ex;
-            y = y - ey;
-        }
-        if (right && rhit) {
-            x = x + rx;
-            y = y + ry;
         }
     }
 }
