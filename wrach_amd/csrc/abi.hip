@@ -138,10 +138,8 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.cnt);
     (void)hipFree(c->buf.tile_state);
     (void)hipFree(c->buf.epoch);
-    (void)hipFree(c->buf.full_small);
-    (void)hipFree(c->buf.full_wave);
-    (void)hipFree(c->buf.full_block);
-    (void)hipFree(c->buf.full_count);
+    (void)hipFree(c->buf.cells);
+    (void)hipFree(c->buf.class_count);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -260,15 +258,19 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
 
     wrach::FrameBuffers &b = c->buf;
     b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell);
-    // A cell goes on a full-cell list when it holds more than kRegCell
-    // particles, so each list needs at most capacity / (kRegCell + 1) entries.
-    b.full_cap = cap / 9 + 64;      // cells of more than 8 particles
-    b.small_cap = cap / 2 + 64;     // cells of 2..8 particles
+    // Queued-cell regions: a class of cells holding at least m particles has
+    // at most capacity / m cells.
+    size_t ncells_total = 0;
+    for (int k = 0; k < wrach::kClasses; ++k) {
+        b.class_off[k] = (uint32_t)ncells_total;
+        b.class_cap[k] = cap / wrach::kClassMinN[k] + 64;
+        ncells_total += b.class_cap[k];
+    }
     {
-        // one wave per full cell, or one lane per small cell, if that fits
-        const uint32_t by_wave = (b.full_cap + wrach::kWavesPerBlock - 1) / wrach::kWavesPerBlock;
-        const uint32_t by_lane = (b.small_cap + wrach::kCollideBlock - 1) / wrach::kCollideBlock;
-        const uint32_t want = by_wave > by_lane ? by_wave : by_lane;
+        // Enough wavefronts for one pass over the largest wave or group list
+        // (every block resident at once at kFullGridMax).
+        const uint32_t waves = b.class_cap[0] * 2 / 64 + b.class_cap[wrach::kWaveClass];
+        const uint32_t want = (waves + wrach::kWavesPerBlock - 1) / wrach::kWavesPerBlock;
         b.full_grid = want < wrach::kFullGridMax ? want : wrach::kFullGridMax;
     }
     auto bail = [&](int rc) { release(c); return rc; };
@@ -292,10 +294,8 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.cnt, ncell + 16));
     WRACH_HIP_C(dmalloc(&b.tile_state, b.scan_tiles));
     WRACH_HIP_C(dmalloc(&b.epoch, 1));
-    WRACH_HIP_C(dmalloc(&b.full_small, b.small_cap));
-    WRACH_HIP_C(dmalloc(&b.full_wave, b.full_cap));
-    WRACH_HIP_C(dmalloc(&b.full_block, b.full_cap));
-    WRACH_HIP_C(dmalloc(&b.full_count, 3));
+    WRACH_HIP_C(dmalloc(&b.cells, ncells_total));
+    WRACH_HIP_C(dmalloc(&b.class_count, wrach::kClasses));
     WRACH_HIP_C(hipMemsetAsync(c->idx_alloc, 0, (nidx + 64) * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.pos, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.vel, 0, cap * sizeof(float2), c->stream));
@@ -308,7 +308,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
         WRACH_HIP_C(hipMemcpyAsync(b.epoch, &first_epoch, sizeof first_epoch, hipMemcpyHostToDevice,
                                    c->stream));
     }
-    WRACH_HIP_C(hipMemsetAsync(b.full_count, 0, 3 * sizeof(uint32_t), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.class_count, 0, wrach::kClasses * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipStreamSynchronize(c->stream));
 #undef WRACH_HIP_C
     *out = c;

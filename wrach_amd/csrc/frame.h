@@ -34,13 +34,10 @@ struct FrameBuffers {
     uint32_t *cnt;        // per-cell counters, zero between frames
     unsigned long long *tile_state;  // scan look-back words
     uint32_t *epoch;      // frame counter tagging the scan's status words; the scatter advances it
-    uint4 *full_small;    // cells of 2..8 particles: (s, n, cell, 0)
-    uint4 *full_wave;     // cells of 9..kWaveSort particles
-    uint4 *full_block;    // cells of more than kWaveSort particles
-    uint32_t *full_count; // entries in full_wave [0], full_block [1], full_small [2];
-                          // reset by the scan
-    uint32_t full_cap;    // capacity of full_wave and full_block
-    uint32_t small_cap;   // capacity of full_small
+    uint4 *cells;         // queued cells (s, n, cell, 0), one region per size class
+    uint32_t *class_count;  // queued cells per class; reset by the scan
+    uint32_t class_off[8];  // first entry of each class's region in `cells`
+    uint32_t class_cap[8];  // entries of each region
     uint32_t scan_tiles;
     uint32_t full_grid;   // blocks of the full-cell kernel
 };
@@ -56,6 +53,17 @@ constexpr int kLdsSort = 2048;           // keys per LDS bitonic sort (block pat
 constexpr int kWaveSort = 1024;          // keys per wave-level LDS bitonic sort
 constexpr int kWavesPerBlock = kCollideBlock / 64;
 constexpr uint32_t kFullGridMax = 1024;  // blocks of the full-cell kernel
+
+// Size classes of queued cells.  Classes 0..5 hold cells of 2, 3-4, 5-8,
+// 9-16, 17-32 and 33-64 particles, processed by groups of 2..64 lanes with
+// one particle per lane; class 6 (65..kWaveSort) takes a wavefront per cell,
+// class 7 a block per cell.
+constexpr int kClasses = 8;
+constexpr int kGroupClasses = 6;
+constexpr int kWaveClass = 6;
+constexpr int kBlockClass = 7;
+// Fewest particles a cell of each class holds (sizes the list regions).
+constexpr uint32_t kClassMinN[kClasses] = {2, 3, 5, 9, 17, 33, 65, kWaveSort + 1};
 
 inline uint32_t scan_tiles_for(uint32_t ncell) {
     return (ncell + kScanTile - 1) / kScanTile;

@@ -469,9 +469,18 @@ __device__ __forceinline__ void clamp_range(const FrameParams &p, uint32_t s, ui
 // block and list.
 constexpr int kCellsPerLane = 4;
 
+// Size class of a cell of n particles (kClasses for none: 0 or 1 particle).
+// For 2 <= n <= 64 the class is ceil(log2(n)) - 1.
+__device__ __forceinline__ uint32_t size_class(uint32_t n) {
+    if (n < 2) return (uint32_t)kClasses;
+    if (n > (uint32_t)kWaveSort) return (uint32_t)kBlockClass;
+    const uint32_t k = 31u - (uint32_t)__builtin_clz(n - 1);
+    return k < (uint32_t)kWaveClass ? k : (uint32_t)kWaveClass;
+}
+
 __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ uint32_t s_cnt[3], s_base[3];
-    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+    __shared__ uint32_t s_cnt[kClasses], s_base[kClasses];
+    if (threadIdx.x < (uint32_t)kClasses) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t c0 = (blockIdx.x * kCollideBlock + threadIdx.x) * kCellsPerLane;
     uint32_t o[kCellsPerLane + 1];
@@ -521,37 +530,26 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
         }
     }
 
-    // Fuller cells: list 2 (2..8), 0 (9..kWaveSort), 1 (more).
-    uint32_t want[3] = {0, 0, 0};
+    // Fuller cells: one list per size class.  LDS positions inside the
+    // block, then one global atomic per class.
+    uint32_t cls[kCellsPerLane], at[kCellsPerLane];
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
-        want[2] += nn[j] >= 2 && nn[j] <= 8;
-        want[0] += nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort;
-        want[1] += nn[j] > (uint32_t)kWaveSort;
+        cls[j] = size_class(nn[j]);
+        at[j] = cls[j] < (uint32_t)kClasses ? atomicAdd(&s_cnt[cls[j]], 1u) : 0u;
     }
-    uint32_t at[3];
-#pragma unroll
-    for (int l = 0; l < 3; ++l) at[l] = want[l] ? atomicAdd(&s_cnt[l], want[l]) : 0u;
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < (uint32_t)kClasses) {
         const uint32_t k = s_cnt[threadIdx.x];
-        s_base[threadIdx.x] = k ? atomicAdd(&b.full_count[threadIdx.x], k) : 0u;
+        s_base[threadIdx.x] = k ? atomicAdd(&b.class_count[threadIdx.x], k) : 0u;
     }
     __syncthreads();
-    if (want[0] | want[1] | want[2]) {
 #pragma unroll
-        for (int j = 0; j < kCellsPerLane; ++j) {
-            const uint4 ent = make_uint4(st[j], nn[j], c0 + j, 0);
-            if (nn[j] >= 2 && nn[j] <= 8) {
-                const uint32_t a = s_base[2] + at[2]++;
-                if (a < b.small_cap) b.full_small[a] = ent;
-            } else if (nn[j] > 8 && nn[j] <= (uint32_t)kWaveSort) {
-                const uint32_t a = s_base[0] + at[0]++;
-                if (a < b.full_cap) b.full_wave[a] = ent;
-            } else if (nn[j] > (uint32_t)kWaveSort) {
-                const uint32_t a = s_base[1] + at[1]++;
-                if (a < b.full_cap) b.full_block[a] = ent;
-            }
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        if (cls[j] < (uint32_t)kClasses) {
+            const uint32_t a = s_base[cls[j]] + at[j];
+            if (a < b.class_cap[cls[j]])
+                b.cells[b.class_off[cls[j]] + a] = make_uint4(st[j], nn[j], c0 + j, 0);
         }
     }
 }
@@ -572,22 +570,26 @@ struct WaveLds {
     uint32_t run_off[64];   // canonical rank of each run's start
 };
 
-// Gauss-Seidel on ranks 0..8 across lanes 0..8 of a wavefront, lane l
-// holding particle l.  In the in-order loop (particles.rs:62-81) pair (i,j)
-// can run at step i + j: the pairs of one step touch disjoint particles and
-// every particle still sees its updates in the loop's order, so 15 steps give
-// the loop's result bit for bit (36 pairs in sequence otherwise).  Both lanes
-// of a pair fetch the other's position and evaluate collide_pair's
-// arithmetic on the same operands; lane i applies -dv, lane j +dv.  All
+// Gauss-Seidel on the first m = min(n, 9) particles of a cell held by lanes
+// gbase + r (r = canonical rank).  In the in-order loop (particles.rs:62-81)
+// pair (i,j) can run at step i + j: the pairs of one step touch disjoint
+// particles and every particle still sees its updates in the loop's order, so
+// 2M - 3 steps (M = min(G, 9)) give the loop's result bit for bit.  Both lanes
+// of a pair fetch the other's position and evaluate collide_pair's arithmetic
+// on the same operands; the lower rank applies -dv, the higher +dv.  All
 // lanes execute the shuffles.
-__device__ __forceinline__ void collide_nine_lanes(float &x, float &y) {
+template <int G>
+__device__ __forceinline__ void collide_group(float &x, float &y, uint32_t r, uint32_t m,
+                                              uint32_t gbase) {
+    constexpr int M = G < kMaxInCell ? G : kMaxInCell;
     const int lane = (int)(threadIdx.x & 63);
+    const int ri = (int)r, mi = (int)m;
 #pragma unroll
-    for (int t = 1; t <= 2 * kMaxInCell - 3; ++t) {
-        const int partner = t - lane;
-        const bool left = lane < partner && partner < kMaxInCell;
-        const bool right = partner >= 0 && partner < lane && lane < kMaxInCell;
-        const int from = (left || right) ? partner : lane;
+    for (int t = 1; t <= 2 * M - 3; ++t) {
+        const int partner = t - ri;
+        const bool left = ri < partner && partner < mi;
+        const bool right = partner >= 0 && partner < ri && ri < mi;
+        const int from = (left || right) ? (int)gbase + partner : lane;
         const float qx = __shfl(x, from, 64);
         const float qy = __shfl(y, from, 64);
         if (left || right) {
@@ -612,6 +614,84 @@ __device__ __forceinline__ void collide_nine_lanes(float &x, float &y) {
                 }
             }
         }
+    }
+}
+
+__device__ __forceinline__ void collide_nine_lanes(float &x, float &y) {
+    collide_group<64>(x, y, threadIdx.x & 63, kMaxInCell, 0);
+}
+
+// Cells of 2 <= n <= G particles (G = 2..64), 64 / G cells per wavefront,
+// one particle per lane.  Each group of G lanes:
+//  1. loads its cell's particles (lane j: stored position j);
+//  2. ranks them by source index (the canonical order) and moves rank r to
+//     lane gbase + r;
+//  3. separates the first min(n, 9) (collide_group);
+//  4. finishes every particle, writes it at its canonical index;
+//  5. claims slots: one atomic per occupied neighbour direction (by the
+//     direction's first lane), a run of consecutive slots in canonical order;
+//     a farther mover takes a slot of its own.
+template <int G>
+__device__ void group_cells(const FrameParams &p, const FrameBuffers &b, const uint4 *list,
+                            uint32_t count, uint32_t first) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t j = lane % G;
+    const uint32_t gbase = lane - j;
+    const uint32_t e = first + lane / G;
+    uint4 cell = make_uint4(0, 0, 0, 0);
+    if (e < count) cell = list[e];
+    const uint32_t s = cell.x, n = cell.y, c = cell.z;
+
+    uint32_t key = 0xffffffffu;
+    float2 q = make_float2(0.0f, 0.0f), w = make_float2(0.0f, 0.0f);
+    if (j < n) {
+        key = b.src[s + j] & kSrcMask;
+        q = b.pos[s + j];
+        w = b.vel[s + j];
+    }
+    // Source indices inside a cell are distinct; empty lanes keep their own
+    // position after the n particles.
+    uint32_t rank = 0;
+#pragma unroll
+    for (int k = 0; k < G; ++k) rank += __shfl(key, (int)gbase + k, 64) < key;
+    if (j >= n) rank = j;
+    const int to = (int)(gbase + rank) * 4;
+    float px = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(q.x)));
+    float py = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(q.y)));
+    float vx = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(w.x)));
+    float vy = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(w.y)));
+
+    // Lane gbase + r now holds rank r.
+    collide_group<G>(px, py, j, n < (uint32_t)kMaxInCell ? n : (uint32_t)kMaxInCell, gbase);
+
+    const bool has = j < n;
+    const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+    uint32_t dir = 10, nkey = 0;
+    if (has) {
+        finish_particle(p, px, py, vx, vy);
+        dir = direction(p, px, py, cx, cy, nkey);
+        b.pos_out[s + j] = make_float2(px, py);
+        b.vel_out[s + j] = make_float2(vx, vy);
+    }
+    // Lanes of this group going the same way.
+    unsigned long long same = 0;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+        const unsigned long long m = __ballot(dir == (uint32_t)d);
+        same = dir == (uint32_t)d ? m : same;
+    }
+    const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
+    same &= gmask;
+    const bool near = dir < 9;
+    const uint32_t rin = (uint32_t)__popcll(same & lanemask_lt());
+    const uint32_t leader = near ? (uint32_t)__builtin_ctzll(same) : lane;
+    uint32_t got = 0;
+    if (has && (!near || rin == 0))
+        got = atomicAdd(&b.cnt[nkey], near ? (uint32_t)__popcll(same) : 1u);
+    const uint32_t base = (uint32_t)__shfl((int)got, (int)leader, 64);
+    if (has) {
+        const uint32_t slot = near ? (base + rin) | (rin == 0 ? kRunFlag : 0u) : (got | kRunFlag);
+        b.kslot[s + j] = make_uint2(nkey, slot);
     }
 }
 
@@ -798,39 +878,61 @@ __device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t 
     }
 }
 
-// Cells queued by collide_kernel: wave-list cells (one wavefront each),
-// cells of 2..8 particles (one lane each, registers, one atomic per
-// neighbour direction), then block-list cells (over kWaveSort particles:
-// block-wide LDS sort, or merge sort past kLdsSort).
+// Work items [start, stop) of group class k (G = 2^(k+1) lanes per cell)
+// that fall to wavefront gwave: items gwave + i * nwaves.
+template <int G>
+__device__ __forceinline__ void group_class(const FrameParams &p, const FrameBuffers &b, int k,
+                                            uint32_t start, uint32_t stop, uint32_t count,
+                                            uint32_t gwave, uint32_t nwaves) {
+    uint32_t v = gwave;
+    if (v < start) v += (start - v + nwaves - 1) / nwaves * nwaves;
+    for (; v < stop; v += nwaves)
+        group_cells<G>(p, b, b.cells + b.class_off[k], count, (v - start) * (64 / G));
+}
+
+// Cells queued by collide_kernel.  Wavefront work items are numbered: first
+// the class-6 cells (one wavefront each, the longest chains), then the group
+// classes from 64 lanes per cell down to 2; every wavefront takes items
+// gwave, gwave + nwaves, ...  Class-7 cells (over kWaveSort particles) follow,
+// one block each (block-wide LDS sort, or merge sort past kLdsSort).
 __global__ __launch_bounds__(kCollideBlock) void full_cells_kernel(FrameParams p, FrameBuffers b) {
     __shared__ WaveLds lds[kWavesPerBlock];
     static_assert(sizeof(WaveLds) * kWavesPerBlock >= kLdsSort * sizeof(unsigned long long),
                   "the block path reuses the wave regions");
-    // Wave-list cells (the long per-cell chains) go to waves from the bottom
-    // of the grid; small cells to lanes counted from the top, so a wave
-    // holding a full cell takes small cells only once the full list has run
-    // out of waves.
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t gwave = blockIdx.x * kWavesPerBlock + wave;
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-    const uint32_t nw = min(b.full_count[0], b.full_cap);
-    for (uint32_t e = gwave; e < nw; e += nwaves) {
-        const uint4 cell = b.full_wave[e];
+    uint32_t cnt[kClasses];
+#pragma unroll
+    for (int k = 0; k < kClasses; ++k) cnt[k] = min(b.class_count[k], b.class_cap[k]);
+    // end[k]: one past the last work item of group class k (items of class
+    // 6 come first).
+    uint32_t end[kGroupClasses];
+    {
+        uint32_t at = cnt[kWaveClass];
+#pragma unroll
+        for (int k = kGroupClasses - 1; k >= 0; --k) {
+            const uint32_t per = 32u >> k;  // cells per wavefront: 64 / 2^(k+1)
+            at += (cnt[k] + per - 1) / per;
+            end[k] = at;
+        }
+    }
+    for (uint32_t v = gwave; v < cnt[kWaveClass]; v += nwaves) {
+        const uint4 cell = b.cells[b.class_off[kWaveClass] + v];
         wave_cell(p, b, cell.x, cell.y, cell.z, lds[wave]);
     }
-    const uint32_t ns = min(b.full_count[2], b.small_cap);
-    const uint32_t nlanes = nwaves * 64;
-    for (uint32_t e = (nwaves - 1 - gwave) * 64 + (threadIdx.x & 63); e < ns; e += nlanes) {
-        const uint4 cell = b.full_small[e];
-        if (cell.y == 2) small_cell<2, true>(p, b, cell.z, cell.x, cell.y);
-        else if (cell.y <= 4) small_cell<4, true>(p, b, cell.z, cell.x, cell.y);
-        else small_cell<8, true>(p, b, cell.z, cell.x, cell.y);
-    }
-    const uint32_t nb = min(b.full_count[1], b.full_cap);
+    // One loop per group class keeps each variant's registers to itself.
+    group_class<64>(p, b, 5, cnt[kWaveClass], end[5], cnt[5], gwave, nwaves);
+    group_class<32>(p, b, 4, end[5], end[4], cnt[4], gwave, nwaves);
+    group_class<16>(p, b, 3, end[4], end[3], cnt[3], gwave, nwaves);
+    group_class<8>(p, b, 2, end[3], end[2], cnt[2], gwave, nwaves);
+    group_class<4>(p, b, 1, end[2], end[1], cnt[1], gwave, nwaves);
+    group_class<2>(p, b, 0, end[1], end[0], cnt[0], gwave, nwaves);
+    const uint32_t nb = cnt[kBlockClass];
     if (nb == 0) return;
     __syncthreads();
     for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
-        const uint4 cell = b.full_block[e];
+        const uint4 cell = b.cells[b.class_off[kBlockClass] + e];
         big_cell<true>(p, b, cell.x, cell.y, reinterpret_cast<unsigned long long *>(lds));
     }
 }
@@ -1036,10 +1138,9 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
     }
     if (tile == 0 && tid == 0) {
         b.idx[0] = 0;
-        b.full_count[0] = 0;  // the full-cell kernel of this frame has finished
-        b.full_count[1] = 0;
-        b.full_count[2] = 0;
     }
+    if (tile == 0 && tid < (uint32_t)kClasses)
+        b.class_count[tid] = 0;  // the full-cell kernel of this frame has finished
     if (tile == b.scan_tiles - 1 && tid == 0) b.idx[p.ncell + 1] = s_prefix + agg;
 }
 
