@@ -30,22 +30,25 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 METRIC = "particle-steps/s & %HBM roofline @1M"
 
 
-def algorithmic_bytes(n: int, cells: int, n_full: int) -> dict:
+def algorithmic_bytes(n: int, cells: int, split: dict) -> dict:
     """Bytes each kernel must move per launch.  Per particle: the collide
     pass reads position+velocity (16) and writes them (16) plus the cell key
-    (4) -- in collide_kernel for cells of <= 16 particles, in the full-cell
-    kernel for the n_full particles of fuller cells; the scatter reads key
-    (4) + position+velocity (16) and writes them (16).  Per cell (C + 2
-    offsets): collide_kernel reads one offset (4); the scan reads a count
-    (4), writes an offset (4) and clears the count (4)."""
+    (4) -- in collide_kernel for one-particle cells, in cells_kernel for the
+    rest; the scatter reads key (4) + position+velocity (16) and writes them
+    (16).  Per cell (C + 2 offsets): collide_kernel reads one offset (4); the
+    scan reads a count (4), writes an offset (4) and clears the count (4)."""
     c2 = cells + 2
-    return {"collide": 36 * (n - n_full) + 4 * c2, "full_cells": 36 * n_full,
+    return {"collide": 36 * split["single"] + 4 * c2, "cells": 36 * (split["wide"] + split["group"]),
             "scan": 12 * c2, "scatter": 36 * n}
 
 
-def particles_in_full_cells(idx: np.ndarray) -> int:
+def particles_by_kernel(idx: np.ndarray) -> dict:
+    """Particles in one-particle cells, in cells of 65..1024 (the wide
+    path) and in the other cells of two or more."""
     cnt = np.diff(idx[1:].astype(np.int64))
-    return int(cnt[cnt > 16].sum())
+    single = int(cnt[cnt == 1].sum())
+    wide = int(cnt[(cnt >= 65) & (cnt <= 1024)].sum())
+    return {"single": single, "wide": wide, "group": int(cnt.sum()) - single - wide}
 
 
 def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
@@ -120,11 +123,12 @@ def main() -> int:
     print(f"[bench] timed {args.steps} frames in {elapsed * 1e3:.2f} ms", file=sys.stderr, flush=True)
 
     # Per-kernel durations: HIP events around each launch, plain launches.
-    full_before = particles_in_full_cells(world.read()[0])
+    split_before = particles_by_kernel(world.read()[0])
     ms = world.step_profiled(args.profile_steps)
-    full_after = particles_in_full_cells(world.read()[0])
+    split_after = particles_by_kernel(world.read()[0])
     avg_us = {k: 1e3 * v / args.profile_steps for k, v in ms.items()}
-    bytes_ = algorithmic_bytes(n, cells, (full_before + full_after) // 2)
+    split = {k: (split_before[k] + split_after[k]) // 2 for k in split_before}
+    bytes_ = algorithmic_bytes(n, cells, split)
     dom = max(avg_us, key=avg_us.get)
     achieved = bytes_[dom] / (avg_us[dom] * 1e-6) / 1e9
     step_bytes = sum(bytes_.values())
@@ -176,7 +180,7 @@ def main() -> int:
             "traffic": traffic,
             "algorithmic_bytes_per_launch": bytes_[dom],
             "kernel_avg_us": avg_us,
-            "particles_in_full_cells": [full_before, full_after],
+            "particles_by_kernel": [split_before, split_after],
             "step_algorithmic_bytes": step_bytes,
             "step_achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
             "step_frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,

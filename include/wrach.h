@@ -80,8 +80,9 @@ typedef struct wrach_device_view {
 } wrach_device_view;
 
 /* Number of kernels timed by wrach_step_profiled, in launch order:
- * 0 collide/integrate/key (cells of <= 16 particles), 1 full cells,
- * 2 exclusive scan, 3 stable scatter. */
+ * 0 collide/integrate/key (one-particle cells; fuller cells are queued),
+ * 1 the queued cells (two or more particles), 2 exclusive scan, 3 stable
+ * scatter. */
 #define WRACH_NUM_FRAME_KERNELS 4
 
 /* WrachConfig::default (config_app.rs:26-35): 480x352, cell size 3

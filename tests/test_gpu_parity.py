@@ -136,10 +136,13 @@ def test_single_and_coincident_particles(oracle):
     run_parity(oracle, cfg, pos, vel, 20, label="coincident+walls")
 
 
-@pytest.mark.parametrize("per_cell", [2, 3, 4, 5, 8, 9, 10, 16, 17, 40, 300, 2048, 2049, 5000])
+@pytest.mark.parametrize("per_cell", [2, 3, 4, 5, 8, 9, 10, 16, 17, 33, 40, 64, 65, 300, 576, 577,
+                                      1024, 1025, 2048, 2049, 5000])
 def test_full_cells(oracle, per_cell):
-    """Register paths (<=16), block-cooperative LDS sort (<=2048) and the
-    merge sort (>2048), all with the 9-particle collision cap."""
+    """Every size class: lane groups (2..64), the wide path (65..576), the
+    block path for 577..1024 (class 6 past the wide registers) and above
+    1024 (LDS sort up to 2048, merge sort past it), all with the 9-particle
+    collision cap."""
     r = np.random.default_rng(per_cell)
     cells = [(3, 4), (10, 2), (11, 2), (20, 7)]
     pos, vel = [], []
@@ -185,6 +188,37 @@ def test_far_movers_across_rows(oracle, per_cell):
     vel[::3] = (W + 5.0, -1.0)
     cfg = WrachConfig(dimensions=(W, H), cell_size=1)
     run_parity(oracle, cfg, pos, vel, 6, label=f"far movers x{per_cell}")
+
+
+@pytest.mark.parametrize("runs", [100, 300])
+def test_many_runs_in_one_cell(oracle, runs):
+    """A cell fed by `runs` far movers from distinct source cells (a run of
+    one each) plus a near run: frame 2 orders it by runs (100, within the
+    wide path's 256) or through the block-wide sort (300)."""
+    r = np.random.default_rng(runs)
+    W, H = 420, 40
+    tx, ty = 10, 20
+    d = np.arange(2, runs + 2, dtype=np.float64)
+    pos = np.stack([tx + d + r.uniform(0.3, 0.7, runs), np.full(runs, ty) + r.uniform(0.3, 0.7, runs)], 1)
+    vel = np.stack([-d, np.zeros(runs)], 1)
+    near = np.stack([np.full(80, tx + 0.5) + r.uniform(-0.4, 0.4, 80), np.full(80, ty + 0.5)], 1)
+    pos = f32(np.concatenate([pos, near, r.uniform(0, [W, H], (200, 2))]))
+    vel = f32(np.concatenate([vel, np.zeros((80, 2)), r.uniform(-1, 1, (200, 2))]))
+    cfg = WrachConfig(dimensions=(W, H), cell_size=1)
+    run_parity(oracle, cfg, pos, vel, 4, label=f"many runs x{runs}")
+
+
+def test_collapsed_columns(oracle):
+    """64x512 world, a jittered lattice of 64 x 256 particles under gravity:
+    the columns land on the floor as 64 cells of ~256 particles, Gauss-Seidel
+    pushes send particles several cells sideways (far movers), corner cells
+    collect many runs.  Exact every 25 frames for 400 frames."""
+    r = np.random.default_rng(11)
+    xs, ys = np.meshgrid(np.arange(64) + 0.5, np.arange(256) + 0.5)
+    pos = f32(np.stack([xs.ravel(), ys.ravel()], 1) + r.uniform(-0.1, 0.1, (64 * 256, 2)))
+    vel = f32(r.normal(0, 0.5, (64 * 256, 2)))
+    cfg = WrachConfig(dimensions=(64, 512), cell_size=1, gravity=(0.0, -0.1))
+    run_parity(oracle, cfg, pos, vel, 400, every=25, label="collapsed columns")
 
 
 def test_errors_leave_state_usable(oracle):
@@ -244,7 +278,7 @@ def test_profiled_step_matches_and_times(oracle):
     with World(wl.config) as w:
         w.add_particles(wl.pos, wl.vel)
         ms = w.step_profiled(5)
-        assert set(ms) == {"collide", "full_cells", "scan", "scatter"}
+        assert set(ms) == {"collide", "cells", "scan", "scatter"}
         assert all(v > 0 for v in ms.values())
         assert_same_state(w.read(), oracle.step(P, *ref, nsteps=5), "profiled")
 
@@ -270,18 +304,22 @@ def test_lattice_1m_full_size(oracle):
 
 
 def test_lattice_1m_rollout_and_invariants(oracle):
-    """300 frames: still exact against the oracle, plus the size-independent
-    invariants (particle count, sorted offsets, every particle in its cell,
-    inside the walls, |v| <= 1)."""
+    """800 frames, past the collapse onto the floor (about 2,050 cells of
+    ~460 particles from frame ~500 on): still exact against the oracle at
+    frames 300 and 800, plus the size-independent invariants (particle count,
+    sorted offsets, every particle in its cell, inside the walls, |v| <= 1)."""
     wl = workloads.lattice_1m()
-    say(f"[rollout] {wl.name} 300 frames")
+    say(f"[rollout] {wl.name} 800 frames")
     P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
     with World(wl.config) as w:
         w.add_particles(wl.pos, wl.vel)
         w.step(300)
+        ref = oracle.step(P, *ref, nsteps=300, nthreads=THREADS)
+        assert_same_state(w.read(), ref, "lattice 300")
+        w.step(500)
         idx, pos, vel = w.read()
-    ref = oracle.step(P, *ref, nsteps=300, nthreads=THREADS)
-    assert_same_state((idx, pos, vel), ref, "lattice 300")
+    ref = oracle.step(P, *ref, nsteps=500, nthreads=THREADS)
+    assert_same_state((idx, pos, vel), ref, "lattice 800")
     W, H = wl.config.dimensions
     gx = W + 1
     assert idx[0] == 0 and idx[1] == 0 and idx[-1] == wl.n

@@ -20,7 +20,7 @@ WRACH_E_DEVICE = -5
 WRACH_E_BUFFER = -6
 WRACH_E_STATE = -7
 NUM_FRAME_KERNELS = 4
-FRAME_KERNEL_NAMES = ("collide", "full_cells", "scan", "scatter")
+FRAME_KERNEL_NAMES = ("collide", "cells", "scan", "scatter")
 
 # Every symbol include/wrach.h declares.
 EXPORTS = (

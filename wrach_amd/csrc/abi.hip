@@ -100,7 +100,7 @@ void free_graphs(wrach_ctx *c) {
 
 int enqueue_frame(wrach_ctx *c) {
     WRACH_HIP(wrach::launch_collide(c->fp, c->buf, c->stream));
-    WRACH_HIP(wrach::launch_full_cells(c->fp, c->buf, c->stream));
+    WRACH_HIP(wrach::launch_cells(c->fp, c->buf, c->stream));
     WRACH_HIP(wrach::launch_scan(c->fp, c->buf, c->stream));
     WRACH_HIP(wrach::launch_scatter(c->fp, c->buf, c->stream));
     return WRACH_OK;
@@ -266,14 +266,16 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
         b.class_cap[k] = cap / wrach::kClassMinN[k] + 64;
         ncells_total += b.class_cap[k];
     }
-    {
-        // Enough wavefronts for one pass over the largest wave or group list
-        // (every block resident at once at kFullGridMax).
-        const uint32_t waves = b.class_cap[0] * 2 / 64 + b.class_cap[wrach::kWaveClass];
-        const uint32_t want = (waves + wrach::kWavesPerBlock - 1) / wrach::kWavesPerBlock;
-        b.full_grid = want < wrach::kFullGridMax ? want : wrach::kFullGridMax;
-    }
     auto bail = [&](int rc) { release(c); return rc; };
+    // Grids: the blocks the device holds at once (the kernels loop over
+    // their work), or fewer for a small world.
+    {
+        const uint32_t want = (b.class_cap[0] * 2 / 64 + wrach::kWavesPerBlock) / wrach::kWavesPerBlock;
+        const uint32_t resident = wrach::resident_blocks_cells();
+        if (resident == 0) return bail(fail(WRACH_E_DEVICE, "occupancy query failed"));
+        b.cells_grid = want < resident ? want : resident;
+        if (b.cells_grid == 0) b.cells_grid = 1;
+    }
 #define WRACH_HIP_C(expr)                                                                   \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \
@@ -396,7 +398,7 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
     for (uint32_t f = 0; f < nframes && rc == WRACH_OK; ++f) {
         hipEvent_t *fe = &ev[2 * K * (size_t)f];
         hipError_t (*const launch[K])(const wrach::FrameParams &, const wrach::FrameBuffers &,
-                                      hipStream_t) = {wrach::launch_collide, wrach::launch_full_cells,
+                                      hipStream_t) = {wrach::launch_collide, wrach::launch_cells,
                                                       wrach::launch_scan, wrach::launch_scatter};
         hipError_t e = hipSuccess;
         for (int k = 0; k < K && e == hipSuccess; ++k) {

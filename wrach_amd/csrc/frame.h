@@ -39,7 +39,7 @@ struct FrameBuffers {
     uint32_t class_off[8];  // first entry of each class's region in `cells`
     uint32_t class_cap[8];  // entries of each region
     uint32_t scan_tiles;
-    uint32_t full_grid;   // blocks of the full-cell kernel
+    uint32_t cells_grid;  // blocks of the queued-cell kernel
 };
 
 constexpr int kCollideBlock = 256;
@@ -50,20 +50,21 @@ constexpr int kScatterBlock = 256;
 constexpr int kMaxInCell = 9;            // cell.rs:29-30
 constexpr int kRegCell = 16;             // cells up to this size sort in registers
 constexpr int kLdsSort = 2048;           // keys per LDS bitonic sort (block path)
-constexpr int kWaveSort = 1024;          // keys per wave-level LDS bitonic sort
 constexpr int kWavesPerBlock = kCollideBlock / 64;
-constexpr uint32_t kFullGridMax = 1024;  // blocks of the full-cell kernel
 
 // Size classes of queued cells.  Classes 0..5 hold cells of 2, 3-4, 5-8,
 // 9-16, 17-32 and 33-64 particles, processed by groups of 2..64 lanes with
-// one particle per lane; class 6 (65..kWaveSort) takes a wavefront per cell,
-// class 7 a block per cell.
+// one particle per lane; class 6 (65..kWideMax) takes a wavefront per cell
+// with its particles in registers; class 7 (more) a block per cell.
+constexpr int kWideChunks = 9;            // 64-particle chunks of a wide-path wavefront
+constexpr uint32_t kWideMax = 64 * kWideChunks;
 constexpr int kClasses = 8;
 constexpr int kGroupClasses = 6;
 constexpr int kWaveClass = 6;
 constexpr int kBlockClass = 7;
 // Fewest particles a cell of each class holds (sizes the list regions).
-constexpr uint32_t kClassMinN[kClasses] = {2, 3, 5, 9, 17, 33, 65, kWaveSort + 1};
+constexpr uint32_t kClassMinN[kClasses] = {2, 3, 5, 9, 17, 33, 65, kWideMax + 1};
+static_assert(sizeof(FrameBuffers::class_off) == kClasses * sizeof(uint32_t), "one region per class");
 
 inline uint32_t scan_tiles_for(uint32_t ncell) {
     return (ncell + kScanTile - 1) / kScanTile;
@@ -72,7 +73,9 @@ inline uint32_t scan_tiles_for(uint32_t ncell) {
 // Launchers (kernels.hip).  Each enqueues on `stream` and returns the launch
 // status.
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
-hipError_t launch_full_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+// Blocks of the queued-cell kernel that the device holds at once (all CUs).
+uint32_t resident_blocks_cells();
 hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);

@@ -6,15 +6,18 @@
 //                      and limit (shaders/physics/src/particle.rs:40-82),
 //                      write pos_out/vel_out, compute the new cell key
 //                      (particles_per_cell.wgsl:14-27), claim a slot in the
-//                      new cell with one atomic.  Fuller cells are queued.
-//   full_cells_kernel  the queued cells: one lane per cell of 2..8 particles,
-//                      one wavefront per cell of 9..kWaveSort, one block per
-//                      larger cell.  Each restores the canonical order (by
-//                      `src`), separates the first 9 particles (Gauss-Seidel,
-//                      particles.rs:62-93), finishes every particle (the
+//                      new cell with one atomic.  Fuller cells are queued by
+//                      size class.
+//   cells_kernel       the queued cells: 65..kWideMax particles a wavefront
+//                      each plus one Gauss-Seidel wavefront per three cells;
+//                      2..64 particles in lane groups of 2..64 (one particle
+//                      per lane); one block per larger cell.  Every path
+//                      restores the canonical order (by
+//                      `src`), separate the first 9 particles (Gauss-Seidel,
+//                      particles.rs:62-93), finish every particle (the
 //                      overflow ones without collisions, cell.rs:79-95),
-//                      writes pos_out/vel_out at the canonical index and
-//                      claims slots with one atomic per neighbour direction.
+//                      write pos_out/vel_out at the canonical index and
+//                      claim slots with one atomic per neighbour direction.
 //   scan_kernel        single-pass exclusive scan of the per-cell counts with
 //                      decoupled look-back; writes [0, start_0.., N] and clears
 //                      the counts for the next frame (prefix_sum.wgsl +
@@ -142,8 +145,18 @@ __device__ __forceinline__ uint32_t direction(const FrameParams &p, float x, flo
     return (uint32_t)((dy + 1) * 3 + (dx + 1));
 }
 
+// Lane index, opaque to the optimiser: values derived from it (lane masks,
+// Gauss-Seidel partners) are then recomputed where they are used instead of
+// being hoisted out of the work loops and held in registers for the whole
+// kernel, which spills scalar registers.
+__device__ __forceinline__ uint32_t lane_id() {
+    uint32_t l = threadIdx.x & 63;
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
-    return (1ull << (threadIdx.x & 63)) - 1;
+    return (1ull << lane_id()) - 1;
 }
 
 // Orders LDS accesses of one wavefront: its LDS instructions execute in
@@ -312,7 +325,7 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
 }
 
 // ---------------------------------------------------------------------------
-// Cells of more than kWaveSort particles: the whole block cooperates.
+// Cells of more than kWideMax particles: the whole block cooperates.
 
 // Ascending bitonic sort of P (a power of two, <= kLdsSort) 64-bit keys in
 // LDS.  Callers synchronise before the call; the last stage ends with a
@@ -365,7 +378,7 @@ __device__ void merge_block(const unsigned long long *x, uint32_t nx,
 // LDS-sorted chunks, then block merges ping-ponging between the cell's own
 // ranges of kslot and sortbuf (both unused by any other cell).  Returns the
 // buffer holding the result.
-__device__ const unsigned long long *sort_large_cell(const FrameBuffers &b, uint32_t s, uint32_t n,
+__device__ __forceinline__ const unsigned long long *sort_large_cell(const FrameBuffers &b, uint32_t s, uint32_t n,
                                                      unsigned long long *sk) {
     const uint32_t tid = threadIdx.x;
     unsigned long long *a = reinterpret_cast<unsigned long long *>(b.kslot) + s;
@@ -395,9 +408,13 @@ __device__ const unsigned long long *sort_large_cell(const FrameBuffers &b, uint
     return from;
 }
 
+template <int G>
+__device__ __forceinline__ void collide_group(float &x, float &y, uint32_t r, uint32_t m,
+                                              uint32_t gbase);
+
 template <bool STEP>
-__device__ void big_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s, uint32_t n,
-                         unsigned long long *sk) {
+__device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s,
+                                         uint32_t n, uint32_t c, unsigned long long *sk) {
     const uint32_t tid = threadIdx.x;
     const unsigned long long *perm;
     if (n <= (uint32_t)kLdsSort) {
@@ -412,34 +429,75 @@ __device__ void big_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s
         perm = sort_large_cell(b, s, n, sk);
     }
     if (STEP) {
-        // Ranks 0..8 collide in one lane; perm[r] and kslot[s+r] are only
-        // touched by the thread that owns rank r.
-        if (tid == 0) {
-            float px[kMaxInCell], py[kMaxInCell], vx[kMaxInCell], vy[kMaxInCell];
-#pragma unroll
-            for (int q = 0; q < kMaxInCell; ++q) {
-                const uint32_t a = (uint32_t)perm[q];
-                const float2 pq = b.pos[s + a];
-                const float2 vq = b.vel[s + a];
-                px[q] = pq.x; py[q] = pq.y; vx[q] = vq.x; vy[q] = vq.y;
+        // Pass 1 over 256-rank chunks in canonical order: finish, write at
+        // s + r, and rank each particle among the cell's particles going the
+        // same way (ballots per wave, offsets across the block's waves, running
+        // totals across chunks).  Ranks 0..8 are separated first by lanes
+        // 0..8 of wave 0 (collide_group).  Per rank, `tmp` keeps (direction,
+        // rank in it), or for a farther mover (key | bit 31, its slot); it is
+        // whichever of kslot / sortbuf does not hold the order.
+        __shared__ uint32_t s_wcnt[kWavesPerBlock][9], s_run[9], s_base[9];
+        const uint32_t lane = tid & 63, wave = tid >> 6;
+        const unsigned long long lt = lanemask_lt();
+        const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+        uint2 *tmp = reinterpret_cast<const void *>(perm) == reinterpret_cast<const void *>(b.kslot + s)
+                         ? reinterpret_cast<uint2 *>(b.sortbuf + s)
+                         : b.kslot + s;
+        if (tid < 9) s_run[tid] = 0;
+        for (uint32_t base = 0; base < n; base += kCollideBlock) {
+            const uint32_t r = base + tid;
+            float2 q = make_float2(0.0f, 0.0f), v = make_float2(0.0f, 0.0f);
+            if (r < n) {
+                const uint32_t a = (uint32_t)perm[r];
+                q = b.pos[s + a];
+                v = b.vel[s + a];
             }
-#pragma unroll
-            for (int i = 0; i < kMaxInCell; ++i)
-#pragma unroll
-                for (int j = i + 1; j < kMaxInCell; ++j) collide_pair(px[i], py[i], px[j], py[j]);
-#pragma unroll
-            for (int q = 0; q < kMaxInCell; ++q) {
-                finish_particle(p, px[q], py[q], vx[q], vy[q]);
-                emit(p, b, s + q, px[q], py[q], vx[q], vy[q]);
+            if (base == 0 && wave == 0) collide_group<64>(q.x, q.y, lane, kMaxInCell, 0);
+            uint32_t dir = 10, key = 0;
+            if (r < n) {
+                finish_particle(p, q.x, q.y, v.x, v.y);
+                b.pos_out[s + r] = q;
+                b.vel_out[s + r] = v;
+                dir = direction(p, q.x, q.y, cx, cy, key);
             }
+            uint32_t rin = 0;
+#pragma unroll
+            for (int d = 0; d < 9; ++d) {
+                const unsigned long long m = __ballot(dir == (uint32_t)d);
+                if (dir == (uint32_t)d) rin = (uint32_t)__popcll(m & lt);
+                if (lane == 0) s_wcnt[wave][d] = (uint32_t)__popcll(m);
+            }
+            __syncthreads();
+            if (dir < 9) {
+                uint32_t off = s_run[dir];
+                for (uint32_t w = 0; w < wave; ++w) off += s_wcnt[w][dir];
+                tmp[r] = make_uint2(dir, off + rin);
+            } else if (dir == 9) {
+                const uint32_t slot = atomicAdd(&b.cnt[key], 1u);
+                tmp[r] = make_uint2(key | kRunFlag, slot | kRunFlag);
+            }
+            __syncthreads();
+            if (tid < 9) {
+                uint32_t t = 0;
+                for (uint32_t w = 0; w < (uint32_t)kWavesPerBlock; ++w) t += s_wcnt[w][tid];
+                s_run[tid] += t;
+            }
+            __syncthreads();
         }
-        for (uint32_t r = kMaxInCell + tid; r < n; r += kCollideBlock) {
-            const uint32_t a = (uint32_t)perm[r];
-            const float2 pq = b.pos[s + a];
-            const float2 vq = b.vel[s + a];
-            float px = pq.x, py = pq.y, vx = vq.x, vy = vq.y;
-            finish_particle(p, px, py, vx, vy);
-            emit(p, b, s + r, px, py, vx, vy);
+        // One atomic per occupied direction, then the slots.
+        if (tid < 9) {
+            const uint32_t tx = cx + tid % 3 - 1, ty = cy + tid / 3 - 1;
+            s_base[tid] = s_run[tid] ? atomicAdd(&b.cnt[ty * p.gx + tx], s_run[tid]) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t r = tid; r < n; r += kCollideBlock) {
+            const uint2 t = tmp[r];
+            if (t.x & kRunFlag) {
+                b.kslot[s + r] = make_uint2(t.x & kSrcMask, t.y);
+            } else {
+                const uint32_t key = (cy + t.x / 3 - 1) * p.gx + (cx + t.x % 3 - 1);
+                b.kslot[s + r] = make_uint2(key, (s_base[t.x] + t.y) | (t.y == 0 ? kRunFlag : 0u));
+            }
         }
     } else {
         for (uint32_t r = tid; r < n; r += kCollideBlock) {
@@ -463,17 +521,17 @@ __device__ __forceinline__ void clamp_range(const FrameParams &p, uint32_t s, ui
 
 // Four consecutive cells per lane (one 128-bit offset load).  Single-particle
 // cells are finished here, their slot atomics in flight together; every
-// fuller cell is queued for full_cells_kernel, which spreads them over the
+// fuller cell is queued for the cell kernels, which spread them over the
 // whole GPU (after a few hundred frames under gravity they crowd into a few
 // rows, which would otherwise land on a few CUs).  One global atomic per
-// block and list.
+// block and class.
 constexpr int kCellsPerLane = 4;
 
 // Size class of a cell of n particles (kClasses for none: 0 or 1 particle).
 // For 2 <= n <= 64 the class is ceil(log2(n)) - 1.
 __device__ __forceinline__ uint32_t size_class(uint32_t n) {
     if (n < 2) return (uint32_t)kClasses;
-    if (n > (uint32_t)kWaveSort) return (uint32_t)kBlockClass;
+    if (n > kWideMax) return (uint32_t)kBlockClass;
     const uint32_t k = 31u - (uint32_t)__builtin_clz(n - 1);
     return k < (uint32_t)kWaveClass ? k : (uint32_t)kWaveClass;
 }
@@ -554,20 +612,15 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
     }
 }
 
-// Wave-private LDS of full_cells_kernel.  The bitonic fallback uses `sort`;
-// the run path uses `perm` and `aux` (source indices, then new keys).
+// Wave-private LDS of a cell wave of the wide path (below).
+constexpr int kWideRuns = 256;  // runs per cell ordered by runs (more: pairwise ranks)
 struct WaveLds {
-    union {
-        unsigned long long sort[kWaveSort];
-        struct {
-            uint32_t perm[kWaveSort];  // perm[rank] = stored position
-            uint32_t aux[kWaveSort];   // src by position, then per rank: direction and
-                                       // rank in it (bit 31 set), or a far mover's key
-        } r;
-    };
-    uint32_t head_key[64];  // first source index of each run
-    uint32_t head_pos[64];  // stored position of each run's start
-    uint32_t run_off[64];   // canonical rank of each run's start
+    uint32_t code[kWideMax];          // per rank >= 9: direction (bit 31 set); 0 for a far mover
+    uint32_t head_key[kWideRuns];     // first source index of each run
+    uint32_t head_pos[kWideRuns];     // stored position of each run's start
+    uint32_t run_off[kWideRuns];      // canonical rank of each run's start minus its position
+    uint2 head_kl[kWideRuns];         // (first source index, length) of each run
+    uint2 dir_tbl[9];                 // per direction: (new cell key, first slot)
 };
 
 // Gauss-Seidel on the first m = min(n, 9) particles of a cell held by lanes
@@ -582,7 +635,7 @@ template <int G>
 __device__ __forceinline__ void collide_group(float &x, float &y, uint32_t r, uint32_t m,
                                               uint32_t gbase) {
     constexpr int M = G < kMaxInCell ? G : kMaxInCell;
-    const int lane = (int)(threadIdx.x & 63);
+    const int lane = (int)lane_id();
     const int ri = (int)r, mi = (int)m;
 #pragma unroll
     for (int t = 1; t <= 2 * M - 3; ++t) {
@@ -617,24 +670,21 @@ __device__ __forceinline__ void collide_group(float &x, float &y, uint32_t r, ui
     }
 }
 
-__device__ __forceinline__ void collide_nine_lanes(float &x, float &y) {
-    collide_group<64>(x, y, threadIdx.x & 63, kMaxInCell, 0);
-}
-
 // Cells of 2 <= n <= G particles (G = 2..64), 64 / G cells per wavefront,
 // one particle per lane.  Each group of G lanes:
 //  1. loads its cell's particles (lane j: stored position j);
 //  2. ranks them by source index (the canonical order) and moves rank r to
 //     lane gbase + r;
-//  3. separates the first min(n, 9) (collide_group);
-//  4. finishes every particle, writes it at its canonical index;
-//  5. claims slots: one atomic per occupied neighbour direction (by the
-//     direction's first lane), a run of consecutive slots in canonical order;
-//     a farther mover takes a slot of its own.
+//  3. hands them to ranked_group (below).
+template <int G>
+__device__ __forceinline__ void ranked_group(const FrameParams &p, const FrameBuffers &b,
+                                             float px, float py, float vx, float vy, uint32_t j,
+                                             uint32_t n, uint32_t s, uint32_t c, uint32_t gbase);
+
 template <int G>
 __device__ void group_cells(const FrameParams &p, const FrameBuffers &b, const uint4 *list,
                             uint32_t count, uint32_t first) {
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lane = lane_id();
     const uint32_t j = lane % G;
     const uint32_t gbase = lane - j;
     const uint32_t e = first + lane / G;
@@ -662,6 +712,20 @@ __device__ void group_cells(const FrameParams &p, const FrameBuffers &b, const u
     float vy = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(w.y)));
 
     // Lane gbase + r now holds rank r.
+    ranked_group<G>(p, b, px, py, vx, vy, j, n, s, c, gbase);
+}
+
+// The particles of ranks 0..n-1 of a cell segment that starts at canonical
+// index s, held by lanes gbase + r of a group of G lanes: separate the first
+// min(n, 9) (collide_group), finish every particle, write it at s + r, and
+// claim slots: one atomic per occupied neighbour direction (by the
+// direction's first lane) reserves a run of consecutive slots in canonical
+// order; a farther mover takes a slot of its own.
+template <int G>
+__device__ __forceinline__ void ranked_group(const FrameParams &p, const FrameBuffers &b,
+                                             float px, float py, float vx, float vy, uint32_t j,
+                                             uint32_t n, uint32_t s, uint32_t c, uint32_t gbase) {
+    const uint32_t lane = lane_id();
     collide_group<G>(px, py, j, n < (uint32_t)kMaxInCell ? n : (uint32_t)kMaxInCell, gbase);
 
     const bool has = j < n;
@@ -695,186 +759,264 @@ __device__ void group_cells(const FrameParams &p, const FrameBuffers &b, const u
     }
 }
 
-// One cell of 8 < n <= kWaveSort particles, processed by one wavefront.
-//  1. Canonical order: runs (flagged in src) ordered by their first source
-//     index -> perm[rank] = stored position; more than 64 runs fall back to
-//     a bitonic sort of (src, position).
-//  2. Gauss-Seidel on ranks 0..8 across lanes 0..8 (collide_nine_lanes).
-//  3. Every rank finished and written at its canonical index; its rank
-//     among the cell's particles going the same way kept in LDS.
-//  4. One atomic per occupied direction reserves a contiguous, canonically
-//     ordered run in the new cell; a farther mover takes its own slot.
-// Loops work in batches of four 64-rank chunks so that four gathers per lane
-// are in flight at once.
-constexpr int kBatch = 4;
+// ---------------------------------------------------------------------------
+// Cells of 65..kWideChunks*64 particles, three per block: waves 0..2 take one
+// cell each, wave 3 separates the first nine particles of all three.
+//
+// Ranks 0..8 and ranks 9..n-1 of a cell claim their slots independently, each
+// direction's share of either part as its own run.  Both runs are ascending
+// in canonical index and every index of the first lies below every index of
+// the second, so in the new cell they are two runs that do not interleave,
+// which is all the order restoration of the next frame needs.  The cell
+// waves therefore never wait for the Gauss-Seidel chain.
 
-__device__ void wave_cell(const FrameParams &p, const FrameBuffers &b, uint32_t s, uint32_t n,
-                          uint32_t c, WaveLds &L) {
-    const uint32_t lane = threadIdx.x & 63;
-    const unsigned long long lt = lanemask_lt();
-    const unsigned long long le = lt | (1ull << lane);
-    uint32_t *perm = L.r.perm;
+constexpr int kWideCells = kWavesPerBlock - 1;
 
-    // 1a. Run starts: flagged particles, and position 0.  src kept in LDS.
-    constexpr int kSrcBatch = 2 * kBatch;
-    uint32_t runs = 0;
-    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kSrcBatch) {
-        uint32_t v[kSrcBatch];
+struct WideStage {
+    float4 q[kWideCells][kMaxInCell];  // ranks 0..8 of each cell: x, y, stored position (bits)
+    uint32_t on[kWideCells];           // nonzero: wave 3 handles ranks 0..8
+};
+
+// Cell wave, before the block barrier: loads in stored order, canonical rank
+// of every stored position (rk), ranks 0..8 staged for wave 3.  The ranks
+// come from the runs; a cell of more than kWideRuns runs is ranked pairwise
+// (source indices in a cell are distinct).
+__device__ __forceinline__ void wide_order(const FrameBuffers &b, uint32_t s, uint32_t n,
+                                           WaveLds &L, float4 *stage, float2 (&pp)[kWideChunks],
+                                           float2 (&vv)[kWideChunks],
+                                           uint32_t (&rk)[kWideChunks]) {
+    const uint32_t lane = lane_id();
+    const unsigned long long le = lanemask_lt() | (1ull << lane);
+    uint32_t sv[kWideChunks];
 #pragma unroll
-        for (int q = 0; q < kSrcBatch; ++q) {
-            const uint32_t i = b0 + 64 * q + lane;
-            v[q] = i < n ? b.src[s + i] : 0u;
-        }
-#pragma unroll
-        for (int q = 0; q < kSrcBatch; ++q) {
-            const uint32_t i = b0 + 64 * q + lane;
-            const bool start = i < n && (i == 0 || (v[q] & kRunFlag));
-            const unsigned long long m = __ballot(start);
-            if (i < n) L.r.aux[i] = v[q];
-            if (start) {
-                const uint32_t rid = runs + (uint32_t)__popcll(m & lt);
-                if (rid < 64) {
-                    L.head_key[rid] = v[q] & kSrcMask;
-                    L.head_pos[rid] = i;
-                }
-            }
-            runs += (uint32_t)__popcll(m);
+    for (int q = 0; q < kWideChunks; ++q) {
+        const uint32_t i = 64 * q + lane;
+        sv[q] = 0;
+        pp[q] = vv[q] = make_float2(0.0f, 0.0f);
+        if (i < n) {
+            sv[q] = b.src[s + i];
+            pp[q] = b.pos[s + i];
+            vv[q] = b.vel[s + i];
         }
     }
-    wave_sync();
-    if (runs <= 64) {
-        // 1b. Rank of each run = particles in runs with a smaller first source.
-        uint32_t hk = 0xffffffffu, len = 0;
-        if (lane < runs) {
-            hk = L.head_key[lane];
-            const uint32_t end = lane + 1 < runs ? L.head_pos[lane + 1] : n;
-            len = end - L.head_pos[lane];
+    // Runs: flagged particles and position 0 start one.
+    uint32_t runs = 0, rid[kWideChunks];
+#pragma unroll
+    for (int q = 0; q < kWideChunks; ++q) {
+        const uint32_t i = 64 * q + lane;
+        const bool start = i < n && (i == 0 || (sv[q] & kRunFlag));
+        const unsigned long long m = __ballot(start);
+        rid[q] = runs + (uint32_t)__popcll(m & le) - 1;
+        if (start && rid[q] < (uint32_t)kWideRuns) {
+            L.head_key[rid[q]] = sv[q] & kSrcMask;
+            L.head_pos[rid[q]] = i;
         }
-        uint32_t off = 0;
-        for (uint32_t m = 0; m < runs; ++m) {
-            const uint32_t km = __shfl(hk, (int)m, 64);
-            const uint32_t lm = __shfl(len, (int)m, 64);
-            if (km < hk) off += lm;
-        }
-        if (lane < runs) L.run_off[lane] = off;
+        runs += (uint32_t)__popcll(m);
+    }
+    if (runs <= (uint32_t)kWideRuns) {
         wave_sync();
-        // 1c. perm[rank] = stored position.
-        uint32_t seen = 0;
-        for (uint32_t base = 0; base < n; base += 64) {
-            const uint32_t i = base + lane;
-            const uint32_t v = i < n ? L.r.aux[i] : 0u;
-            const bool start = i < n && (i == 0 || (v & kRunFlag));
-            const unsigned long long m = __ballot(start);
-            if (i < n) {
-                const uint32_t rid = seen + (uint32_t)__popcll(m & le) - 1;
-                perm[L.run_off[rid] + (i - L.head_pos[rid])] = i;
+        // Lane l owns runs l, l + 64, ...
+        constexpr int kOwn = kWideRuns / 64;
+#pragma unroll
+        for (int k = 0; k < kOwn; ++k) {
+            const uint32_t r = lane + 64 * k;
+            if (r < runs) {
+                const uint32_t end = r + 1 < runs ? L.head_pos[r + 1] : n;
+                L.head_kl[r] = make_uint2(L.head_key[r], end - L.head_pos[r]);
             }
-            seen += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        // Canonical rank of a run's start: particles in runs with a smaller
+        // first source index.  Stored as rank - position so that
+        // rank = run_off + i.
+        uint32_t hk[kOwn], off[kOwn];
+#pragma unroll
+        for (int k = 0; k < kOwn; ++k) {
+            const uint32_t r = lane + 64 * k;
+            hk[k] = r < runs ? L.head_key[r] : 0u;
+            off[k] = 0;
+        }
+#pragma unroll 2
+        for (uint32_t j = 0; j < runs; ++j) {
+            const uint2 e = L.head_kl[j];
+#pragma unroll
+            for (int k = 0; k < kOwn; ++k) off[k] += e.x < hk[k] ? e.y : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kOwn; ++k) {
+            const uint32_t r = lane + 64 * k;
+            if (r < runs) L.run_off[r] = off[k] - L.head_pos[r];
+        }
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < kWideChunks; ++q) {
+            const uint32_t i = 64 * q + lane;
+            rk[q] = i < n ? L.run_off[rid[q]] + i : 0xffffffffu;
         }
     } else {
-        uint32_t P = 64;
-        while (P < n) P <<= 1;
-        for (uint32_t i = lane; i < P; i += 64)
-            L.sort[i] = i < n ? ((unsigned long long)(b.src[s + i] & kSrcMask) << 32) | i : ~0ull;
+        // Pairwise: rank = number of smaller source indices (code[] is free
+        // until after the block barrier).
         wave_sync();
-        for (uint32_t k = 2; k <= P; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = lane; i < (P >> 1); i += 64) {
-                    const uint32_t a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-                    const uint32_t d = a | j;
-                    const unsigned long long x = L.sort[a], y = L.sort[d];
-                    if ((x > y) == ((a & k) == 0)) {
-                        L.sort[a] = y;
-                        L.sort[d] = x;
-                    }
-                }
-                wave_sync();
-            }
+#pragma unroll
+        for (int q = 0; q < kWideChunks; ++q) {
+            const uint32_t i = 64 * q + lane;
+            if (i < n) L.code[i] = sv[q] & kSrcMask;
+            rk[q] = 0;
         }
-        // Compact the low words into perm in place, one chunk of 64 at a
-        // time: chunk q writes bytes [256q, 256q+256), which chunk q or an
-        // earlier one has already read.
-        for (uint32_t base = 0; base < n; base += 64) {
-            const uint32_t i = base + lane;
-            const uint32_t a = i < n ? (uint32_t)L.sort[i] : 0u;
-            wave_sync();
-            if (i < n) perm[i] = a;
-            wave_sync();
+        wave_sync();
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint32_t o = L.code[j];
+#pragma unroll
+            for (int q = 0; q < kWideChunks; ++q) rk[q] += o < (sv[q] & kSrcMask);
+        }
+#pragma unroll
+        for (int q = 0; q < kWideChunks; ++q)
+            if (64 * q + lane >= n) rk[q] = 0xffffffffu;
+    }
+#pragma unroll
+    for (int q = 0; q < kWideChunks; ++q) {
+        const uint32_t i = 64 * q + lane;
+        if (i < n && rk[q] < (uint32_t)kMaxInCell)
+            stage[rk[q]] = make_float4(pp[q].x, pp[q].y, __uint_as_float(i), 0.0f);
+    }
+}
+
+// Cell wave, after the barrier: ranks 9..n-1.  Finish in stored order, write
+// at s + rank, keep each rank's direction (or a far mover's key) in LDS; then
+// in canonical order count per direction, one atomic per occupied direction,
+// and the slots.
+__device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffers &b, uint32_t s,
+                                          uint32_t n, uint32_t c, WaveLds &L,
+                                          const float2 (&pp)[kWideChunks],
+                                          const float2 (&vv)[kWideChunks],
+                                          const uint32_t (&rk)[kWideChunks]) {
+    const uint32_t lane = lane_id();
+    const unsigned long long lt = lanemask_lt();
+    const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+    uint32_t *code = L.code;
+    uint32_t fkey[kWideChunks];  // a farther mover's new key, or ~0
+#pragma unroll
+    for (int q = 0; q < kWideChunks; ++q) {
+        fkey[q] = 0xffffffffu;
+        // rk is ~0 past the particles: one compare covers both bounds.
+        if (rk[q] - (uint32_t)kMaxInCell < n - (uint32_t)kMaxInCell) {
+            float px = pp[q].x, py = pp[q].y, vx = vv[q].x, vy = vv[q].y;
+            finish_particle(p, px, py, vx, vy);
+            b.pos_out[s + rk[q]] = make_float2(px, py);
+            b.vel_out[s + rk[q]] = make_float2(vx, vy);
+            uint32_t key;
+            const uint32_t dir = direction(p, px, py, cx, cy, key);
+            code[rk[q]] = dir < 9 ? (kRunFlag | (dir << 27)) : 0u;
+            if (dir == 9) fkey[q] = key;
         }
     }
     wave_sync();
-
-    // 2 + 3. Finish and write every rank; count particles per direction;
-    //        keep each rank's new key in aux.
-    const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+    // Canonical order, chunk by chunk: each near mover's rank in its
+    // direction, written back into code[] (a rolled loop keeps the ballot
+    // masks of one chunk live at a time).
     uint32_t count[9];
 #pragma unroll
     for (int d = 0; d < 9; ++d) count[d] = 0;
-    for (uint32_t b0 = 0; b0 < n; b0 += 64 * kBatch) {
-        float2 lp[kBatch], lv[kBatch];
+#pragma unroll 1
+    for (uint32_t r0 = kMaxInCell; r0 < n; r0 += 64) {
+        const uint32_t r = r0 + lane;
+        const uint32_t w = r < n ? code[r] : 0u;
+        const uint32_t dir = (w & kRunFlag) ? (w >> 27) & 15u : 10u;
+        uint32_t rin = 0;
 #pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const uint32_t r = b0 + 64 * q + lane;
-            lp[q] = lv[q] = make_float2(0.0f, 0.0f);
-            if (r < n) {
-                const uint32_t a = perm[r];
-                lp[q] = b.pos[s + a];
-                lv[q] = b.vel[s + a];
-            }
+        for (int d = 0; d < 9; ++d) {
+            const unsigned long long m = __ballot(dir == (uint32_t)d);
+            if (dir == (uint32_t)d) rin = count[d] + (uint32_t)__popcll(m & lt);
+            count[d] += (uint32_t)__popcll(m);
         }
-        if (b0 == 0) collide_nine_lanes(lp[0].x, lp[0].y);  // ranks 0..8 (n > 8)
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const uint32_t r = b0 + 64 * q + lane;
-            uint32_t dir = 10, key;
-            if (r < n) {
-                float px = lp[q].x, py = lp[q].y, vx = lv[q].x, vy = lv[q].y;
-                finish_particle(p, px, py, vx, vy);
-                b.pos_out[s + r] = make_float2(px, py);
-                b.vel_out[s + r] = make_float2(vx, vy);
-                dir = direction(p, px, py, cx, cy, key);
-            }
-            // rank of this particle among the cell's particles going the
-            // same way, in canonical order
-            uint32_t rank = 0;
-#pragma unroll
-            for (int d = 0; d < 9; ++d) {
-                const unsigned long long m = __ballot(dir == (uint32_t)d);
-                if (dir == (uint32_t)d) rank = count[d] + (uint32_t)__popcll(m & lt);
-                count[d] += (uint32_t)__popcll(m);
-            }
-            // near: bit 31 | direction << 27 | rank; far: the key
-            if (r < n) L.r.aux[r] = dir < 9 ? (kRunFlag | (dir << 27) | rank) : key;
-        }
+        if (dir < 9) code[r] = kRunFlag | (dir << 27) | rin;
     }
-
-    // 4. One atomic per occupied direction (lanes 0..8 together), then the
-    //    slots, in rank order within each direction.
+    // Every atomic of the cell in one round: one per occupied direction
+    // (lanes 0..8), one per farther mover.
     uint32_t mine = 0;
 #pragma unroll
     for (int d = 0; d < 9; ++d) mine = lane == (uint32_t)d ? count[d] : mine;
-    uint32_t got = 0;
-    if (lane < 9 && mine > 0) {
+    uint32_t got = 0, tkey = 0;
+    if (lane < 9) {
         const uint32_t tx = cx + lane % 3 - 1, ty = cy + lane / 3 - 1;
-        got = atomicAdd(&b.cnt[ty * p.gx + tx], mine);
+        tkey = ty * p.gx + tx;
+        if (mine > 0) got = atomicAdd(&b.cnt[tkey], mine);
     }
-    if (lane < 9) L.head_key[lane] = got;  // run bases; the run heads are no longer needed
+    uint32_t fslot[kWideChunks];
+#pragma unroll
+    for (int q = 0; q < kWideChunks; ++q)
+        fslot[q] = fkey[q] != 0xffffffffu ? atomicAdd(&b.cnt[fkey[q]], 1u) : 0u;
+    // Per direction: (new key, first slot), then the slots.
+    if (lane < 9) L.dir_tbl[lane] = make_uint2(tkey, got);
     wave_sync();
-    for (uint32_t base = 0; base < n; base += 64) {
-        const uint32_t r = base + lane;
-        if (r < n) {
-            const uint32_t v = L.r.aux[r];
-            uint32_t key, slot;
-            if (v & kRunFlag) {
-                const uint32_t dir = (v >> 27) & 15u, rank = v & 0x07ffffffu;
-                key = (cy + dir / 3 - 1) * p.gx + (cx + dir % 3 - 1);
-                slot = (L.head_key[dir] + rank) | (rank == 0 ? kRunFlag : 0u);
-            } else {
-                key = v;
-                slot = atomicAdd(&b.cnt[key], 1u) | kRunFlag;
-            }
-            b.kslot[s + r] = make_uint2(key, slot);
+#pragma unroll 1
+    for (uint32_t r0 = kMaxInCell; r0 < n; r0 += 64) {
+        const uint32_t r = r0 + lane;
+        const uint32_t w = r < n ? code[r] : 0u;
+        if (w & kRunFlag) {
+            const uint32_t rin = w & 0x07ffffffu;
+            const uint2 t = L.dir_tbl[(w >> 27) & 15u];
+            b.kslot[s + r] = make_uint2(t.x, (t.y + rin) | (rin == 0 ? kRunFlag : 0u));
         }
+    }
+#pragma unroll
+    for (int q = 0; q < kWideChunks; ++q)
+        if (fkey[q] != 0xffffffffu) b.kslot[s + rk[q]] = make_uint2(fkey[q], fslot[q] | kRunFlag);
+}
+
+// One block item: cells 3 * item + 0..2 of the class-6 list.
+__device__ __forceinline__ void wide_item(const FrameParams &p, const FrameBuffers &b,
+                                          uint32_t item, uint32_t count, WaveLds *lds,
+                                          WideStage &st) {
+    // readfirstlane: the compiler then knows the wave index is uniform and
+    // branches on it instead of running both sides under lane masks.
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = lane_id();
+    const uint4 *list = b.cells + b.class_off[kWaveClass];
+    // Defined on every path, so that they are not carried around the item
+    // loop (and kept live through wave 3's branch).
+    float2 pp[kWideChunks], vv[kWideChunks];
+    uint32_t rk[kWideChunks];
+#pragma unroll
+    for (int q = 0; q < kWideChunks; ++q) {
+        pp[q] = vv[q] = make_float2(0.0f, 0.0f);
+        rk[q] = 0;
+    }
+    uint4 cell = make_uint4(0, 0, 0, 0);
+    bool mine = false, fast = false;
+    __syncthreads();  // the previous item's stage has been read
+    if (wave < (uint32_t)kWideCells) {
+        const uint32_t e = item * kWideCells + wave;
+        mine = e < count;
+        if (mine) {
+            cell = list[e];
+            if (cell.y <= kWideMax) {  // always, by the class; guards a corrupt list
+                wide_order(b, cell.x, cell.y, lds[wave], st.q[wave], pp, vv, rk);
+                fast = true;
+            }
+        }
+        if (lane == 0) st.on[wave] = fast ? 1u : 0u;
+    }
+    __syncthreads();
+    if (wave == (uint32_t)kWideCells) {
+        // Lanes 16g + j, j < 9: rank j of cell g.
+        const uint32_t g = lane / 16, j = lane % 16;
+        const uint32_t e = item * kWideCells + g;
+        uint32_t s = 0, c = 0, m = 0;
+        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float2 v = make_float2(0.0f, 0.0f);
+        if (g < (uint32_t)kWideCells && e < count && st.on[g]) {
+            const uint4 ce = list[e];
+            s = ce.x;
+            c = ce.z;
+            m = kMaxInCell;
+            if (j < (uint32_t)kMaxInCell) {
+                q = st.q[g][j];
+                v = b.vel[s + __float_as_uint(q.z)];  // stored position of rank j
+            }
+        }
+        ranked_group<16>(p, b, q.x, q.y, v.x, v.y, j, m, s, c, 16 * g);
+    } else if (fast) {
+        wide_rest(p, b, cell.x, cell.y, cell.z, lds[wave], pp, vv, rk);
     }
 }
 
@@ -890,26 +1032,37 @@ __device__ __forceinline__ void group_class(const FrameParams &p, const FrameBuf
         group_cells<G>(p, b, b.cells + b.class_off[k], count, (v - start) * (64 / G));
 }
 
-// Cells queued by collide_kernel.  Wavefront work items are numbered: first
-// the class-6 cells (one wavefront each, the longest chains), then the group
-// classes from 64 lanes per cell down to 2; every wavefront takes items
-// gwave, gwave + nwaves, ...  Class-7 cells (over kWaveSort particles) follow,
-// one block each (block-wide LDS sort, or merge sort past kLdsSort).
-__global__ __launch_bounds__(kCollideBlock) void full_cells_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ WaveLds lds[kWavesPerBlock];
-    static_assert(sizeof(WaveLds) * kWavesPerBlock >= kLdsSort * sizeof(unsigned long long),
-                  "the block path reuses the wave regions");
-    const uint32_t wave = threadIdx.x >> 6;
+// Every queued cell:
+//  1. class 6 (65..kWideMax particles), three cells per block item, on the
+//     first blocks;
+//  2. the group classes from 64 lanes per cell down to 2, wavefront work
+//     items spread over every wavefront, numbered from the first block
+//     without a class-6 item so that those take group work first;
+//  3. class 7 (over kWideMax particles), one block each (block-wide LDS sort,
+//     or merge sort past kLdsSort).
+__global__ __launch_bounds__(kCollideBlock) void cells_kernel(FrameParams p, FrameBuffers b) {
+    __shared__ union {
+        struct {
+            WaveLds w[kWavesPerBlock];
+            WideStage stage;
+        } wide;
+        unsigned long long sort[kLdsSort];
+    } sh;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gwave = blockIdx.x * kWavesPerBlock + wave;
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     uint32_t cnt[kClasses];
 #pragma unroll
     for (int k = 0; k < kClasses; ++k) cnt[k] = min(b.class_count[k], b.class_cap[k]);
-    // end[k]: one past the last work item of group class k (items of class
-    // 6 come first).
+
+    const uint32_t items6 = (cnt[kWaveClass] + kWideCells - 1) / kWideCells;
+    for (uint32_t it = blockIdx.x; it < items6; it += gridDim.x)
+        wide_item(p, b, it, cnt[kWaveClass], sh.wide.w, sh.wide.stage);
+
+    // end[k]: one past the last work item of group class k, largest first.
     uint32_t end[kGroupClasses];
     {
-        uint32_t at = cnt[kWaveClass];
+        uint32_t at = 0;
 #pragma unroll
         for (int k = kGroupClasses - 1; k >= 0; --k) {
             const uint32_t per = 32u >> k;  // cells per wavefront: 64 / 2^(k+1)
@@ -917,23 +1070,22 @@ __global__ __launch_bounds__(kCollideBlock) void full_cells_kernel(FrameParams p
             end[k] = at;
         }
     }
-    for (uint32_t v = gwave; v < cnt[kWaveClass]; v += nwaves) {
-        const uint4 cell = b.cells[b.class_off[kWaveClass] + v];
-        wave_cell(p, b, cell.x, cell.y, cell.z, lds[wave]);
-    }
+    const uint32_t busy = (items6 < gridDim.x ? items6 : gridDim.x) * kWavesPerBlock;
+    const uint32_t rw = gwave >= busy ? gwave - busy : gwave + nwaves - busy;
     // One loop per group class keeps each variant's registers to itself.
-    group_class<64>(p, b, 5, cnt[kWaveClass], end[5], cnt[5], gwave, nwaves);
-    group_class<32>(p, b, 4, end[5], end[4], cnt[4], gwave, nwaves);
-    group_class<16>(p, b, 3, end[4], end[3], cnt[3], gwave, nwaves);
-    group_class<8>(p, b, 2, end[3], end[2], cnt[2], gwave, nwaves);
-    group_class<4>(p, b, 1, end[2], end[1], cnt[1], gwave, nwaves);
-    group_class<2>(p, b, 0, end[1], end[0], cnt[0], gwave, nwaves);
+    group_class<64>(p, b, 5, 0, end[5], cnt[5], rw, nwaves);
+    group_class<32>(p, b, 4, end[5], end[4], cnt[4], rw, nwaves);
+    group_class<16>(p, b, 3, end[4], end[3], cnt[3], rw, nwaves);
+    group_class<8>(p, b, 2, end[3], end[2], cnt[2], rw, nwaves);
+    group_class<4>(p, b, 1, end[2], end[1], cnt[1], rw, nwaves);
+    group_class<2>(p, b, 0, end[1], end[0], cnt[0], rw, nwaves);
+
     const uint32_t nb = cnt[kBlockClass];
     if (nb == 0) return;
     __syncthreads();
     for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
         const uint4 cell = b.cells[b.class_off[kBlockClass] + e];
-        big_cell<true>(p, b, cell.x, cell.y, reinterpret_cast<unsigned long long *>(lds));
+        big_cell<true>(p, b, cell.x, cell.y, cell.z, sh.sort);
     }
 }
 
@@ -970,7 +1122,7 @@ __global__ __launch_bounds__(kCollideBlock) void canonical_kernel(FrameParams p,
     }
     __syncthreads();
     const uint32_t nb = nbig;
-    for (uint32_t q = 0; q < nb; ++q) big_cell<false>(p, b, big_s[q], big_n[q], sk);
+    for (uint32_t q = 0; q < nb; ++q) big_cell<false>(p, b, big_s[q], big_n[q], 0, sk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1006,7 +1158,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // Exclusive prefix of `tile` from its predecessors' status words, by one
 // wavefront.  Lane l of window w reads tile j - 64w - l.
 __device__ uint32_t look_back(const FrameBuffers &b, uint32_t tile, uint32_t epoch) {
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lane = lane_id();
     uint32_t excl = 0;
     int64_t j = (int64_t)tile - 1;
     while (true) {
@@ -1171,10 +1323,21 @@ hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_full_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    hipLaunchKernelGGL(full_cells_kernel, dim3(b.full_grid), dim3(kCollideBlock), 0, stream, p, b);
+hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
+    hipLaunchKernelGGL(cells_kernel, dim3(b.cells_grid), dim3(kCollideBlock), 0, stream, p, b);
     return hipGetLastError();
 }
+
+static uint32_t resident_blocks(const void *kernel) {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kCollideBlock, 0) != hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu > 0 ? per_cu : 1) * (uint32_t)(cus > 0 ? cus : 1);
+}
+
+uint32_t resident_blocks_cells() { return resident_blocks(reinterpret_cast<const void *>(cells_kernel)); }
 
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
     const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
