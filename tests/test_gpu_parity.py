@@ -136,13 +136,12 @@ def test_single_and_coincident_particles(oracle):
     run_parity(oracle, cfg, pos, vel, 20, label="coincident+walls")
 
 
-@pytest.mark.parametrize("per_cell", [2, 3, 4, 5, 8, 9, 10, 16, 17, 33, 40, 64, 65, 300, 576, 577,
-                                      1024, 1025, 2048, 2049, 5000])
+@pytest.mark.parametrize("per_cell", [2, 3, 4, 5, 8, 9, 10, 16, 17, 33, 40, 64, 65, 300, 577, 704,
+                                      705, 1024, 2048, 2049, 5000])
 def test_full_cells(oracle, per_cell):
-    """Every size class: lane groups (2..64), the wide path (65..576), the
-    block path for 577..1024 (class 6 past the wide registers) and above
-    1024 (LDS sort up to 2048, merge sort past it), all with the 9-particle
-    collision cap."""
+    """Every size class: lane groups (2..64), the wide path (65..704, its
+    particles in registers) and the block path past it (LDS sort up to 2048,
+    merge sort past it), all with the 9-particle collision cap."""
     r = np.random.default_rng(per_cell)
     cells = [(3, 4), (10, 2), (11, 2), (20, 7)]
     pos, vel = [], []

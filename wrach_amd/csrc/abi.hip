@@ -139,7 +139,7 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.tile_state);
     (void)hipFree(c->buf.epoch);
     (void)hipFree(c->buf.cells);
-    (void)hipFree(c->buf.class_count);
+    (void)hipFree(c->buf.class_ctr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -259,12 +259,18 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     wrach::FrameBuffers &b = c->buf;
     b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell);
     // Queued-cell regions: a class of cells holding at least m particles has
-    // at most capacity / m cells.
+    // at most capacity / m cells.  Each class: kSlices slices sharing that
+    // bound, then an overflow list that can hold all of it.
     size_t ncells_total = 0;
     for (int k = 0; k < wrach::kClasses; ++k) {
-        b.class_off[k] = (uint32_t)ncells_total;
         b.class_cap[k] = cap / wrach::kClassMinN[k] + 64;
-        ncells_total += b.class_cap[k];
+        b.slice_cap[k] = (b.class_cap[k] + wrach::kSlices - 1) / wrach::kSlices + 16;
+        b.class_off[k] = (uint32_t)ncells_total;
+        ncells_total += (size_t)wrach::kSlices * b.slice_cap[k] + b.class_cap[k];
+    }
+    if (ncells_total > 0xffffffffu) {
+        delete c;
+        return fail(WRACH_E_TOO_MANY_CELLS, "queued-cell lists exceed 2^32 entries");
     }
     auto bail = [&](int rc) { release(c); return rc; };
     // Grids: the blocks the device holds at once (the kernels loop over
@@ -297,7 +303,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.tile_state, b.scan_tiles));
     WRACH_HIP_C(dmalloc(&b.epoch, 1));
     WRACH_HIP_C(dmalloc(&b.cells, ncells_total));
-    WRACH_HIP_C(dmalloc(&b.class_count, wrach::kClasses));
+    WRACH_HIP_C(dmalloc(&b.class_ctr, wrach::kCtrWords));
     WRACH_HIP_C(hipMemsetAsync(c->idx_alloc, 0, (nidx + 64) * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.pos, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.vel, 0, cap * sizeof(float2), c->stream));
@@ -310,7 +316,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
         WRACH_HIP_C(hipMemcpyAsync(b.epoch, &first_epoch, sizeof first_epoch, hipMemcpyHostToDevice,
                                    c->stream));
     }
-    WRACH_HIP_C(hipMemsetAsync(b.class_count, 0, wrach::kClasses * sizeof(uint32_t), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.class_ctr, 0, wrach::kCtrWords * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipStreamSynchronize(c->stream));
 #undef WRACH_HIP_C
     *out = c;

@@ -35,9 +35,10 @@ struct FrameBuffers {
     unsigned long long *tile_state;  // scan look-back words
     uint32_t *epoch;      // frame counter tagging the scan's status words; the scatter advances it
     uint4 *cells;         // queued cells (s, n, cell, 0), one region per size class
-    uint32_t *class_count;  // queued cells per class; reset by the scan
+    uint32_t *class_ctr;  // fill of every slice and overflow list (ctr_index); reset by the scan
     uint32_t class_off[8];  // first entry of each class's region in `cells`
-    uint32_t class_cap[8];  // entries of each region
+    uint32_t slice_cap[8];  // entries per slice of a class
+    uint32_t class_cap[8];  // most cells a class can hold (its overflow list's size)
     uint32_t scan_tiles;
     uint32_t cells_grid;  // blocks of the queued-cell kernel
 };
@@ -56,7 +57,7 @@ constexpr int kWavesPerBlock = kCollideBlock / 64;
 // 9-16, 17-32 and 33-64 particles, processed by groups of 2..64 lanes with
 // one particle per lane; class 6 (65..kWideMax) takes a wavefront per cell
 // with its particles in registers; class 7 (more) a block per cell.
-constexpr int kWideChunks = 9;            // 64-particle chunks of a wide-path wavefront
+constexpr int kWideChunks = 11;           // 64-particle chunks of a wide-path wavefront
 constexpr uint32_t kWideMax = 64 * kWideChunks;
 constexpr int kClasses = 8;
 constexpr int kGroupClasses = 6;
@@ -65,6 +66,18 @@ constexpr int kBlockClass = 7;
 // Fewest particles a cell of each class holds (sizes the list regions).
 constexpr uint32_t kClassMinN[kClasses] = {2, 3, 5, 9, 17, 33, 65, kWideMax + 1};
 static_assert(sizeof(FrameBuffers::class_off) == kClasses * sizeof(uint32_t), "one region per class");
+
+// A class's region is kSlices slices of slice_cap entries, then an overflow
+// list of class_cap entries.  Collide block b appends to slice b % kSlices
+// and spills to the overflow list when the slice is full: counters that many
+// blocks hit serialise their atomics, one instruction at a time per cache
+// line, so every counter has a 128-byte line of its own.
+constexpr int kSlices = 64;
+constexpr int kCtrStride = 32;  // words between counters
+__host__ __device__ constexpr uint32_t ctr_index(int k, int s) {  // s == kSlices: overflow
+    return (uint32_t)((k * (kSlices + 1) + s) * kCtrStride);
+}
+constexpr uint32_t kCtrWords = ctr_index(kClasses, 0);
 
 inline uint32_t scan_tiles_for(uint32_t ncell) {
     return (ncell + kScanTile - 1) / kScanTile;

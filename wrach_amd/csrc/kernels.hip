@@ -159,6 +159,22 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << lane_id()) - 1;
 }
 
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // Orders LDS accesses of one wavefront: its LDS instructions execute in
 // program order, so a compiler fence is all a wave-private region needs.
 __device__ __forceinline__ void wave_sync() {
@@ -537,26 +553,27 @@ __device__ __forceinline__ uint32_t size_class(uint32_t n) {
 }
 
 __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ uint32_t s_cnt[kClasses], s_base[kClasses];
+    __shared__ uint32_t s_cnt[kClasses], s_base[kClasses], s_in[kClasses], s_ovf[kClasses];
     if (threadIdx.x < (uint32_t)kClasses) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t c0 = (blockIdx.x * kCollideBlock + threadIdx.x) * kCellsPerLane;
-    uint32_t o[kCellsPerLane + 1];
-    if (c0 + kCellsPerLane <= p.ncell) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(b.idx + 1 + c0);  // 16-byte aligned
-        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-        o[4] = b.idx[c0 + 5];
-    } else {
-        // idx has ncell + 2 entries; entry c + 1 exists for every c <= ncell.
-#pragma unroll
-        for (int j = 0; j <= kCellsPerLane; ++j)
-            o[j] = c0 + j <= p.ncell ? b.idx[c0 + 1 + j] : (j ? o[j - 1] : 0u);
-    }
-    uint32_t st[kCellsPerLane], nn[kCellsPerLane];
+    // Lane l of a wavefront takes cells w0 + l, w0 + 64 + l, ... of the
+    // wavefront's 256, so that each atomic instruction covers 64 neighbouring
+    // cells (a few cache lines).  A cell's end is the next lane's start; lane
+    // 63 reads it (idx has ncell + 2 entries, entry c + 1 exists for every
+    // c <= ncell).
+    const uint32_t w0 = (blockIdx.x * kCollideBlock + (threadIdx.x & ~63u)) * kCellsPerLane;
+    const uint32_t ln = threadIdx.x & 63;
+    uint32_t st[kCellsPerLane], nn[kCellsPerLane], cid[kCellsPerLane];
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
-        st[j] = o[j];
-        nn[j] = c0 + j < p.ncell ? o[j + 1] - o[j] : 0u;
+        cid[j] = w0 + 64 * j + ln;
+        st[j] = cid[j] <= p.ncell ? b.idx[cid[j] + 1] : p.n;
+    }
+#pragma unroll
+    for (int j = 0; j < kCellsPerLane; ++j) {
+        uint32_t end = (uint32_t)__shfl_down((int)st[j], 1, 64);
+        if (ln == 63) end = cid[j] + 1 <= p.ncell ? b.idx[cid[j] + 2] : p.n;
+        nn[j] = cid[j] < p.ncell ? end - st[j] : 0u;
         clamp_range(p, st[j], nn[j]);
     }
 
@@ -597,17 +614,33 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
         at[j] = cls[j] < (uint32_t)kClasses ? atomicAdd(&s_cnt[cls[j]], 1u) : 0u;
     }
     __syncthreads();
+    // Slice blockIdx % kSlices; what does not fit goes to the overflow list.
+    const uint32_t slice = blockIdx.x % kSlices;
     if (threadIdx.x < (uint32_t)kClasses) {
-        const uint32_t k = s_cnt[threadIdx.x];
-        s_base[threadIdx.x] = k ? atomicAdd(&b.class_count[threadIdx.x], k) : 0u;
+        const uint32_t k = threadIdx.x, m = s_cnt[k];
+        uint32_t a = 0, in = 0, o = 0;
+        if (m) {
+            a = atomicAdd(&b.class_ctr[ctr_index(k, slice)], m);
+            const uint32_t room = a < b.slice_cap[k] ? b.slice_cap[k] - a : 0u;
+            in = m < room ? m : room;
+            if (m > in) o = atomicAdd(&b.class_ctr[ctr_index(k, kSlices)], m - in);
+        }
+        s_base[k] = a;
+        s_in[k] = in;
+        s_ovf[k] = o;
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
-        if (cls[j] < (uint32_t)kClasses) {
-            const uint32_t a = s_base[cls[j]] + at[j];
-            if (a < b.class_cap[cls[j]])
-                b.cells[b.class_off[cls[j]] + a] = make_uint4(st[j], nn[j], c0 + j, 0);
+        const uint32_t k = cls[j];
+        if (k < (uint32_t)kClasses) {
+            const uint4 ent = make_uint4(st[j], nn[j], cid[j], 0);
+            if (at[j] < s_in[k]) {
+                b.cells[b.class_off[k] + slice * b.slice_cap[k] + s_base[k] + at[j]] = ent;
+            } else {
+                const uint32_t o = s_ovf[k] + at[j] - s_in[k];
+                if (o < b.class_cap[k]) b.cells[b.class_off[k] + kSlices * b.slice_cap[k] + o] = ent;
+            }
         }
     }
 }
@@ -682,7 +715,7 @@ __device__ __forceinline__ void ranked_group(const FrameParams &p, const FrameBu
                                              uint32_t n, uint32_t s, uint32_t c, uint32_t gbase);
 
 template <int G>
-__device__ void group_cells(const FrameParams &p, const FrameBuffers &b, const uint4 *list,
+__device__ __forceinline__ void group_cells(const FrameParams &p, const FrameBuffers &b, const uint4 *list,
                             uint32_t count, uint32_t first) {
     const uint32_t lane = lane_id();
     const uint32_t j = lane % G;
@@ -964,14 +997,13 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
         if (fkey[q] != 0xffffffffu) b.kslot[s + rk[q]] = make_uint2(fkey[q], fslot[q] | kRunFlag);
 }
 
-// One block item: cells 3 * item + 0..2 of the class-6 list.
+// One block item: cells list[0..2], of which `count` exist.
 __device__ __forceinline__ void wide_item(const FrameParams &p, const FrameBuffers &b,
-                                          uint32_t item, uint32_t count, WaveLds *lds,
+                                          const uint4 *list, uint32_t count, WaveLds *lds,
                                           WideStage &st) {
     // readfirstlane: the compiler then knows the wave index is uniform and
     // branches on it instead of running both sides under lane masks.
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = lane_id();
-    const uint4 *list = b.cells + b.class_off[kWaveClass];
     // Defined on every path, so that they are not carried around the item
     // loop (and kept live through wave 3's branch).
     float2 pp[kWideChunks], vv[kWideChunks];
@@ -985,7 +1017,7 @@ __device__ __forceinline__ void wide_item(const FrameParams &p, const FrameBuffe
     bool mine = false, fast = false;
     __syncthreads();  // the previous item's stage has been read
     if (wave < (uint32_t)kWideCells) {
-        const uint32_t e = item * kWideCells + wave;
+        const uint32_t e = wave;
         mine = e < count;
         if (mine) {
             cell = list[e];
@@ -1000,7 +1032,7 @@ __device__ __forceinline__ void wide_item(const FrameParams &p, const FrameBuffe
     if (wave == (uint32_t)kWideCells) {
         // Lanes 16g + j, j < 9: rank j of cell g.
         const uint32_t g = lane / 16, j = lane % 16;
-        const uint32_t e = item * kWideCells + g;
+        const uint32_t e = g;
         uint32_t s = 0, c = 0, m = 0;
         float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float2 v = make_float2(0.0f, 0.0f);
@@ -1020,16 +1052,80 @@ __device__ __forceinline__ void wide_item(const FrameParams &p, const FrameBuffe
     }
 }
 
-// Work items [start, stop) of group class k (G = 2^(k+1) lanes per cell)
-// that fall to wavefront gwave: items gwave + i * nwaves.
+// Work items of the queued classes.  Class k's items take `per` cells each
+// (64 / G for a group class, three for class 6, one for class 7) and are
+// numbered slice by slice, then the overflow list.  ItemMap holds, per
+// class, the inclusive prefix of items over the slices (in LDS), the slice
+// fills and the totals.
+struct ItemMap {
+    uint32_t incl[kClasses][kSlices];
+    uint32_t fill[kClasses][kSlices + 1];  // [kSlices]: the overflow list
+    uint32_t in_slices[kClasses];          // items in the slices
+    uint32_t items[kClasses];              // all items of the class
+};
+
+__host__ __device__ constexpr uint32_t cells_per_item(int k) {
+    return k < kGroupClasses ? (32u >> k) : k == kWaveClass ? (uint32_t)kWideCells : 1u;
+}
+
+// Fills the map: wave w takes classes w and w + 4, lane l slice l.
+__device__ __forceinline__ void build_item_map(const FrameBuffers &b, ItemMap &m) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = lane_id();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k = (int)wave + 4 * h;
+        // constant indices into the kernel-argument arrays (a variable index
+        // would copy them to scratch memory)
+        uint32_t scap = 0, ccap = 0;
+#pragma unroll
+        for (int kk = 0; kk < kClasses; ++kk) {
+            scap = k == kk ? b.slice_cap[kk] : scap;
+            ccap = k == kk ? b.class_cap[kk] : ccap;
+        }
+        const uint32_t per = cells_per_item(k);
+        const uint32_t f = min(b.class_ctr[ctr_index(k, (int)lane)], scap);
+        const uint32_t incl = wave_inclusive_sum((f + per - 1) / per);
+        m.incl[k][lane] = incl;
+        m.fill[k][lane] = f;
+        if (lane == 63) {
+            const uint32_t fo = min(b.class_ctr[ctr_index(k, kSlices)], ccap);
+            m.fill[k][kSlices] = fo;
+            m.in_slices[k] = incl;
+            m.items[k] = incl + (fo + per - 1) / per;
+        }
+    }
+}
+
+// Item v (below m.items[k], the same in every lane) of class k: its first
+// cell and how many cells of its list segment start there.
+__device__ __forceinline__ const uint4 *locate(const FrameBuffers &b, const ItemMap &m, int k,
+                                               uint32_t v, uint32_t &avail) {
+    const uint32_t per = cells_per_item(k);
+    uint32_t seg, local;
+    if (v < m.in_slices[k]) {
+        seg = (uint32_t)__popcll(__ballot(m.incl[k][lane_id()] <= v));
+        local = v - (seg ? m.incl[k][seg - 1] : 0u);
+    } else {
+        seg = kSlices;
+        local = v - m.in_slices[k];
+    }
+    avail = m.fill[k][seg] - local * per;
+    return b.cells + b.class_off[k] + seg * b.slice_cap[k] + local * per;
+}
+
+// Work items [start, stop) of the global numbering that belong to group
+// class k (G = 2^(k+1) lanes per cell) and fall to wavefront gwave.
 template <int G>
-__device__ __forceinline__ void group_class(const FrameParams &p, const FrameBuffers &b, int k,
-                                            uint32_t start, uint32_t stop, uint32_t count,
-                                            uint32_t gwave, uint32_t nwaves) {
+__device__ __forceinline__ void group_class(const FrameParams &p, const FrameBuffers &b,
+                                            const ItemMap &m, int k, uint32_t start,
+                                            uint32_t stop, uint32_t gwave, uint32_t nwaves) {
     uint32_t v = gwave;
     if (v < start) v += (start - v + nwaves - 1) / nwaves * nwaves;
-    for (; v < stop; v += nwaves)
-        group_cells<G>(p, b, b.cells + b.class_off[k], count, (v - start) * (64 / G));
+    for (; v < stop; v += nwaves) {
+        uint32_t avail;
+        const uint4 *list = locate(b, m, k, v - start, avail);
+        group_cells<G>(p, b, list, avail, 0);
+    }
 }
 
 // Every queued cell:
@@ -1048,16 +1144,19 @@ __global__ __launch_bounds__(kCollideBlock) void cells_kernel(FrameParams p, Fra
         } wide;
         unsigned long long sort[kLdsSort];
     } sh;
+    __shared__ ItemMap map;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t gwave = blockIdx.x * kWavesPerBlock + wave;
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-    uint32_t cnt[kClasses];
-#pragma unroll
-    for (int k = 0; k < kClasses; ++k) cnt[k] = min(b.class_count[k], b.class_cap[k]);
+    build_item_map(b, map);
+    __syncthreads();
 
-    const uint32_t items6 = (cnt[kWaveClass] + kWideCells - 1) / kWideCells;
-    for (uint32_t it = blockIdx.x; it < items6; it += gridDim.x)
-        wide_item(p, b, it, cnt[kWaveClass], sh.wide.w, sh.wide.stage);
+    const uint32_t items6 = map.items[kWaveClass];
+    for (uint32_t it = blockIdx.x; it < items6; it += gridDim.x) {
+        uint32_t avail;
+        const uint4 *list = locate(b, map, kWaveClass, it, avail);
+        wide_item(p, b, list, avail, sh.wide.w, sh.wide.stage);
+    }
 
     // end[k]: one past the last work item of group class k, largest first.
     uint32_t end[kGroupClasses];
@@ -1065,26 +1164,26 @@ __global__ __launch_bounds__(kCollideBlock) void cells_kernel(FrameParams p, Fra
         uint32_t at = 0;
 #pragma unroll
         for (int k = kGroupClasses - 1; k >= 0; --k) {
-            const uint32_t per = 32u >> k;  // cells per wavefront: 64 / 2^(k+1)
-            at += (cnt[k] + per - 1) / per;
+            at += map.items[k];
             end[k] = at;
         }
     }
     const uint32_t busy = (items6 < gridDim.x ? items6 : gridDim.x) * kWavesPerBlock;
     const uint32_t rw = gwave >= busy ? gwave - busy : gwave + nwaves - busy;
     // One loop per group class keeps each variant's registers to itself.
-    group_class<64>(p, b, 5, 0, end[5], cnt[5], rw, nwaves);
-    group_class<32>(p, b, 4, end[5], end[4], cnt[4], rw, nwaves);
-    group_class<16>(p, b, 3, end[4], end[3], cnt[3], rw, nwaves);
-    group_class<8>(p, b, 2, end[3], end[2], cnt[2], rw, nwaves);
-    group_class<4>(p, b, 1, end[2], end[1], cnt[1], rw, nwaves);
-    group_class<2>(p, b, 0, end[1], end[0], cnt[0], rw, nwaves);
+    group_class<64>(p, b, map, 5, 0, end[5], rw, nwaves);
+    group_class<32>(p, b, map, 4, end[5], end[4], rw, nwaves);
+    group_class<16>(p, b, map, 3, end[4], end[3], rw, nwaves);
+    group_class<8>(p, b, map, 2, end[3], end[2], rw, nwaves);
+    group_class<4>(p, b, map, 1, end[2], end[1], rw, nwaves);
+    group_class<2>(p, b, map, 0, end[1], end[0], rw, nwaves);
 
-    const uint32_t nb = cnt[kBlockClass];
+    const uint32_t nb = map.items[kBlockClass];
     if (nb == 0) return;
     __syncthreads();
     for (uint32_t e = blockIdx.x; e < nb; e += gridDim.x) {
-        const uint4 cell = b.cells[b.class_off[kBlockClass] + e];
+        uint32_t avail;
+        const uint4 cell = *locate(b, map, kBlockClass, e, avail);
         big_cell<true>(p, b, cell.x, cell.y, cell.z, sh.sort);
     }
 }
@@ -1137,22 +1236,6 @@ constexpr int kLookWindows = 4;  // 64-tile windows read per look-back round
 __device__ __forceinline__ unsigned long long status_word(unsigned long long flag, uint32_t epoch,
                                                           uint32_t value) {
     return (flag << 62) | ((unsigned long long)(epoch & kEpochMask) << 32) | value;
-}
-
-__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t u = __shfl_up(v, off, 64);
-        if (lane >= off) v += u;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
 }
 
 // Exclusive prefix of `tile` from its predecessors' status words, by one
@@ -1291,8 +1374,9 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
     if (tile == 0 && tid == 0) {
         b.idx[0] = 0;
     }
-    if (tile == 0 && tid < (uint32_t)kClasses)
-        b.class_count[tid] = 0;  // the full-cell kernel of this frame has finished
+    if (tile == 0)  // the cell kernel of this frame has finished with the lists
+        for (uint32_t i = tid; i < (uint32_t)(kClasses * (kSlices + 1)); i += kScanThreads)
+            b.class_ctr[i * kCtrStride] = 0;
     if (tile == b.scan_tiles - 1 && tid == 0) b.idx[p.ncell + 1] = s_prefix + agg;
 }
 
