@@ -248,8 +248,9 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     fp.x1 = fp.ax + c->settings.view_dimensions[0];
     fp.y1 = fp.ay + c->settings.view_dimensions[1];
     fp.cs = cs;
-    fp.gravx = cfg->gravity[0];
-    fp.gravy = cfg->gravity[1];
+    // gravity * dt rounded to f32 once here, as the device would per particle
+    fp.gdx = cfg->gravity[0] != 0.0f ? cfg->gravity[0] * cfg->dt : -0.0f;
+    fp.gdy = cfg->gravity[1] != 0.0f ? cfg->gravity[1] * cfg->dt : -0.0f;
     fp.dt = cfg->dt;
     fp.gx = (uint32_t)gx;
     fp.gy = (uint32_t)gy;

@@ -14,8 +14,12 @@ struct FrameParams {
     float ax, ay;        // view_anchor
     float x1, y1;        // view_anchor + view_dimensions
     float cs;            // cell size as f32 (particles_per_cell.wgsl:18)
-    float gravx, gravy;  // extension; 0 skips the add
-    float dt;            // extension; 1 integrates p += v exactly
+    // Extension: v += gravity * dt, then p += v * dt.  gdx/gdy hold
+    // gravity * dt, or -0.0 for a zero component: adding -0.0 leaves every
+    // value unchanged, -0.0 included, which is what skipping the add does.
+    // p + v * 1 is p + v exactly, so dt == 1 needs no special case.
+    float gdx, gdy;
+    float dt;
     uint32_t gx, gy;     // grid_dimensions
     uint32_t ncell;      // gx * gy
     uint32_t n;          // particles_in_frame_count

@@ -54,19 +54,15 @@ constexpr uint32_t kSrcMask = 0x7fffffffu;
 
 // Particle::integrate + enforce_boundaries + enforce_velocity
 // (shaders/physics/src/particle.rs:40-82) with the gravity/dt extension
-// placed just before the integration.  A zero gravity component is skipped so
-// that -0.0 velocities keep their sign, as in the reference.
+// placed just before the integration.  A zero gravity component adds -0.0
+// (FrameParams), which leaves -0.0 velocities their sign, as skipping the add
+// in the reference does.
 __device__ __forceinline__ void finish_particle(const FrameParams &p, float &px, float &py,
                                                 float &vx, float &vy) {
-    if (p.gravx != 0.0f) vx = vx + p.gravx * p.dt;
-    if (p.gravy != 0.0f) vy = vy + p.gravy * p.dt;
-    if (p.dt == 1.0f) {
-        px = px + vx;
-        py = py + vy;
-    } else {
-        px = px + vx * p.dt;
-        py = py + vy * p.dt;
-    }
+    vx = vx + p.gdx;
+    vy = vy + p.gdy;
+    px = px + vx * p.dt;
+    py = py + vy * p.dt;
     if (px > p.x1) { px = p.x1; vx = vx * -1.0f; }
     if (px < p.ax) { px = p.ax; vx = vx * -1.0f; }
     if (py > p.y1) { py = p.y1; vy = vy * -1.0f; }
@@ -98,8 +94,10 @@ __device__ __forceinline__ void collide_pair(float &xi, float &yi, float &xj, fl
 }
 
 // Cell coordinates of particles_per_cell.wgsl:14-27: floor((x - ax) / cs)
-// per axis.  The clamp only matters for a position outside the walls, which
-// the step never produces; it keeps every key inside the count array.
+// per axis, for a finished particle.  The walls leave x in [ax, x1] with
+// ax = 0 and x1 = W, and the division is monotonic, so the key lies in
+// [0, gx - 1] (gx - 1 = floor(W / cs) on the host, the same f32 division);
+// the max only maps a NaN position, which no upload lets in, to column 0.
 __device__ __forceinline__ void cell_xy(const FrameParams &p, float x, float y, uint32_t &kx,
                                         uint32_t &ky) {
     float fx, fy;
@@ -110,10 +108,8 @@ __device__ __forceinline__ void cell_xy(const FrameParams &p, float x, float y, 
         fx = floorf((x - p.ax) / p.cs);
         fy = floorf((y - p.ay) / p.cs);
     }
-    fx = fminf(fmaxf(fx, 0.0f), (float)(p.gx - 1));
-    fy = fminf(fmaxf(fy, 0.0f), (float)(p.gy - 1));
-    kx = (uint32_t)fx;
-    ky = (uint32_t)fy;
+    kx = (uint32_t)fmaxf(fx, 0.0f);
+    ky = (uint32_t)fmaxf(fy, 0.0f);
 }
 
 __device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, float y) {
@@ -869,11 +865,19 @@ __device__ __forceinline__ void wide_order(const FrameBuffers &b, uint32_t s, ui
             hk[k] = r < runs ? L.head_key[r] : 0u;
             off[k] = 0;
         }
+        if (runs <= 64) {  // the common case: one run per lane at most
+#pragma unroll 4
+            for (uint32_t j = 0; j < runs; ++j) {
+                const uint2 e = L.head_kl[j];
+                off[0] += e.x < hk[0] ? e.y : 0u;
+            }
+        } else {
 #pragma unroll 2
-        for (uint32_t j = 0; j < runs; ++j) {
-            const uint2 e = L.head_kl[j];
+            for (uint32_t j = 0; j < runs; ++j) {
+                const uint2 e = L.head_kl[j];
 #pragma unroll
-            for (int k = 0; k < kOwn; ++k) off[k] += e.x < hk[k] ? e.y : 0u;
+                for (int k = 0; k < kOwn; ++k) off[k] += e.x < hk[k] ? e.y : 0u;
+            }
         }
 #pragma unroll
         for (int k = 0; k < kOwn; ++k) {
@@ -1062,6 +1066,8 @@ struct ItemMap {
     uint32_t fill[kClasses][kSlices + 1];  // [kSlices]: the overflow list
     uint32_t in_slices[kClasses];          // items in the slices
     uint32_t items[kClasses];              // all items of the class
+    uint32_t off[kClasses];                // class_off and slice_cap, kept here
+    uint32_t scap[kClasses];               // rather than in scalar registers
 };
 
 __host__ __device__ constexpr uint32_t cells_per_item(int k) {
@@ -1076,11 +1082,12 @@ __device__ __forceinline__ void build_item_map(const FrameBuffers &b, ItemMap &m
         const int k = (int)wave + 4 * h;
         // constant indices into the kernel-argument arrays (a variable index
         // would copy them to scratch memory)
-        uint32_t scap = 0, ccap = 0;
+        uint32_t scap = 0, ccap = 0, off = 0;
 #pragma unroll
         for (int kk = 0; kk < kClasses; ++kk) {
             scap = k == kk ? b.slice_cap[kk] : scap;
             ccap = k == kk ? b.class_cap[kk] : ccap;
+            off = k == kk ? b.class_off[kk] : off;
         }
         const uint32_t per = cells_per_item(k);
         const uint32_t f = min(b.class_ctr[ctr_index(k, (int)lane)], scap);
@@ -1092,6 +1099,8 @@ __device__ __forceinline__ void build_item_map(const FrameBuffers &b, ItemMap &m
             m.fill[k][kSlices] = fo;
             m.in_slices[k] = incl;
             m.items[k] = incl + (fo + per - 1) / per;
+            m.off[k] = off;
+            m.scap[k] = scap;
         }
     }
 }
@@ -1110,7 +1119,7 @@ __device__ __forceinline__ const uint4 *locate(const FrameBuffers &b, const Item
         local = v - m.in_slices[k];
     }
     avail = m.fill[k][seg] - local * per;
-    return b.cells + b.class_off[k] + seg * b.slice_cap[k] + local * per;
+    return b.cells + m.off[k] + seg * m.scap[k] + local * per;
 }
 
 // Work items [start, stop) of the global numbering that belong to group
