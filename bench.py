@@ -9,7 +9,8 @@ gravity (0, -0.1).  Prints one JSON line.
 
 roofline: the dominant kernel's algorithmic bytes per launch (per particle
 and per cell figures below) over its average duration, measured with HIP
-events recorded on the library's own stream around each launch.
+events that hipExtLaunchKernel stamps with each kernel's start and end on
+the library's own stream (wrach_step_profiled).
 cpu_baseline: the oracle's multi-threaded frame on the host, on a bounded
 number of frames of the same workload.
 """
@@ -122,7 +123,7 @@ def main() -> int:
     value = n * args.steps / elapsed
     print(f"[bench] timed {args.steps} frames in {elapsed * 1e3:.2f} ms", file=sys.stderr, flush=True)
 
-    # Per-kernel durations: HIP events around each launch, plain launches.
+    # Per-kernel durations: kernel-stamped HIP events, plain launches.
     split_before = particles_by_kernel(world.read()[0])
     ms = world.step_profiled(args.profile_steps)
     split_after = particles_by_kernel(world.read()[0])

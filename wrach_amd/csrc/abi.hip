@@ -405,14 +405,12 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
     for (uint32_t f = 0; f < nframes && rc == WRACH_OK; ++f) {
         hipEvent_t *fe = &ev[2 * K * (size_t)f];
         hipError_t (*const launch[K])(const wrach::FrameParams &, const wrach::FrameBuffers &,
-                                      hipStream_t) = {wrach::launch_collide, wrach::launch_cells,
-                                                      wrach::launch_scan, wrach::launch_scatter};
+                                      hipStream_t, hipEvent_t, hipEvent_t) = {
+            wrach::launch_collide, wrach::launch_cells, wrach::launch_scan, wrach::launch_scatter};
         hipError_t e = hipSuccess;
-        for (int k = 0; k < K && e == hipSuccess; ++k) {
-            e = hipEventRecord(fe[2 * k], c->stream);
-            if (e == hipSuccess) e = launch[k](c->fp, c->buf, c->stream);
-            if (e == hipSuccess) e = hipEventRecord(fe[2 * k + 1], c->stream);
-        }
+        // The launch stamps both events with the kernel's own start and end.
+        for (int k = 0; k < K && e == hipSuccess; ++k)
+            e = launch[k](c->fp, c->buf, c->stream, fe[2 * k], fe[2 * k + 1]);
         if (e != hipSuccess) rc = fail(WRACH_E_DEVICE, std::string("profiled frame: ") + hipGetErrorString(e));
     }
     hipError_t se = hipStreamSynchronize(c->stream);

@@ -88,13 +88,18 @@ inline uint32_t scan_tiles_for(uint32_t ncell) {
 }
 
 // Launchers (kernels.hip).  Each enqueues on `stream` and returns the launch
-// status.
-hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
-hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+// status; the frame kernels' optional events are stamped with the kernel's
+// start and end.
+hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                          hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                        hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // Blocks of the queued-cell kernel that the device holds at once (all CUs).
 uint32_t resident_blocks_cells();
-hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
-hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                       hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                          hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream);
 

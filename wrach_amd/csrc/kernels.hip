@@ -41,6 +41,8 @@
 // a*b+c is a rounded multiply then a rounded add, and f32 division and
 // sqrtf are the correctly rounded sequences -- the native Rust semantics of
 // the reference's shader crate.
+#include <hip/hip_ext.h>
+
 #include "frame.h"
 
 namespace wrach {
@@ -1412,16 +1414,29 @@ __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
 
 }  // namespace
 
-hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    const uint32_t per_block = kCollideBlock * kCellsPerLane;
-    const uint32_t grid = (p.ncell + per_block - 1) / per_block;
-    hipLaunchKernelGGL(collide_kernel, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
+// The frame launchers take optional start/stop events, which hipExtLaunchKernel
+// stamps with the kernel's own start and end (wrach_step_profiled).
+template <typename K>
+static hipError_t launch(K kernel, uint32_t grid, uint32_t block, hipStream_t stream,
+                         hipEvent_t start, hipEvent_t stop, const FrameParams &p,
+                         const FrameBuffers &b) {
+    if (start || stop)
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, start, stop, 0, p, b);
+    else
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, p, b);
     return hipGetLastError();
 }
 
-hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    hipLaunchKernelGGL(cells_kernel, dim3(b.cells_grid), dim3(kCollideBlock), 0, stream, p, b);
-    return hipGetLastError();
+hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                          hipEvent_t start, hipEvent_t stop) {
+    const uint32_t per_block = kCollideBlock * kCellsPerLane;
+    return launch(collide_kernel, (p.ncell + per_block - 1) / per_block, kCollideBlock, stream,
+                  start, stop, p, b);
+}
+
+hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                        hipEvent_t start, hipEvent_t stop) {
+    return launch(cells_kernel, b.cells_grid, kCollideBlock, stream, start, stop, p, b);
 }
 
 static uint32_t resident_blocks(const void *kernel) {
@@ -1441,15 +1456,15 @@ hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    hipLaunchKernelGGL(scan_kernel, dim3(b.scan_tiles), dim3(kScanThreads), 0, stream, p, b);
-    return hipGetLastError();
+hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                       hipEvent_t start, hipEvent_t stop) {
+    return launch(scan_kernel, b.scan_tiles, kScanThreads, stream, start, stop, p, b);
 }
 
-hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
-    const uint32_t grid = (p.n + kScatterBlock - 1) / kScatterBlock;
-    hipLaunchKernelGGL(scatter_kernel, dim3(grid), dim3(kScatterBlock), 0, stream, p, b);
-    return hipGetLastError();
+hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                          hipEvent_t start, hipEvent_t stop) {
+    return launch(scatter_kernel, (p.n + kScatterBlock - 1) / kScatterBlock, kScatterBlock, stream,
+                  start, stop, p, b);
 }
 
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
