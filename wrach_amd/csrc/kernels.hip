@@ -1396,15 +1396,38 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
 
 // ---------------------------------------------------------------------------
 
+// kScatterPerLane particles per lane, strided across the wavefront: every
+// load of the dependent chain (kslot -> start offset -> destination) is in
+// flight for all of them at once.
 __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, FrameBuffers b) {
-    const uint32_t i = blockIdx.x * kScatterBlock + threadIdx.x;
-    if (i == 0) *b.epoch = *b.epoch + 1;  // the frame's scan has finished
-    if (i >= p.n) return;
-    const uint2 ks = b.kslot[i];
-    const uint32_t dest = b.idx[ks.x + 1] + (ks.y & kSrcMask);
-    b.pos[dest] = b.pos_out[i];
-    b.vel[dest] = b.vel_out[i];
-    b.src[dest] = i | (ks.y & kRunFlag);
+    const uint32_t w0 = (blockIdx.x * kScatterBlock + (threadIdx.x & ~63u)) * kScatterPerLane;
+    const uint32_t ln = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *b.epoch = *b.epoch + 1;  // the frame's scan has finished
+    uint2 ks[kScatterPerLane];
+    float2 q[kScatterPerLane], v[kScatterPerLane];
+    uint32_t start[kScatterPerLane];
+#pragma unroll
+    for (int j = 0; j < kScatterPerLane; ++j) {
+        const uint32_t i = w0 + 64 * j + ln;
+        if (i < p.n) {
+            ks[j] = b.kslot[i];
+            q[j] = b.pos_out[i];
+            v[j] = b.vel_out[i];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kScatterPerLane; ++j)
+        if (w0 + 64 * j + ln < p.n) start[j] = b.idx[ks[j].x + 1];
+#pragma unroll
+    for (int j = 0; j < kScatterPerLane; ++j) {
+        const uint32_t i = w0 + 64 * j + ln;
+        if (i < p.n) {
+            const uint32_t dest = start[j] + (ks[j].y & kSrcMask);
+            b.pos[dest] = q[j];
+            b.vel[dest] = v[j];
+            b.src[dest] = i | (ks[j].y & kRunFlag);
+        }
+    }
 }
 
 __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
@@ -1463,8 +1486,9 @@ hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t 
 
 hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                           hipEvent_t start, hipEvent_t stop) {
-    return launch(scatter_kernel, (p.n + kScatterBlock - 1) / kScatterBlock, kScatterBlock, stream,
-                  start, stop, p, b);
+    const uint32_t per_block = kScatterBlock * kScatterPerLane;
+    return launch(scatter_kernel, (p.n + per_block - 1) / per_block, kScatterBlock, stream, start,
+                  stop, p, b);
 }
 
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
