@@ -712,24 +712,45 @@ __device__ __forceinline__ void ranked_group(const FrameParams &p, const FrameBu
                                              float px, float py, float vx, float vy, uint32_t j,
                                              uint32_t n, uint32_t s, uint32_t c, uint32_t gbase);
 
+// One lane's share of a group item: its cell (s, n, c) and, in lane j of
+// the group, stored particle j.  group_load issues the loads; group_run
+// waits for them, so a wavefront can load its next item while it computes.
+struct GroupItem {
+    uint32_t s, n, c, key;
+    float2 q, w;
+};
+
 template <int G>
-__device__ __forceinline__ void group_cells(const FrameParams &p, const FrameBuffers &b, const uint4 *list,
-                            uint32_t count, uint32_t first) {
+__device__ __forceinline__ uint4 group_entry(const uint4 *list, uint32_t count) {
+    const uint32_t e = lane_id() / G;
+    return e < count ? list[e] : make_uint4(0, 0, 0, 0);
+}
+
+template <int G>
+__device__ __forceinline__ GroupItem group_load(const FrameBuffers &b, uint4 cell) {
+    const uint32_t j = lane_id() % G;
+    GroupItem it;
+    it.s = cell.x;
+    it.n = cell.y;
+    it.c = cell.z;
+    it.key = 0xffffffffu;
+    it.q = it.w = make_float2(0.0f, 0.0f);
+    if (j < it.n) {
+        it.key = b.src[it.s + j] & kSrcMask;
+        it.q = b.pos[it.s + j];
+        it.w = b.vel[it.s + j];
+    }
+    return it;
+}
+
+template <int G>
+__device__ __forceinline__ void group_run(const FrameParams &p, const FrameBuffers &b,
+                                          const GroupItem &it) {
     const uint32_t lane = lane_id();
     const uint32_t j = lane % G;
     const uint32_t gbase = lane - j;
-    const uint32_t e = first + lane / G;
-    uint4 cell = make_uint4(0, 0, 0, 0);
-    if (e < count) cell = list[e];
-    const uint32_t s = cell.x, n = cell.y, c = cell.z;
-
-    uint32_t key = 0xffffffffu;
-    float2 q = make_float2(0.0f, 0.0f), w = make_float2(0.0f, 0.0f);
-    if (j < n) {
-        key = b.src[s + j] & kSrcMask;
-        q = b.pos[s + j];
-        w = b.vel[s + j];
-    }
+    const uint32_t s = it.s, n = it.n, c = it.c, key = it.key;
+    const float2 q = it.q, w = it.w;
     // Source indices inside a cell are distinct; empty lanes keep their own
     // position after the n particles.
     uint32_t rank = 0;
@@ -1126,16 +1147,48 @@ __device__ __forceinline__ const uint4 *locate(const FrameBuffers &b, const Item
 
 // Work items [start, stop) of the global numbering that belong to group
 // class k (G = 2^(k+1) lanes per cell) and fall to wavefront gwave.
+// Items of 2..8-lane groups are short chains of dependent loads and a few
+// steps of work, and a wavefront takes several per frame before the
+// collapse: their loads are pipelined, the next item's cells and particles
+// in flight while the current one computes.
 template <int G>
 __device__ __forceinline__ void group_class(const FrameParams &p, const FrameBuffers &b,
                                             const ItemMap &m, int k, uint32_t start,
                                             uint32_t stop, uint32_t gwave, uint32_t nwaves) {
     uint32_t v = gwave;
     if (v < start) v += (start - v + nwaves - 1) / nwaves * nwaves;
-    for (; v < stop; v += nwaves) {
-        uint32_t avail;
-        const uint4 *list = locate(b, m, k, v - start, avail);
-        group_cells<G>(p, b, list, avail, 0);
+    if (v >= stop) return;
+    uint32_t avail;
+    const uint4 *list = locate(b, m, k, v - start, avail);
+    if (G > 8) {
+        for (;;) {
+            group_run<G>(p, b, group_load<G>(b, group_entry<G>(list, avail)));
+            v += nwaves;
+            if (v >= stop) break;
+            list = locate(b, m, k, v - start, avail);
+        }
+        return;
+    }
+    // Entries run two items ahead, particles one.
+    GroupItem cur = group_load<G>(b, group_entry<G>(list, avail));
+    uint4 ent = make_uint4(0, 0, 0, 0);
+    if (v + nwaves < stop) {
+        list = locate(b, m, k, v + nwaves - start, avail);
+        ent = group_entry<G>(list, avail);
+    }
+    for (;;) {
+        const uint32_t vn = v + nwaves;
+        const bool more = vn < stop;
+        GroupItem nxt;
+        if (more) nxt = group_load<G>(b, ent);
+        if (vn + nwaves < stop) {
+            list = locate(b, m, k, vn + nwaves - start, avail);
+            ent = group_entry<G>(list, avail);
+        }
+        group_run<G>(p, b, cur);
+        if (!more) break;
+        cur = nxt;
+        v = vn;
     }
 }
 
