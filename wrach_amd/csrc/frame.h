@@ -52,7 +52,7 @@ constexpr int kScanThreads = 512;
 constexpr int kScanItems = 16;            // per thread
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kScatterBlock = 256;
-constexpr int kScatterPerLane = 4;
+constexpr int kScatterPerLane = 2;
 constexpr int kMaxInCell = 9;            // cell.rs:29-30
 constexpr int kRegCell = 16;             // cells up to this size sort in registers
 constexpr int kLdsSort = 2048;           // keys per LDS bitonic sort (block path)
