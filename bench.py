@@ -13,6 +13,8 @@ events that hipExtLaunchKernel stamps with each kernel's start and end on
 the library's own stream (wrach_step_profiled).
 cpu_baseline: the oracle's multi-threaded frame on the host, on a bounded
 number of frames of the same workload.
+upload: wall time of one wrach_add_particles of the workload into a fresh
+context, binned on the device and packed on the host (not part of the step).
 """
 from __future__ import annotations
 
@@ -85,6 +87,25 @@ def copy_probe(nbytes: int) -> int:
     return 0
 
 
+def upload_times(wl) -> dict:
+    """Wall time of wrach_add_particles of the whole workload into a fresh
+    context: the store binned on the device (the default; the particles cross
+    PCIe once) and packed on the host (WRACH_FLAG_HOST_PACK; the packed arrays
+    cross PCIe).  Both include the host-to-device copies."""
+    from dataclasses import replace
+
+    from wrach_amd import World
+
+    out = {"particles": wl.n}
+    for name, host in (("device_ms", False), ("host_ms", True)):
+        with World(replace(wl.config, host_pack=host)) as w:
+            t0 = time.perf_counter()
+            w.add_particles(wl.pos, wl.vel)
+            w.sync()
+            out[name] = round((time.perf_counter() - t0) * 1e3, 3)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +163,8 @@ def main() -> int:
     world.close()
     print(f"[bench] kernel avg us: {json.dumps({k: round(v, 3) for k, v in avg_us.items()})}",
           file=sys.stderr, flush=True)
+    upload = upload_times(wl)
+    print(f"[bench] upload ms: {json.dumps(upload)}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -187,6 +210,7 @@ def main() -> int:
             "step_frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": cpu,
+        "upload": upload,
     }
     print(json.dumps(line), flush=True)
     return 0

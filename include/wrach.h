@@ -43,6 +43,12 @@ typedef enum wrach_status {
     WRACH_E_STATE = -7           /* device state failed a consistency check */
 } wrach_status;
 
+/* Config flags.  WRACH_FLAG_HOST_PACK: wrach_add_particles packs the store on
+ * the host, as ParticleStore::create_packed_data does (particle_store.rs:
+ * 90-103); without it the store is binned on the device (same packed
+ * triple, state.rs:90-101 semantics either way). */
+#define WRACH_FLAG_HOST_PACK 0x1u
+
 /* WrachConfig (runners/bevy/src/config_app.rs:10-35) plus extensions.
  * gravity = (0,0) and dt = 1 give the reference's step bit for bit. */
 typedef struct wrach_config {
@@ -53,7 +59,7 @@ typedef struct wrach_config {
     uint8_t reserved0;
     float gravity[2];                 /* extension: v += g*dt before integrate */
     float dt;                         /* extension: p += v*dt */
-    uint32_t flags;                   /* must be 0 */
+    uint32_t flags;                   /* WRACH_FLAG_* bits; 0 for the defaults */
     int32_t device;                   /* HIP device ordinal */
 } wrach_config;
 
@@ -100,10 +106,12 @@ void wrach_destroy(wrach_ctx *ctx);
 
 /* WrachAPI::add_particles (runners/api/src/lib.rs:78-81) ->
  * WrachState::add_particles (runners/bevy/src/state.rs:90-101): appends n
- * particles (pos_xy, vel_xy: 2n floats each) to the host store, re-packs the
+ * particles (pos_xy, vel_xy: 2n floats each) to the store, re-packs the
  * whole store (particle_store.rs:54-103, spatial_bin.rs:103-149; particles
- * outside the grid stay in the store but are not simulated) and uploads it,
- * replacing the device state.  On error the store is left unchanged. */
+ * outside the grid stay in the store but are not simulated), replacing the
+ * device state.  The store lives on the device and only the n new particles
+ * cross PCIe; with WRACH_FLAG_HOST_PACK it lives on the host and the packed
+ * arrays are uploaded.  On error the store is left unchanged. */
 int wrach_add_particles(wrach_ctx *ctx, const float *pos_xy,
                         const float *vel_xy, uint32_t n);
 

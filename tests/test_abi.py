@@ -26,6 +26,15 @@ def test_header_declares_the_python_export_list():
     assert declared_functions() == sorted(_lib.EXPORTS)
 
 
+def test_header_constants_match_python():
+    text = open(HEADER).read()
+    import re
+    flag = int(re.search(r"#define WRACH_FLAG_HOST_PACK (0x[0-9a-fA-F]+)u", text).group(1), 16)
+    assert flag == _lib.WRACH_FLAG_HOST_PACK
+    kernels = int(re.search(r"#define WRACH_NUM_FRAME_KERNELS (\d+)", text).group(1))
+    assert kernels == _lib.NUM_FRAME_KERNELS == len(_lib.FRAME_KERNEL_NAMES)
+
+
 def test_library_exports_every_declared_symbol(wrach_lib_path):
     lib = ctypes.CDLL(wrach_lib_path)
     for name in declared_functions():

@@ -254,6 +254,83 @@ def test_add_particles_repacks_the_store(oracle):
         assert_same_state(w.read(), ref, "repacked")
 
 
+def _upload_batches(cs, W, H, seed):
+    """Appends for the upload tests: uniform particles reaching outside the
+    world, particles on cell borders and one float below them, and special
+    coordinates (signed zeros, the far edges, infinities, huge and subnormal
+    negatives), each batch with finite velocities."""
+    r = np.random.default_rng(seed)
+    out = []
+    for n in (300, 1, 0, 250):
+        p = r.uniform(-5, max(W, H) + 8, (n, 2)).astype(np.float32)
+        q = n // 3
+        border = (r.integers(-1, max(W, H) // cs + 2, (q, 2)) * cs).astype(np.float32)
+        p[:q] = border
+        p[q:2 * q] = np.nextafter(border, np.float32(-np.inf))
+        out.append((p, r.uniform(-1, 1, (n, 2)).astype(np.float32)))
+    special = f32([[0.0, 0.0], [-0.0, -0.0], [W, H], [W, 0], [0, H], [np.inf, 1], [-np.inf, 1],
+                   [1, np.inf], [3e38, 2], [-3e38, 2], [-1e-30, 5], [5, -1e-30], [-1e-40, 5],
+                   [5, -1e-40], [1e-40, 1e-40], [3 * cs, 2 * cs], [-cs, cs]])
+    out.append((special, f32(r.uniform(-1, 1, special.shape))))
+    return out
+
+
+@pytest.mark.parametrize("cs", [1, 3, 7])
+def test_device_upload_matches_host_packer(oracle, cs):
+    """The device binning of the store (the default) against the host packer
+    (WRACH_FLAG_HOST_PACK) and the oracle's pack (spatial_bin.rs:48-149),
+    after every append; then frames from both states agree bit for bit."""
+    W, H = 61, 47
+    say(f"[upload] device vs host packer, cs={cs} dims=({W},{H})")
+    cfg_d = WrachConfig(dimensions=(W, H), cell_size=cs, gravity=(0.0, -0.1))
+    cfg_h = WrachConfig(dimensions=(W, H), cell_size=cs, gravity=(0.0, -0.1), host_pack=True)
+    allp, allv = [], []
+    with World(cfg_d) as wd, World(cfg_h) as wh:
+        for k, (p, v) in enumerate(_upload_batches(cs, W, H, 200 + cs)):
+            wd.add_particles(p, v)
+            wh.add_particles(p, v)
+            allp.append(p)
+            allv.append(v)
+            _, ref = oracle_world(oracle, cfg_d, np.concatenate(allp), np.concatenate(allv))
+            assert_same_state(wd.read(), ref, f"device pack after append {k}")
+            assert_same_state(wh.read(), ref, f"host pack after append {k}")
+        P, _ = oracle_world(oracle, cfg_d, np.concatenate(allp), np.concatenate(allv))
+        wd.step(6)
+        wh.step(6)
+        ref = oracle.step(P, *ref, nsteps=6)
+        assert_same_state(wd.read(), ref, "6 frames after the device pack")
+        assert_same_state(wh.read(), ref, "6 frames after the host pack")
+
+
+def test_device_upload_full_cells_and_capacity(oracle):
+    """Device binning of crowded cells (one particle per store index, so the
+    first frame orders up to hundreds of one-particle runs per cell) and a
+    capacity error that leaves store and state as they were."""
+    say("[upload] crowded cells and capacity")
+    r = np.random.default_rng(9)
+    cfg = WrachConfig(dimensions=(40, 30), cell_size=1, gravity=(0.0, -0.1))
+    with World(cfg) as w:
+        cap = w.capacity
+        cells = r.integers(0, 8, (cap - 200,))
+        p = f32(np.stack([cells * 4.0 + 0.25, np.full(len(cells), 3.5)], 1))
+        p += f32(r.uniform(0, 0.5, p.shape))
+        v = f32(r.uniform(-1, 1, p.shape))
+        w.add_particles(p, v)
+        _, ref = oracle_world(oracle, cfg, p, v)
+        assert_same_state(w.read(), ref, "crowded upload")
+        with pytest.raises(WrachError) as e:
+            w.add_particles(f32(r.uniform(0, 40, (300, 2))), f32(np.zeros((300, 2))))
+        assert e.value.code == _lib.WRACH_E_CAPACITY
+        assert_same_state(w.read(), ref, "after the refused append")
+        P, _ = oracle_world(oracle, cfg, p, v)
+        w.step(4)
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=4), "4 frames")
+        extra_p, extra_v = f32(r.uniform(0, 40, (150, 2))), f32(r.uniform(-1, 1, (150, 2)))
+        w.add_particles(extra_p, extra_v)
+        _, ref = oracle_world(oracle, cfg, np.concatenate([p, extra_p]), np.concatenate([v, extra_v]))
+        assert_same_state(w.read(), ref, "append after the refusal")
+
+
 def test_graph_replay_matches_plain_launches(oracle):
     """Replays of the captured frame graph, with eager work and readbacks in
     between, against plain stream launches."""

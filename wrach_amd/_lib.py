@@ -19,6 +19,7 @@ WRACH_E_NONFINITE = -4
 WRACH_E_DEVICE = -5
 WRACH_E_BUFFER = -6
 WRACH_E_STATE = -7
+WRACH_FLAG_HOST_PACK = 0x1
 NUM_FRAME_KERNELS = 4
 FRAME_KERNEL_NAMES = ("collide", "cells", "scan", "scatter")
 

@@ -24,7 +24,9 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 class WrachConfig:
     """WrachConfig (config_app.rs:10-35); defaults of config_app.rs:26-35.
 
-    gravity and dt are extensions: (0, 0) and 1.0 reproduce the reference."""
+    gravity and dt are extensions: (0, 0) and 1.0 reproduce the reference.
+    host_pack packs uploads on the host (the reference's CPU packer) instead
+    of binning the store on the device; the packed triple is the same."""
 
     dimensions: tuple = (480, 352)
     boundaries_as_dimensions: bool = False
@@ -32,6 +34,7 @@ class WrachConfig:
     gravity: tuple = (0.0, 0.0)
     dt: float = 1.0
     device: int = 0
+    host_pack: bool = False
 
     def to_c(self) -> _lib.WrachConfigC:
         c = _lib.WrachConfigC()
@@ -42,6 +45,7 @@ class WrachConfig:
         c.gravity[0], c.gravity[1] = float(self.gravity[0]), float(self.gravity[1])
         c.dt = float(self.dt)
         c.device = int(self.device)
+        c.flags = _lib.WRACH_FLAG_HOST_PACK if self.host_pack else 0
         return c
 
 

@@ -82,10 +82,17 @@ struct wrach_ctx {
     hipGraphExec_t graph_many = nullptr;
     bool use_graph = true;
     bool canonical_valid = false;
-    // Host store: every particle ever added, in insertion order.  Cells of
+    // The store: every particle ever added, in insertion order.  Cells of
     // the hashmap store keep insertion order, so a stable sort by cell of
-    // this list is the packed layout (particle_store.rs:54-59).
+    // this list is the packed layout (particle_store.rs:54-59).  On the host
+    // with WRACH_FLAG_HOST_PACK, else on the device (dstore_*), where dks
+    // holds each store particle's (cell, slot) during a re-pack.
+    bool host_pack = false;
     std::vector<float> store_pos, store_vel;
+    float2 *dstore_pos = nullptr, *dstore_vel = nullptr;
+    uint2 *dks = nullptr;
+    uint32_t *dvalid = nullptr;
+    size_t dstore_n = 0, dstore_cap = 0;
 };
 
 namespace {
@@ -138,6 +145,10 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.cnt);
     (void)hipFree(c->buf.tile_state);
     (void)hipFree(c->buf.epoch);
+    (void)hipFree(c->dstore_pos);
+    (void)hipFree(c->dstore_vel);
+    (void)hipFree(c->dks);
+    (void)hipFree(c->dvalid);
     (void)hipFree(c->buf.cells);
     (void)hipFree(c->buf.class_ctr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -186,6 +197,124 @@ uint32_t pack_store(const wrach_ctx *c, std::vector<uint32_t> &indices,
     return n;
 }
 
+
+// wrach_add_particles with WRACH_FLAG_HOST_PACK: the store on the host,
+// packed there (pack_store) and the packed arrays uploaded.
+int add_host_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint32_t n) {
+    const size_t old = c->store_pos.size();
+    c->store_pos.insert(c->store_pos.end(), pos_xy, pos_xy + 2 * (size_t)n);
+    c->store_vel.insert(c->store_vel.end(), vel_xy, vel_xy + 2 * (size_t)n);
+    std::vector<uint32_t> indices;
+    std::vector<float> pos, vel;
+    const uint32_t packed = pack_store(c, indices, pos, vel);
+    if (packed > c->capacity) {
+        c->store_pos.resize(old);
+        c->store_vel.resize(old);
+        return fail(WRACH_E_CAPACITY, "particles in frame (" + std::to_string(packed) +
+                                          ") exceed the buffer capacity (" +
+                                          std::to_string(c->capacity) + ")");
+    }
+    WRACH_HIP(hipMemcpyAsync(c->buf.idx, indices.data(), indices.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, c->stream));
+    if (packed) {
+        WRACH_HIP(hipMemcpyAsync(c->buf.pos, pos.data(), pos.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+        WRACH_HIP(hipMemcpyAsync(c->buf.vel, vel.data(), vel.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    WRACH_HIP(wrach::launch_iota(c->buf.src, packed, c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (c->fp.n != packed) free_graphs(c);
+    c->fp.n = packed;
+    c->settings.particles_in_frame_count = packed;
+    return WRACH_OK;
+}
+
+// Room for `total` store particles on the device, keeping the first
+// dstore_n.  Grows by doubling.
+int reserve_device_store(wrach_ctx *c, size_t total) {
+    if (total <= c->dstore_cap) return WRACH_OK;
+    size_t cap = c->dstore_cap ? c->dstore_cap : 1024;
+    while (cap < total) cap *= 2;
+    float2 *p = nullptr, *v = nullptr;
+    uint2 *k = nullptr;
+    hipError_t e = hipMalloc(&p, cap * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&v, cap * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&k, cap * sizeof(uint2));
+    if (e == hipSuccess && c->dstore_n) {
+        e = hipMemcpyAsync(p, c->dstore_pos, c->dstore_n * sizeof(float2), hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(v, c->dstore_vel, c->dstore_n * sizeof(float2), hipMemcpyDeviceToDevice,
+                               c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        (void)hipFree(v);
+        (void)hipFree(k);
+        return fail(WRACH_E_DEVICE, std::string("device store: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(c->dstore_pos);
+    (void)hipFree(c->dstore_vel);
+    (void)hipFree(c->dks);
+    c->dstore_pos = p;
+    c->dstore_vel = v;
+    c->dks = k;
+    c->dstore_cap = cap;
+    return WRACH_OK;
+}
+
+// wrach_add_particles without WRACH_FLAG_HOST_PACK: the new particles are
+// appended to the device store, which is binned on the device (cell and slot
+// per particle, the frame's scan, a scatter).  Nothing of the state changes
+// before the capacity check has passed.
+int add_device_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint32_t n) {
+    const size_t total = c->dstore_n + n;
+    // a store index and the run flag share a 32-bit source word
+    if (total > (size_t)0x7fffffffu) return fail(WRACH_E_CAPACITY, "store exceeds 2^31 - 1 particles");
+    {
+        const int rc = reserve_device_store(c, total);
+        if (rc) return rc;
+    }
+    if (n) {
+        WRACH_HIP(hipMemcpyAsync(c->dstore_pos + c->dstore_n, pos_xy, (size_t)n * sizeof(float2),
+                                 hipMemcpyHostToDevice, c->stream));
+        WRACH_HIP(hipMemcpyAsync(c->dstore_vel + c->dstore_n, vel_xy, (size_t)n * sizeof(float2),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    wrach::UploadParams u{};
+    u.pos = c->dstore_pos;
+    u.vel = c->dstore_vel;
+    u.ks = c->dks;
+    u.valid = c->dvalid;
+    u.n = (uint32_t)total;
+    u.cs = (float)c->cfg.cell_size;
+    u.gx = c->settings.grid_dimensions[0];
+    u.gy = c->settings.grid_dimensions[1];
+    u.cap = c->capacity;
+    WRACH_HIP(hipMemsetAsync(c->dvalid, 0, sizeof(uint32_t), c->stream));
+    WRACH_HIP(wrach::launch_upload_bin(u, c->buf, c->stream));
+    uint32_t packed = 0;
+    WRACH_HIP(hipMemcpyAsync(&packed, c->dvalid, sizeof packed, hipMemcpyDeviceToHost, c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (packed > c->capacity) {
+        // the counts are the only state touched: clear them for the next frame
+        WRACH_HIP(hipMemsetAsync(c->buf.cnt, 0, (c->ncell + 16) * sizeof(uint32_t), c->stream));
+        WRACH_HIP(hipStreamSynchronize(c->stream));
+        return fail(WRACH_E_CAPACITY, "particles in frame (" + std::to_string(packed) +
+                                          ") exceed the buffer capacity (" +
+                                          std::to_string(c->capacity) + ")");
+    }
+    WRACH_HIP(wrach::launch_scan(c->fp, c->buf, c->stream));
+    WRACH_HIP(wrach::launch_upload_scatter(u, c->buf, c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    c->dstore_n = total;
+    if (c->fp.n != packed) free_graphs(c);
+    c->fp.n = packed;
+    c->settings.particles_in_frame_count = packed;
+    return WRACH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -208,7 +337,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     if (!cfg || !out) return fail(WRACH_E_INVALID, "wrach_create: null argument");
     *out = nullptr;
     if (cfg->cell_size == 0) return fail(WRACH_E_INVALID, "cell_size must be >= 1");
-    if (cfg->flags != 0) return fail(WRACH_E_INVALID, "unknown flags");
+    if (cfg->flags & ~WRACH_FLAG_HOST_PACK) return fail(WRACH_E_INVALID, "unknown flags");
     if (!std::isfinite(cfg->gravity[0]) || !std::isfinite(cfg->gravity[1]) || !std::isfinite(cfg->dt))
         return fail(WRACH_E_INVALID, "gravity and dt must be finite");
     int ndev = 0;
@@ -231,6 +360,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
 
     wrach_ctx *c = new wrach_ctx();
     c->cfg = *cfg;
+    c->host_pack = (cfg->flags & WRACH_FLAG_HOST_PACK) != 0;
     c->ncell = ncell;
     c->capacity = cap;
     c->settings.view_dimensions[0] = (float)cfg->width;
@@ -305,6 +435,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.epoch, 1));
     WRACH_HIP_C(dmalloc(&b.cells, ncells_total));
     WRACH_HIP_C(dmalloc(&b.class_ctr, wrach::kCtrWords));
+    WRACH_HIP_C(dmalloc(&c->dvalid, 1));
     WRACH_HIP_C(hipMemsetAsync(c->idx_alloc, 0, (nidx + 64) * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.pos, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.vel, 0, cap * sizeof(float2), c->stream));
@@ -333,34 +464,10 @@ int wrach_add_particles(wrach_ctx *c, const float *pos_xy, const float *vel_xy, 
     for (uint32_t i = 0; i < 2 * n; ++i)
         if (std::isnan(pos_xy[i]) || std::isnan(vel_xy[i]))
             return fail(WRACH_E_NONFINITE, "NaN in particle " + std::to_string(i / 2));
-    const size_t old = c->store_pos.size();
-    c->store_pos.insert(c->store_pos.end(), pos_xy, pos_xy + 2 * (size_t)n);
-    c->store_vel.insert(c->store_vel.end(), vel_xy, vel_xy + 2 * (size_t)n);
-    std::vector<uint32_t> indices;
-    std::vector<float> pos, vel;
-    const uint32_t packed = pack_store(c, indices, pos, vel);
-    if (packed > c->capacity) {
-        c->store_pos.resize(old);
-        c->store_vel.resize(old);
-        return fail(WRACH_E_CAPACITY, "particles in frame (" + std::to_string(packed) +
-                                          ") exceed the buffer capacity (" +
-                                          std::to_string(c->capacity) + ")");
-    }
     WRACH_HIP(hipSetDevice(c->cfg.device));
     WRACH_HIP(hipStreamSynchronize(c->stream));
-    WRACH_HIP(hipMemcpyAsync(c->buf.idx, indices.data(), indices.size() * sizeof(uint32_t),
-                             hipMemcpyHostToDevice, c->stream));
-    if (packed) {
-        WRACH_HIP(hipMemcpyAsync(c->buf.pos, pos.data(), pos.size() * sizeof(float),
-                                 hipMemcpyHostToDevice, c->stream));
-        WRACH_HIP(hipMemcpyAsync(c->buf.vel, vel.data(), vel.size() * sizeof(float),
-                                 hipMemcpyHostToDevice, c->stream));
-    }
-    WRACH_HIP(wrach::launch_iota(c->buf.src, packed, c->stream));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
-    if (c->fp.n != packed) free_graphs(c);
-    c->fp.n = packed;
-    c->settings.particles_in_frame_count = packed;
+    const int rc = c->host_pack ? add_host_packed(c, pos_xy, vel_xy, n) : add_device_packed(c, pos_xy, vel_xy, n);
+    if (rc) return rc;
     c->canonical_valid = false;
     g_err.clear();
     return WRACH_OK;

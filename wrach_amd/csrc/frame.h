@@ -104,4 +104,23 @@ hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream);
 
+// Upload binning on the device: the store (every particle added, in
+// insertion order) and where its particles go.
+struct UploadParams {
+    const float2 *pos;  // store positions
+    const float2 *vel;  // store velocities
+    uint2 *ks;          // per store particle: (cell key or ~0, slot in the cell)
+    uint32_t *valid;    // particles inside the grid (one counter)
+    uint32_t n;         // store size, < 2^31
+    float cs;           // cell side
+    uint32_t gx, gy;    // grid dimensions
+    uint32_t cap;       // packed-array capacity
+};
+// Cell and slot of every store particle, counts into b.cnt, inside-grid
+// total into u.valid (zeroed by the caller).
+hipError_t launch_upload_bin(const UploadParams &u, const FrameBuffers &b, hipStream_t stream);
+// After launch_scan: copy each particle to idx[key + 1] + slot, marked as a
+// run of its own (src = store index | run flag).
+hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, hipStream_t stream);
+
 }  // namespace wrach

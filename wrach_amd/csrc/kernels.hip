@@ -1488,6 +1488,62 @@ __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
     if (i < n) dst[i] = i;
 }
 
+// ---------------------------------------------------------------------------
+// Upload binning (SpatialBin::create_packed_data, spatial_bin.rs:103-149, on
+// the device).  The CPU packer is a stable counting sort of the store by
+// cell; here each particle takes a slot with one atomic, so a cell's
+// particles land in arbitrary order, each marked as a run of its own.  The
+// order inside a cell is then the one every frame already restores (runs
+// ordered by source index, here the store index), and the readback's
+// canonical pass applies it as well: the packed triple is the CPU packer's.
+
+// Rust's `f as i32`: saturating, NaN -> 0.
+__device__ __forceinline__ int32_t saturating_i32(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return 0x7fffffff;
+    if (f <= -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)f;
+}
+
+// f32::div_euclid for b > 0 (spatial_bin.rs:48-64): the truncated IEEE
+// quotient, one less when the remainder is negative.
+__device__ __forceinline__ float div_euclid_pos(float a, float b) {
+    const float q = truncf(a / b);
+    return fmodf(a, b) < 0.0f ? q - 1.0f : q;
+}
+
+__global__ __launch_bounds__(256) void upload_bin_kernel(UploadParams u, FrameBuffers b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    if (i < u.n) {
+        const float2 q = u.pos[i];
+        const int32_t cx = saturating_i32(div_euclid_pos(q.x, u.cs));
+        const int32_t cy = saturating_i32(div_euclid_pos(q.y, u.cs));
+        in = cx >= 0 && cy >= 0 && (uint32_t)cx < u.gx && (uint32_t)cy < u.gy;
+        uint2 ks = make_uint2(0xffffffffu, 0u);
+        if (in) {
+            ks.x = (uint32_t)cy * u.gx + (uint32_t)cx;
+            ks.y = atomicAdd(&b.cnt[ks.x], 1u);
+        }
+        u.ks[i] = ks;
+    }
+    const unsigned long long m = __ballot(in);
+    if (lane_id() == 0 && m != 0) atomicAdd(u.valid, (uint32_t)__popcll(m));
+}
+
+__global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, FrameBuffers b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *b.epoch = *b.epoch + 1;  // the scan has finished
+    if (i >= u.n) return;
+    const uint2 ks = u.ks[i];
+    if (ks.x == 0xffffffffu) return;
+    const uint32_t dest = b.idx[ks.x + 1] + ks.y;
+    if (dest >= u.cap) return;  // never, with a consistent scan
+    b.pos[dest] = u.pos[i];
+    b.vel[dest] = u.vel[i];
+    b.src[dest] = i | kRunFlag;
+}
+
 }  // namespace
 
 // The frame launchers take optional start/stop events, which hipExtLaunchKernel
@@ -1547,6 +1603,18 @@ hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_upload_bin(const UploadParams &u, const FrameBuffers &b, hipStream_t stream) {
+    if (u.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(upload_bin_kernel, dim3((u.n + 255) / 256), dim3(256), 0, stream, u, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, hipStream_t stream) {
+    const uint32_t blocks = u.n ? (u.n + 255) / 256 : 1;  // block 0 advances the epoch
+    hipLaunchKernelGGL(upload_scatter_kernel, dim3(blocks), dim3(256), 0, stream, u, b);
     return hipGetLastError();
 }
 
