@@ -150,6 +150,25 @@ int wrach_device_state(wrach_ctx *ctx, wrach_device_view *out);
  * pointers stay valid until the next wrach_step / wrach_read. */
 int wrach_canonical_device(wrach_ctx *ctx, wrach_device_view *out);
 
+/* ParticleStore::update_from_gpu, a TODO in the reference
+ * (particle_store.rs:74-85, with cells_to_read_from_gpu, :22-26): the store
+ * becomes the simulated state -- the frame's particles in canonical packed
+ * order, plus the store's particles outside the grid, which are never
+ * simulated -- so that a later wrach_add_particles continues from the
+ * simulated state instead of restarting from the uploads.  The device state
+ * is unchanged.  Synchronises. */
+int wrach_store_write_back(wrach_ctx *ctx);
+
+/* Restores a packed triple saved with wrach_read (PackedData,
+ * spatial_bin.rs:22-33; the TODO "Save GPU data to CPU memory and disk",
+ * particle_store.rs:74-76).  n_indices must be the grid cells + 2 and
+ * indices [0, 0, ...] non-decreasing up to n_particles <= capacity; NaN is
+ * rejected as in wrach_add_particles.  The state and the store become the
+ * triple's particles (the frames take each index range as that cell's
+ * particles, as the reference's pass does).  On error nothing changes. */
+int wrach_load_packed(wrach_ctx *ctx, const uint32_t *indices, size_t n_indices,
+                      const float *pos_xy, const float *vel_xy, uint32_t n_particles);
+
 /* 1 (default): frames are replayed from a captured hipGraph; 0: plain
  * stream launches. */
 int wrach_set_graph(wrach_ctx *ctx, int enable);

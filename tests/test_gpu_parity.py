@@ -331,6 +331,90 @@ def test_device_upload_full_cells_and_capacity(oracle):
         assert_same_state(w.read(), ref, "append after the refusal")
 
 
+def test_checkpoint_round_trip(oracle):
+    """A packed triple saved with read() and restored into a fresh context
+    (wrach_load_packed) continues exactly as the original does."""
+    wl = workloads.small_uniform(seed=5)
+    say("[checkpoint] save, restore, continue")
+    w_, h_ = wl.config.dimensions
+    with World(wl.config) as a, World(wl.config) as b:
+        a.add_particles(wl.pos, wl.vel)
+        a.step(12)
+        idx, pos, vel = a.read()
+        b.load_packed(idx, pos, vel)
+        assert_same_state(b.read(), (idx, pos, vel), "restored")
+        a.step(9)
+        b.step(9)
+        P = oracle.params(w_, h_, wl.config.cell_size, len(pos), gravity=wl.config.gravity,
+                          dt=wl.config.dt)
+        ref = oracle.step(P, idx, pos, vel, nsteps=9)
+        assert_same_state(a.read(), ref, "original after 9 more frames")
+        assert_same_state(b.read(), ref, "restored after 9 frames")
+        # the restored context's store is the checkpoint: a re-pack gives it back
+        b.load_packed(idx, pos, vel)
+        b.add_particles(f32(np.zeros((0, 2))), f32(np.zeros((0, 2))))
+        assert_same_state(b.read(), (idx, pos, vel), "re-pack of the restored store")
+
+
+def test_load_packed_rejects_bad_triples(oracle):
+    say("[checkpoint] rejected triples leave the state")
+    cfg = WrachConfig(dimensions=(20, 12), cell_size=1)
+    r = np.random.default_rng(31)
+    with World(cfg) as w:
+        w.add_particles(f32(r.uniform(0, 20, (120, 2))), f32(r.uniform(-1, 1, (120, 2))))
+        w.step(3)
+        idx, pos, vel = w.read()
+        before = (idx.copy(), pos.copy(), vel.copy())
+        k = int(np.flatnonzero(np.diff(idx.astype(np.int64)) > 0)[1]) + 1
+        bad_order = idx.copy()
+        bad_order[k] = bad_order[k + 1] + 1
+        cases = [
+            ("short indices", idx[:-1], pos, vel, _lib.WRACH_E_INVALID),
+            ("first offset", np.concatenate([[1], idx[1:]]), pos, vel, _lib.WRACH_E_INVALID),
+            ("decreasing", bad_order, pos, vel, _lib.WRACH_E_INVALID),
+            ("last offset", idx, pos[:-1], vel[:-1], _lib.WRACH_E_INVALID),
+            ("NaN", idx, np.where(np.arange(len(pos))[:, None] == 5, np.nan, pos), vel,
+             _lib.WRACH_E_NONFINITE),
+        ]
+        cap = w.capacity
+        big = np.zeros_like(idx)
+        big[2:] = cap + 1
+        cases.append(("over capacity", big, np.ones((cap + 1, 2)), np.zeros((cap + 1, 2)),
+                      _lib.WRACH_E_CAPACITY))
+        for name, i_, p_, v_, code in cases:
+            with pytest.raises(WrachError) as e:
+                w.load_packed(i_, p_, v_)
+            assert e.value.code == code, name
+            assert_same_state(w.read(), before, f"after the rejected {name}")
+
+
+@pytest.mark.parametrize("host", [False, True], ids=["device-store", "host-store"])
+def test_store_write_back_then_append(oracle, host):
+    """wrach_store_write_back (update_from_gpu, particle_store.rs:74-85): the
+    store becomes the simulated particles in canonical order, so a re-pack
+    reproduces the state and an append continues from it."""
+    say(f"[write-back] host_pack={host}")
+    cfg = WrachConfig(dimensions=(40, 30), cell_size=1, gravity=(0.0, -0.1), host_pack=host)
+    r = np.random.default_rng(52)
+    a_p = f32(r.uniform(-10, 50, (400, 2)))
+    a_v = f32(r.uniform(-1, 1, (400, 2)))
+    b_p = f32(r.uniform(0, 40, (80, 2)))
+    b_v = f32(r.uniform(-1, 1, (80, 2)))
+    with World(cfg) as w:
+        w.add_particles(a_p, a_v)
+        w.step(8)
+        idx8, p8, v8 = w.read()
+        w.store_write_back()
+        assert_same_state(w.read(), (idx8, p8, v8), "write-back leaves the state")
+        w.add_particles(f32(np.zeros((0, 2))), f32(np.zeros((0, 2))))
+        assert_same_state(w.read(), (idx8, p8, v8), "re-pack of the written-back store")
+        w.add_particles(b_p, b_v)
+        P, ref = oracle_world(oracle, cfg, np.concatenate([p8, b_p]), np.concatenate([v8, b_v]))
+        assert_same_state(w.read(), ref, "append after the write-back")
+        w.step(5)
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=5), "5 frames after the append")
+
+
 def test_graph_replay_matches_plain_launches(oracle):
     """Replays of the captured frame graph, with eager work and readbacks in
     between, against plain stream launches."""

@@ -36,6 +36,8 @@ EXPORTS = (
     "wrach_sync",
     "wrach_device_state",
     "wrach_canonical_device",
+    "wrach_store_write_back",
+    "wrach_load_packed",
     "wrach_set_graph",
     "wrach_step_profiled",
     "wrach_last_error",
@@ -113,6 +115,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_sync.argtypes = [vp]
     L.wrach_device_state.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
     L.wrach_canonical_device.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
+    L.wrach_store_write_back.argtypes = [vp]
+    L.wrach_load_packed.argtypes = [vp, u32p, ctypes.c_size_t, f32p, f32p, ctypes.c_uint32]
     L.wrach_set_graph.argtypes = [vp, ctypes.c_int]
     L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.wrach_last_error.argtypes = []

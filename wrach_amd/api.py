@@ -131,6 +131,21 @@ class World:
     def step(self, nframes: int = 1) -> None:
         check(self._L.wrach_step(self._h, int(nframes)))
 
+    def store_write_back(self) -> None:
+        """The store becomes the simulated state (wrach_store_write_back)."""
+        check(self._L.wrach_store_write_back(self._h))
+
+    def load_packed(self, indices, pos, vel) -> None:
+        """Restores a packed triple saved with read() (wrach_load_packed)."""
+        indices = np.ascontiguousarray(indices, np.uint32)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 2)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 2)
+        if pos.shape != vel.shape:
+            raise ValueError("positions and velocities differ in length")
+        check(self._L.wrach_load_packed(self._h, indices.ctypes.data_as(_u32p), len(indices),
+                                        pos.ctypes.data_as(_f32p), vel.ctypes.data_as(_f32p),
+                                        len(pos)))
+
     def sync(self) -> None:
         check(self._L.wrach_sync(self._h))
 

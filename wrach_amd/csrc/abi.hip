@@ -230,18 +230,54 @@ int add_host_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint
     return WRACH_OK;
 }
 
+// Device store arrays: room for `cap` particles, grown by doubling.
+size_t store_cap_for(size_t n) {
+    size_t cap = 1024;
+    while (cap < n) cap *= 2;
+    return cap;
+}
+
+int alloc_store(size_t cap, float2 **p, float2 **v, uint2 **k) {
+    *p = *v = nullptr;
+    *k = nullptr;
+    hipError_t e = hipMalloc(p, cap * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(v, cap * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(k, cap * sizeof(uint2));
+    if (e != hipSuccess) {
+        (void)hipFree(*p);
+        (void)hipFree(*v);
+        (void)hipFree(*k);
+        *p = *v = nullptr;
+        *k = nullptr;
+        return fail(WRACH_E_DEVICE, std::string("device store: ") + hipGetErrorString(e));
+    }
+    return WRACH_OK;
+}
+
+// Replaces the store arrays (the old ones are no longer in use).
+void swap_store(wrach_ctx *c, float2 *p, float2 *v, uint2 *k, size_t cap) {
+    (void)hipFree(c->dstore_pos);
+    (void)hipFree(c->dstore_vel);
+    (void)hipFree(c->dks);
+    c->dstore_pos = p;
+    c->dstore_vel = v;
+    c->dks = k;
+    c->dstore_cap = cap;
+}
+
 // Room for `total` store particles on the device, keeping the first
-// dstore_n.  Grows by doubling.
+// dstore_n.
 int reserve_device_store(wrach_ctx *c, size_t total) {
     if (total <= c->dstore_cap) return WRACH_OK;
-    size_t cap = c->dstore_cap ? c->dstore_cap : 1024;
-    while (cap < total) cap *= 2;
-    float2 *p = nullptr, *v = nullptr;
-    uint2 *k = nullptr;
-    hipError_t e = hipMalloc(&p, cap * sizeof(float2));
-    if (e == hipSuccess) e = hipMalloc(&v, cap * sizeof(float2));
-    if (e == hipSuccess) e = hipMalloc(&k, cap * sizeof(uint2));
-    if (e == hipSuccess && c->dstore_n) {
+    const size_t cap = store_cap_for(total);
+    float2 *p, *v;
+    uint2 *k;
+    {
+        const int rc = alloc_store(cap, &p, &v, &k);
+        if (rc) return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (c->dstore_n) {
         e = hipMemcpyAsync(p, c->dstore_pos, c->dstore_n * sizeof(float2), hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(v, c->dstore_vel, c->dstore_n * sizeof(float2), hipMemcpyDeviceToDevice,
@@ -254,14 +290,23 @@ int reserve_device_store(wrach_ctx *c, size_t total) {
         (void)hipFree(k);
         return fail(WRACH_E_DEVICE, std::string("device store: ") + hipGetErrorString(e));
     }
-    (void)hipFree(c->dstore_pos);
-    (void)hipFree(c->dstore_vel);
-    (void)hipFree(c->dks);
-    c->dstore_pos = p;
-    c->dstore_vel = v;
-    c->dks = k;
-    c->dstore_cap = cap;
+    swap_store(c, p, v, k, cap);
     return WRACH_OK;
+}
+
+// The binning parameters over the first n store particles.
+wrach::UploadParams store_params(const wrach_ctx *c, size_t n) {
+    wrach::UploadParams u{};
+    u.pos = c->dstore_pos;
+    u.vel = c->dstore_vel;
+    u.ks = c->dks;
+    u.valid = c->dvalid;
+    u.n = (uint32_t)n;
+    u.cs = (float)c->cfg.cell_size;
+    u.gx = c->settings.grid_dimensions[0];
+    u.gy = c->settings.grid_dimensions[1];
+    u.cap = c->capacity;
+    return u;
 }
 
 // wrach_add_particles without WRACH_FLAG_HOST_PACK: the new particles are
@@ -282,16 +327,7 @@ int add_device_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, ui
         WRACH_HIP(hipMemcpyAsync(c->dstore_vel + c->dstore_n, vel_xy, (size_t)n * sizeof(float2),
                                  hipMemcpyHostToDevice, c->stream));
     }
-    wrach::UploadParams u{};
-    u.pos = c->dstore_pos;
-    u.vel = c->dstore_vel;
-    u.ks = c->dks;
-    u.valid = c->dvalid;
-    u.n = (uint32_t)total;
-    u.cs = (float)c->cfg.cell_size;
-    u.gx = c->settings.grid_dimensions[0];
-    u.gy = c->settings.grid_dimensions[1];
-    u.cap = c->capacity;
+    const wrach::UploadParams u = store_params(c, total);
     WRACH_HIP(hipMemsetAsync(c->dvalid, 0, sizeof(uint32_t), c->stream));
     WRACH_HIP(wrach::launch_upload_bin(u, c->buf, c->stream));
     uint32_t packed = 0;
@@ -590,6 +626,132 @@ int wrach_canonical_device(wrach_ctx *c, wrach_device_view *out) {
     out->n_particles = c->fp.n;
     out->n_indices = (uint32_t)(c->ncell + 2);
     out->stream = c->stream;
+    return WRACH_OK;
+}
+
+int wrach_store_write_back(wrach_ctx *c) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    wrach_device_view view{};
+    {
+        const int rc = wrach_canonical_device(c, &view);
+        if (rc) return rc;
+    }
+    const uint32_t n = c->fp.n;
+    if (c->host_pack) {
+        // the frame's particles, then the old store's particles outside the grid
+        std::vector<float> sp(2 * (size_t)n), sv(2 * (size_t)n);
+        if (n) {
+            WRACH_HIP(hipMemcpyAsync(sp.data(), view.positions, sp.size() * sizeof(float),
+                                     hipMemcpyDeviceToHost, c->stream));
+            WRACH_HIP(hipMemcpyAsync(sv.data(), view.velocities, sv.size() * sizeof(float),
+                                     hipMemcpyDeviceToHost, c->stream));
+        }
+        WRACH_HIP(hipStreamSynchronize(c->stream));
+        const uint32_t gx = c->settings.grid_dimensions[0], gy = c->settings.grid_dimensions[1];
+        const float cs = (float)c->cfg.cell_size;
+        for (size_t i = 0; i < c->store_pos.size() / 2; ++i) {
+            int32_t cx, cy;
+            cell_coord(c->store_pos[2 * i], c->store_pos[2 * i + 1], cs, cx, cy);
+            if (cx >= 0 && cy >= 0 && (uint32_t)cx < gx && (uint32_t)cy < gy) continue;
+            sp.insert(sp.end(), {c->store_pos[2 * i], c->store_pos[2 * i + 1]});
+            sv.insert(sv.end(), {c->store_vel[2 * i], c->store_vel[2 * i + 1]});
+        }
+        c->store_pos.swap(sp);
+        c->store_vel.swap(sv);
+        g_err.clear();
+        return WRACH_OK;
+    }
+    // Device store: new arrays holding the frame's particles, then the old
+    // store's particles outside the grid.
+    const size_t cap = store_cap_for((size_t)n + c->dstore_n);
+    float2 *p, *v;
+    uint2 *k;
+    {
+        const int rc = alloc_store(cap, &p, &v, &k);
+        if (rc) return rc;
+    }
+    uint32_t outside = 0;
+    hipError_t e = hipSuccess;
+    if (n) {
+        e = hipMemcpyAsync(p, view.positions, (size_t)n * sizeof(float2), hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(v, view.velocities, (size_t)n * sizeof(float2), hipMemcpyDeviceToDevice,
+                               c->stream);
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(c->dvalid, 0, sizeof(uint32_t), c->stream);
+    if (e == hipSuccess) e = wrach::launch_store_outside(store_params(c, c->dstore_n), p, v, n, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&outside, c->dvalid, sizeof outside, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        (void)hipFree(v);
+        (void)hipFree(k);
+        return fail(WRACH_E_DEVICE, std::string("store write-back: ") + hipGetErrorString(e));
+    }
+    swap_store(c, p, v, k, cap);
+    c->dstore_n = (size_t)n + outside;
+    g_err.clear();
+    return WRACH_OK;
+}
+
+int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, const float *pos_xy,
+                      const float *vel_xy, uint32_t n) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (!indices || (n > 0 && (!pos_xy || !vel_xy))) return fail(WRACH_E_INVALID, "null arrays");
+    if (n_indices != c->ncell + 2)
+        return fail(WRACH_E_INVALID, "indices must hold the grid cells + 2 offsets (" +
+                                         std::to_string(c->ncell + 2) + ")");
+    if (indices[0] != 0 || indices[1] != 0) return fail(WRACH_E_INVALID, "indices must start [0, 0, ...]");
+    for (size_t i = 1; i + 1 < n_indices; ++i)
+        if (indices[i + 1] < indices[i])
+            return fail(WRACH_E_INVALID, "indices decrease at " + std::to_string(i + 1));
+    if (indices[n_indices - 1] != n)
+        return fail(WRACH_E_INVALID, "the last offset is not the particle count");
+    if (n > c->capacity)
+        return fail(WRACH_E_CAPACITY, "particles (" + std::to_string(n) + ") exceed the buffer capacity (" +
+                                          std::to_string(c->capacity) + ")");
+    for (size_t i = 0; i < 2 * (size_t)n; ++i)
+        if (std::isnan(pos_xy[i]) || std::isnan(vel_xy[i]))
+            return fail(WRACH_E_NONFINITE, "NaN in particle " + std::to_string(i / 2));
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    float2 *p = nullptr, *v = nullptr;
+    uint2 *k = nullptr;
+    size_t cap = c->dstore_cap;
+    if (!c->host_pack && n > c->dstore_cap) {
+        cap = store_cap_for(n);
+        const int rc = alloc_store(cap, &p, &v, &k);
+        if (rc) return rc;
+    }
+    WRACH_HIP(hipMemcpyAsync(c->buf.idx, indices, n_indices * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             c->stream));
+    if (n) {
+        WRACH_HIP(hipMemcpyAsync(c->buf.pos, pos_xy, (size_t)n * sizeof(float2), hipMemcpyHostToDevice,
+                                 c->stream));
+        WRACH_HIP(hipMemcpyAsync(c->buf.vel, vel_xy, (size_t)n * sizeof(float2), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    WRACH_HIP(wrach::launch_iota(c->buf.src, n, c->stream));
+    if (c->host_pack) {
+        c->store_pos.assign(pos_xy, pos_xy + 2 * (size_t)n);
+        c->store_vel.assign(vel_xy, vel_xy + 2 * (size_t)n);
+    } else {
+        if (p) swap_store(c, p, v, k, cap);
+        if (n) {
+            WRACH_HIP(hipMemcpyAsync(c->dstore_pos, c->buf.pos, (size_t)n * sizeof(float2),
+                                     hipMemcpyDeviceToDevice, c->stream));
+            WRACH_HIP(hipMemcpyAsync(c->dstore_vel, c->buf.vel, (size_t)n * sizeof(float2),
+                                     hipMemcpyDeviceToDevice, c->stream));
+        }
+        c->dstore_n = n;
+    }
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (c->fp.n != n) free_graphs(c);
+    c->fp.n = n;
+    c->settings.particles_in_frame_count = n;
+    c->canonical_valid = false;
+    g_err.clear();
     return WRACH_OK;
 }
 

@@ -119,6 +119,10 @@ struct UploadParams {
 // Cell and slot of every store particle, counts into b.cnt, inside-grid
 // total into u.valid (zeroed by the caller).
 hipError_t launch_upload_bin(const UploadParams &u, const FrameBuffers &b, hipStream_t stream);
+// The store's particles outside the grid, copied to dpos/dvel from index
+// base on, their number counted into u.valid (zeroed by the caller).
+hipError_t launch_store_outside(const UploadParams &u, float2 *dpos, float2 *dvel, uint32_t base,
+                                hipStream_t stream);
 // After launch_scan: copy each particle to idx[key + 1] + slot, marked as a
 // run of its own (src = store index | run flag).
 hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, hipStream_t stream);

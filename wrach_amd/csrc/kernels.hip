@@ -1512,23 +1512,51 @@ __device__ __forceinline__ float div_euclid_pos(float a, float b) {
     return fmodf(a, b) < 0.0f ? q - 1.0f : q;
 }
 
+// The packer's cell of a store particle (spatial_bin.rs:48-64); false
+// outside the grid.
+__device__ __forceinline__ bool store_cell(const UploadParams &u, float2 q, uint32_t &key) {
+    const int32_t cx = saturating_i32(div_euclid_pos(q.x, u.cs));
+    const int32_t cy = saturating_i32(div_euclid_pos(q.y, u.cs));
+    const bool in = cx >= 0 && cy >= 0 && (uint32_t)cx < u.gx && (uint32_t)cy < u.gy;
+    key = in ? (uint32_t)cy * u.gx + (uint32_t)cx : 0xffffffffu;
+    return in;
+}
+
 __global__ __launch_bounds__(256) void upload_bin_kernel(UploadParams u, FrameBuffers b) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool in = false;
     if (i < u.n) {
-        const float2 q = u.pos[i];
-        const int32_t cx = saturating_i32(div_euclid_pos(q.x, u.cs));
-        const int32_t cy = saturating_i32(div_euclid_pos(q.y, u.cs));
-        in = cx >= 0 && cy >= 0 && (uint32_t)cx < u.gx && (uint32_t)cy < u.gy;
         uint2 ks = make_uint2(0xffffffffu, 0u);
-        if (in) {
-            ks.x = (uint32_t)cy * u.gx + (uint32_t)cx;
-            ks.y = atomicAdd(&b.cnt[ks.x], 1u);
-        }
+        in = store_cell(u, u.pos[i], ks.x);
+        if (in) ks.y = atomicAdd(&b.cnt[ks.x], 1u);
         u.ks[i] = ks;
     }
     const unsigned long long m = __ballot(in);
     if (lane_id() == 0 && m != 0) atomicAdd(u.valid, (uint32_t)__popcll(m));
+}
+
+// The store's particles outside the grid, appended at dpos/dvel[base + ...]
+// (wrach_store_write_back), one atomic on u.valid per wavefront.  Their
+// order among themselves is free: they never enter a cell.
+__global__ __launch_bounds__(256) void store_outside_kernel(UploadParams u, float2 *dpos,
+                                                            float2 *dvel, uint32_t base) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool out = false;
+    float2 q = make_float2(0.0f, 0.0f);
+    if (i < u.n) {
+        q = u.pos[i];
+        uint32_t key;
+        out = !store_cell(u, q, key);
+    }
+    const unsigned long long m = __ballot(out);
+    uint32_t at = 0;
+    if (lane_id() == 0 && m != 0) at = atomicAdd(u.valid, (uint32_t)__popcll(m));
+    at = (uint32_t)__shfl((int)at, 0, 64);
+    if (out) {
+        const uint32_t j = base + at + (uint32_t)__popcll(m & lanemask_lt());
+        dpos[j] = q;
+        dvel[j] = u.vel[i];
+    }
 }
 
 __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, FrameBuffers b) {
@@ -1609,6 +1637,14 @@ hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
 hipError_t launch_upload_bin(const UploadParams &u, const FrameBuffers &b, hipStream_t stream) {
     if (u.n == 0) return hipSuccess;
     hipLaunchKernelGGL(upload_bin_kernel, dim3((u.n + 255) / 256), dim3(256), 0, stream, u, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_store_outside(const UploadParams &u, float2 *dpos, float2 *dvel, uint32_t base,
+                                hipStream_t stream) {
+    if (u.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(store_outside_kernel, dim3((u.n + 255) / 256), dim3(256), 0, stream, u, dpos,
+                       dvel, base);
     return hipGetLastError();
 }
 
