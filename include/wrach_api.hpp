@@ -41,6 +41,7 @@ struct WrachConfig {
     Vec2 gravity{0.0f, 0.0f};  // extension: (0, 0) is the reference's step
     float dt = 1.0f;           // extension: 1 is the reference's step
     int device = 0;
+    bool host_pack = false;    // pack uploads on the host (WRACH_FLAG_HOST_PACK)
 };
 
 struct PackedData {
@@ -85,6 +86,7 @@ class WrachAPI {
         c.gravity[1] = config.gravity.y;
         c.dt = config.dt;
         c.device = config.device;
+        c.flags = config.host_pack ? WRACH_FLAG_HOST_PACK : 0u;
         check(wrach_create(&c, &ctx_));
         state_.config = config;
         check(wrach_settings(ctx_, &state_.shader_settings));
@@ -130,6 +132,23 @@ class WrachAPI {
 
     // Not in the reference: run n frames without a readback.
     void step(uint32_t nframes) { check(wrach_step(ctx_, nframes)); }
+
+    // ParticleStore::update_from_gpu (a TODO in particle_store.rs:74-85): the
+    // store becomes the simulated particles, so add_particles continues from
+    // them.
+    void update_store_from_gpu() { check(wrach_store_write_back(ctx_)); }
+
+    // Restores a packed triple saved from get_simulation_state() after a tick
+    // (only its first indices.back() particles are read).
+    void load_packed(const PackedData &d) {
+        const uint32_t n = d.indices.empty() ? 0u : d.indices.back();
+        if (d.positions.size() < n || d.velocities.size() < n)
+            throw Error(WRACH_E_BUFFER, "wrach: packed arrays shorter than the last offset");
+        check(wrach_load_packed(ctx_, d.indices.data(), d.indices.size(),
+                                reinterpret_cast<const float *>(d.positions.data()),
+                                reinterpret_cast<const float *>(d.velocities.data()), n));
+        check(wrach_settings(ctx_, &state_.shader_settings));
+    }
     wrach_ctx *raw() { return ctx_; }
 
   private:

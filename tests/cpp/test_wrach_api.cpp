@@ -74,11 +74,48 @@ static void capacity_error() {
     CHECK(wrach.get_simulation_state().shader_settings.particles_in_frame_count == 0);
 }
 
+// Not in the reference: a checkpoint of the packed triple restored into a
+// second world continues like the first; the written-back store re-packs to
+// the simulated state (host store and device store).
+static void checkpoint_and_write_back(bool host_pack) {
+    wrach::WrachConfig cfg;
+    cfg.dimensions = {30, 20};
+    cfg.cell_size = 1;
+    cfg.gravity = {0.0f, -0.1f};
+    cfg.host_pack = host_pack;
+    std::vector<wrach::Particle> ps;
+    for (int i = 0; i < 300; ++i)
+        ps.push_back({{(float)(i % 29) + 0.3f, (float)(i / 29) * 1.7f + 0.2f},
+                      {(float)((i * 7) % 5) * 0.2f - 0.4f, (float)((i * 3) % 7) * 0.1f - 0.3f}});
+    wrach::WrachAPI a(cfg), b(cfg);
+    a.add_particles(ps);
+    for (int i = 0; i < 6; ++i) a.tick();
+    const wrach::PackedData saved = a.get_simulation_state().packed_data;
+    b.load_packed(saved);
+    for (int i = 0; i < 4; ++i) {
+        a.tick();
+        b.tick();
+    }
+    CHECK(a.get_simulation_state().packed_data.indices == b.get_simulation_state().packed_data.indices);
+    CHECK(a.positions == b.positions);
+    CHECK(a.velocities == b.velocities);
+    a.update_store_from_gpu();
+    a.add_particles({});  // re-pack of the written-back store
+    a.step(0);
+    b.step(0);
+    a.tick();
+    b.tick();
+    CHECK(a.positions == b.positions);
+    CHECK(a.velocities == b.velocities);
+}
+
 int main() {
     api_returns_data();
     prefix_sum_for_small_arrays();
     packed_data();
     capacity_error();
+    checkpoint_and_write_back(false);
+    checkpoint_and_write_back(true);
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);
     return failures ? 1 : 0;
 }
