@@ -48,8 +48,8 @@ struct FrameBuffers {
 };
 
 constexpr int kCollideBlock = 256;
-constexpr int kScanThreads = 512;
-constexpr int kScanItems = 16;            // per thread
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;             // per thread
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kScatterBlock = 256;
 constexpr int kScatterPerLane = 2;

@@ -1360,7 +1360,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
     const uint32_t epoch = *b.epoch & kEpochMask;
     const uint32_t base = tile * kScanTile + tid * kScanItems;
 
-    // Load 16 counts and clear them for the next frame.
+    // Load kScanItems counts and clear them for the next frame.
     uint32_t v[kScanItems];
     if (base + kScanItems <= p.ncell) {
         uint4 *q = reinterpret_cast<uint4 *>(b.cnt + base);
