@@ -169,6 +169,31 @@ int wrach_store_write_back(wrach_ctx *ctx);
 int wrach_load_packed(wrach_ctx *ctx, const uint32_t *indices, size_t n_indices,
                       const float *pos_xy, const float *vel_xy, uint32_t n_particles);
 
+/* Launch shapes of the frame (no counterpart in the reference, whose passes
+ * have fixed workgroup sizes).  Each field takes one of the listed values. */
+typedef struct wrach_tuning {
+    uint32_t scan_threads;     /* threads per scan tile: 1024 or 512 */
+    uint32_t scan_items;       /* counts per thread: (1024, 8), (512, 16), (1024, 4), (512, 8) */
+    uint32_t scatter_per_lane; /* particles per lane of the scatter: 2, 1 or 4 */
+    uint32_t cells_grid;       /* blocks of the queued-cell kernel, 1..4x the default */
+    float frame_us;            /* wrach_autotune: the chosen shapes' frame time; else 0 */
+} wrach_tuning;
+
+/* The current launch shapes (frame_us = 0). */
+int wrach_get_tuning(wrach_ctx *ctx, wrach_tuning *out);
+
+/* Sets the launch shapes; WRACH_E_INVALID for a value outside the lists
+ * above.  Results do not depend on the shapes. */
+int wrach_set_tuning(wrach_ctx *ctx, const wrach_tuning *tuning);
+
+/* Autotuner: runs `frames` frames (after one untimed frame) with each
+ * candidate shape of one knob at a time -- scan tile, scatter width, cells
+ * grid -- on the current state, keeps the fastest, and restores the state
+ * after every trial, so the simulation continues as if this call had not
+ * been made.  The choice reflects the current state (occupancy changes as a
+ * world evolves).  *out (may be NULL) receives it.  Synchronises. */
+int wrach_autotune(wrach_ctx *ctx, uint32_t frames, wrach_tuning *out);
+
 /* 1 (default): frames are replayed from a captured hipGraph; 0: plain
  * stream launches. */
 int wrach_set_graph(wrach_ctx *ctx, int enable);

@@ -415,6 +415,61 @@ def test_store_write_back_then_append(oracle, host):
         assert_same_state(w.read(), oracle.step(P, *ref, nsteps=5), "5 frames after the append")
 
 
+SCAN_SHAPES = [(1024, 8), (512, 16), (1024, 4), (512, 8)]
+
+
+@pytest.mark.parametrize("scan", SCAN_SHAPES, ids=lambda s: f"scan{s[0]}x{s[1]}")
+@pytest.mark.parametrize("per_lane", [2, 1, 4], ids=lambda k: f"scatter{k}")
+def test_every_launch_shape_matches_the_oracle(oracle, scan, per_lane):
+    """Every scan tile shape and scatter width the autotuner can pick, on a
+    world with one- to several-hundred-particle cells and a scan of many
+    tiles, over frames with the captured graph."""
+    say(f"[tuning] scan {scan} scatter {per_lane}")
+    r = np.random.default_rng(11)
+    cfg = WrachConfig(dimensions=(300, 200), cell_size=1, gravity=(0.0, -0.1))
+    pos = np.concatenate([r.uniform(0, 300, (9000, 2)),
+                          np.repeat(f32([[17.5, 3.5], [150.25, 40.75], [299.5, 0.5]]), [90, 300, 40], 0)
+                          + r.uniform(-0.2, 0.2, (430, 2))]).astype(np.float32)
+    vel = f32(r.uniform(-1, 1, pos.shape))
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        g0 = w.tuning()["cells_grid"]
+        w.set_tuning(scan_threads=scan[0], scan_items=scan[1], scatter_per_lane=per_lane,
+                     cells_grid=max(1, g0 // 2) if per_lane == 1 else g0)
+        t = w.tuning()
+        assert (t["scan_threads"], t["scan_items"], t["scatter_per_lane"]) == (*scan, per_lane)
+        assert_same_state(w.read(), ref, "upload (scan of the store)")
+        w.step(17)
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=17), "17 frames")
+
+
+def test_autotune_keeps_the_state(oracle):
+    """wrach_autotune times every candidate on the live state and restores
+    it after each trial: the world continues as if it had not run."""
+    wl = workloads.small_uniform(seed=6)
+    say("[tuning] autotune")
+    P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        w.step(4)
+        got = w.autotune(3)
+        assert (got["scan_threads"], got["scan_items"]) in SCAN_SHAPES
+        assert got["scatter_per_lane"] in (1, 2, 4) and got["cells_grid"] >= 1 and got["frame_us"] > 0
+        assert {k: v for k, v in w.tuning().items() if k != "frame_us"} == \
+            {k: v for k, v in got.items() if k != "frame_us"}
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=4), "after autotune")
+        w.step(6)
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=10), "6 frames after autotune")
+        with pytest.raises(WrachError) as e:
+            w.set_tuning(scan_threads=256)
+        assert e.value.code == _lib.WRACH_E_INVALID
+        with pytest.raises(WrachError):
+            w.set_tuning(scatter_per_lane=3)
+        with pytest.raises(WrachError):
+            w.set_tuning(cells_grid=0)
+
+
 def test_graph_replay_matches_plain_launches(oracle):
     """Replays of the captured frame graph, with eager work and readbacks in
     between, against plain stream launches."""

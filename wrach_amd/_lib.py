@@ -38,6 +38,9 @@ EXPORTS = (
     "wrach_canonical_device",
     "wrach_store_write_back",
     "wrach_load_packed",
+    "wrach_get_tuning",
+    "wrach_set_tuning",
+    "wrach_autotune",
     "wrach_set_graph",
     "wrach_step_profiled",
     "wrach_last_error",
@@ -65,6 +68,16 @@ class WorldSettingsC(ctypes.Structure):
         ("grid_dimensions", ctypes.c_uint32 * 2),
         ("cell_size", ctypes.c_uint32),
         ("particles_in_frame_count", ctypes.c_uint32),
+    ]
+
+
+class TuningC(ctypes.Structure):
+    _fields_ = [
+        ("scan_threads", ctypes.c_uint32),
+        ("scan_items", ctypes.c_uint32),
+        ("scatter_per_lane", ctypes.c_uint32),
+        ("cells_grid", ctypes.c_uint32),
+        ("frame_us", ctypes.c_float),
     ]
 
 
@@ -117,6 +130,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_canonical_device.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
     L.wrach_store_write_back.argtypes = [vp]
     L.wrach_load_packed.argtypes = [vp, u32p, ctypes.c_size_t, f32p, f32p, ctypes.c_uint32]
+    L.wrach_get_tuning.argtypes = [vp, ctypes.POINTER(TuningC)]
+    L.wrach_set_tuning.argtypes = [vp, ctypes.POINTER(TuningC)]
+    L.wrach_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(TuningC)]
     L.wrach_set_graph.argtypes = [vp, ctypes.c_int]
     L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.wrach_last_error.argtypes = []

@@ -131,6 +131,31 @@ class World:
     def step(self, nframes: int = 1) -> None:
         check(self._L.wrach_step(self._h, int(nframes)))
 
+    _TUNING = ("scan_threads", "scan_items", "scatter_per_lane", "cells_grid", "frame_us")
+
+    def tuning(self) -> dict:
+        """Current launch shapes (wrach_get_tuning)."""
+        t = _lib.TuningC()
+        check(self._L.wrach_get_tuning(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k in self._TUNING}
+
+    def set_tuning(self, **shapes) -> None:
+        """Sets launch shapes (wrach_set_tuning); unnamed ones stay."""
+        t = _lib.TuningC()
+        check(self._L.wrach_get_tuning(self._h, ctypes.byref(t)))
+        for k, v in shapes.items():
+            if k not in self._TUNING[:4]:
+                raise ValueError(f"unknown launch shape {k}")
+            setattr(t, k, int(v))
+        check(self._L.wrach_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, frames: int = 20) -> dict:
+        """Times the candidate launch shapes on the current state and keeps
+        the fastest (wrach_autotune); the state is unchanged."""
+        t = _lib.TuningC()
+        check(self._L.wrach_autotune(self._h, int(frames), ctypes.byref(t)))
+        return {k: getattr(t, k) for k in self._TUNING}
+
     def store_write_back(self) -> None:
         """The store becomes the simulated state (wrach_store_write_back)."""
         check(self._L.wrach_store_write_back(self._h))

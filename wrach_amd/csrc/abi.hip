@@ -7,6 +7,7 @@
 #include "frame.h"
 #include "wrach.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -93,6 +94,7 @@ struct wrach_ctx {
     uint2 *dks = nullptr;
     uint32_t *dvalid = nullptr;
     size_t dstore_n = 0, dstore_cap = 0;
+    uint32_t cells_grid0 = 1;  // the cells kernel's grid chosen at creation
 };
 
 namespace {
@@ -424,7 +426,9 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     fp.n = 0;
 
     wrach::FrameBuffers &b = c->buf;
-    b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell);
+    b.scan_shape = 0;
+    b.scatter_shape = 0;
+    b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell, b.scan_shape);
     // Queued-cell regions: a class of cells holding at least m particles has
     // at most capacity / m cells.  Each class: kSlices slices sharing that
     // bound, then an overflow list that can hold all of it.
@@ -448,6 +452,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
         if (resident == 0) return bail(fail(WRACH_E_DEVICE, "occupancy query failed"));
         b.cells_grid = want < resident ? want : resident;
         if (b.cells_grid == 0) b.cells_grid = 1;
+        c->cells_grid0 = b.cells_grid;
     }
 #define WRACH_HIP_C(expr)                                                                   \
     do {                                                                                    \
@@ -467,7 +472,9 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.kslot, cap));
     WRACH_HIP_C(dmalloc(&b.sortbuf, cap));
     WRACH_HIP_C(dmalloc(&b.cnt, ncell + 16));
-    WRACH_HIP_C(dmalloc(&b.tile_state, b.scan_tiles));
+    // status words for the smallest scan tile of kScanShapes
+    const size_t max_tiles = (ncell + wrach::kMinScanTile - 1) / wrach::kMinScanTile;
+    WRACH_HIP_C(dmalloc(&b.tile_state, max_tiles));
     WRACH_HIP_C(dmalloc(&b.epoch, 1));
     WRACH_HIP_C(dmalloc(&b.cells, ncells_total));
     WRACH_HIP_C(dmalloc(&b.class_ctr, wrach::kCtrWords));
@@ -478,7 +485,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     WRACH_HIP_C(hipMemsetAsync(b.pos_out, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.vel_out, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.cnt, 0, (ncell + 16) * sizeof(uint32_t), c->stream));
-    WRACH_HIP_C(hipMemsetAsync(b.tile_state, 0, b.scan_tiles * sizeof(unsigned long long), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.tile_state, 0, max_tiles * sizeof(unsigned long long), c->stream));
     {
         static const uint32_t first_epoch = 1;  // status words start at 0, never valid
         WRACH_HIP_C(hipMemcpyAsync(b.epoch, &first_epoch, sizeof first_epoch, hipMemcpyHostToDevice,
@@ -751,6 +758,177 @@ int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, c
     c->fp.n = n;
     c->settings.particles_in_frame_count = n;
     c->canonical_valid = false;
+    g_err.clear();
+    return WRACH_OK;
+}
+
+namespace {
+
+// Candidate grids of the cells kernel around the creation default g0.
+std::vector<uint32_t> cells_grid_candidates(uint32_t g0) {
+    std::vector<uint32_t> out;
+    for (const uint32_t g : {g0, std::max(1u, g0 * 3 / 4), std::max(1u, g0 / 2), g0 + g0 / 2})
+        if (std::find(out.begin(), out.end(), g) == out.end()) out.push_back(g);
+    return out;
+}
+
+wrach_tuning current_tuning(const wrach_ctx *c) {
+    wrach_tuning t{};
+    t.scan_threads = (uint32_t)wrach::kScanShapes[c->buf.scan_shape].threads;
+    t.scan_items = (uint32_t)wrach::kScanShapes[c->buf.scan_shape].items;
+    t.scatter_per_lane = (uint32_t)wrach::kScatterPerLane[c->buf.scatter_shape];
+    t.cells_grid = c->buf.cells_grid;
+    t.frame_us = 0.0f;
+    return t;
+}
+
+void apply_shapes(wrach_ctx *c, uint32_t scan_shape, uint32_t scatter_shape, uint32_t grid) {
+    if (c->buf.scan_shape != scan_shape || c->buf.scatter_shape != scatter_shape ||
+        c->buf.cells_grid != grid)
+        free_graphs(c);  // captured with the old shapes
+    c->buf.scan_shape = scan_shape;
+    c->buf.scatter_shape = scatter_shape;
+    c->buf.cells_grid = grid;
+    c->buf.scan_tiles = wrach::scan_tiles_for((uint32_t)c->ncell, scan_shape);
+}
+
+}  // namespace
+
+int wrach_get_tuning(wrach_ctx *c, wrach_tuning *out) {
+    if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
+    *out = current_tuning(c);
+    return WRACH_OK;
+}
+
+int wrach_set_tuning(wrach_ctx *c, const wrach_tuning *t) {
+    if (!c || !t) return fail(WRACH_E_INVALID, "null argument");
+    int scan = -1, scatter = -1;
+    for (int i = 0; i < wrach::kNumScanShapes; ++i)
+        if ((uint32_t)wrach::kScanShapes[i].threads == t->scan_threads &&
+            (uint32_t)wrach::kScanShapes[i].items == t->scan_items)
+            scan = i;
+    for (int i = 0; i < wrach::kNumScatterShapes; ++i)
+        if ((uint32_t)wrach::kScatterPerLane[i] == t->scatter_per_lane) scatter = i;
+    if (scan < 0) return fail(WRACH_E_INVALID, "no such scan tile shape");
+    if (scatter < 0) return fail(WRACH_E_INVALID, "no such scatter width");
+    if (t->cells_grid == 0 || t->cells_grid > 4 * c->cells_grid0)
+        return fail(WRACH_E_INVALID, "cells grid outside 1.." + std::to_string(4 * c->cells_grid0));
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    apply_shapes(c, (uint32_t)scan, (uint32_t)scatter, t->cells_grid);
+    return WRACH_OK;
+}
+
+int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (frames == 0) frames = 1;
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    const uint32_t n = c->fp.n;
+    if (n == 0) {  // nothing to time: an empty world does not change
+        if (out) *out = current_tuning(c);
+        return WRACH_OK;
+    }
+    // The state every trial starts from: offsets, positions, velocities and
+    // source words (counts, list counters and scan words are clean between
+    // frames).
+    const size_t nidx = c->ncell + 2;
+    void *s_idx = nullptr, *s_pos = nullptr, *s_vel = nullptr, *s_src = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(s_idx);
+        (void)hipFree(s_pos);
+        (void)hipFree(s_vel);
+        (void)hipFree(s_src);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    auto copy = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream);
+    };
+    hipError_t e = hipMalloc(&s_idx, nidx * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s_pos, (size_t)n * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&s_vel, (size_t)n * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&s_src, (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = copy(s_idx, c->buf.idx, nidx * sizeof(uint32_t));
+    if (e == hipSuccess) e = copy(s_pos, c->buf.pos, (size_t)n * sizeof(float2));
+    if (e == hipSuccess) e = copy(s_vel, c->buf.vel, (size_t)n * sizeof(float2));
+    if (e == hipSuccess) e = copy(s_src, c->buf.src, (size_t)n * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(WRACH_E_DEVICE, std::string("autotune snapshot: ") + hipGetErrorString(e));
+    }
+    auto restore = [&]() {
+        hipError_t r = copy(c->buf.idx, s_idx, nidx * sizeof(uint32_t));
+        if (r == hipSuccess) r = copy(c->buf.pos, s_pos, (size_t)n * sizeof(float2));
+        if (r == hipSuccess) r = copy(c->buf.vel, s_vel, (size_t)n * sizeof(float2));
+        if (r == hipSuccess) r = copy(c->buf.src, s_src, (size_t)n * sizeof(uint32_t));
+        if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+        return r;
+    };
+    // Microseconds per frame of the current shapes, state restored after.
+    auto trial = [&](float &us) -> int {
+        int rc = enqueue_frame(c);
+        if (rc) return rc;
+        WRACH_HIP(hipEventRecord(e0, c->stream));
+        for (uint32_t f = 0; f < frames && rc == WRACH_OK; ++f) rc = enqueue_frame(c);
+        if (rc) return rc;
+        WRACH_HIP(hipEventRecord(e1, c->stream));
+        WRACH_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        WRACH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        us = 1000.0f * ms / (float)frames;
+        WRACH_HIP(restore());
+        return WRACH_OK;
+    };
+    uint32_t scan = c->buf.scan_shape, scatter = c->buf.scatter_shape, grid = c->buf.cells_grid;
+    float best_us = 0.0f;
+    int rc = WRACH_OK;
+    // One knob at a time, the others at their current best.
+    for (int knob = 0; knob < 3 && rc == WRACH_OK; ++knob) {
+        std::vector<uint32_t> cand;
+        if (knob == 0)
+            for (int i = 0; i < wrach::kNumScanShapes; ++i) cand.push_back((uint32_t)i);
+        else if (knob == 1)
+            for (int i = 0; i < wrach::kNumScatterShapes; ++i) cand.push_back((uint32_t)i);
+        else
+            cand = cells_grid_candidates(c->cells_grid0);
+        uint32_t best = knob == 0 ? scan : knob == 1 ? scatter : grid;
+        float best_here = -1.0f;
+        for (const uint32_t v : cand) {
+            c->buf.scan_shape = knob == 0 ? v : scan;
+            c->buf.scatter_shape = knob == 1 ? v : scatter;
+            c->buf.cells_grid = knob == 2 ? v : grid;
+            c->buf.scan_tiles = wrach::scan_tiles_for((uint32_t)c->ncell, c->buf.scan_shape);
+            float us = 0.0f;
+            rc = trial(us);
+            if (rc) break;
+            if (best_here < 0.0f || us < best_here) {
+                best_here = us;
+                best = v;
+            }
+        }
+        if (knob == 0) scan = best;
+        else if (knob == 1) scatter = best;
+        else grid = best;
+        best_us = best_here;
+    }
+    if (rc) (void)restore();  // a failed trial: put the state back as far as the device allows
+    cleanup();
+    // The trials changed the shapes in place: the captured graphs are stale.
+    free_graphs(c);
+    c->buf.scan_shape = scan;
+    c->buf.scatter_shape = scatter;
+    c->buf.cells_grid = grid;
+    c->buf.scan_tiles = wrach::scan_tiles_for((uint32_t)c->ncell, scan);
+    c->canonical_valid = false;
+    if (rc) return rc;
+    if (out) {
+        *out = current_tuning(c);
+        out->frame_us = best_us;
+    }
     g_err.clear();
     return WRACH_OK;
 }

@@ -45,14 +45,24 @@ struct FrameBuffers {
     uint32_t class_cap[8];  // most cells a class can hold (its overflow list's size)
     uint32_t scan_tiles;
     uint32_t cells_grid;  // blocks of the queued-cell kernel
+    uint32_t scan_shape;     // index into kScanShapes
+    uint32_t scatter_shape;  // index into kScatterPerLane
 };
 
 constexpr int kCollideBlock = 256;
-constexpr int kScanThreads = 1024;
-constexpr int kScanItems = 8;             // per thread
-constexpr int kScanTile = kScanThreads * kScanItems;
+// Launch shapes the autotuner (wrach_autotune) chooses among; entry 0 of
+// each table is the default.  Scan tiles: threads x counts per thread
+// (1024 x 8 is the fastest in the 1M-particle frame).  Scatter: particles
+// per lane of a 256-thread block.
+struct ScanShape {
+    int threads, items;
+};
+constexpr ScanShape kScanShapes[] = {{1024, 8}, {512, 16}, {1024, 4}, {512, 8}};
+constexpr int kNumScanShapes = 4;
+constexpr uint32_t kMinScanTile = 4096;  // smallest threads x items above (sizes tile_state)
 constexpr int kScatterBlock = 256;
-constexpr int kScatterPerLane = 2;
+constexpr int kScatterPerLane[] = {2, 1, 4};
+constexpr int kNumScatterShapes = 3;
 constexpr int kMaxInCell = 9;            // cell.rs:29-30
 constexpr int kRegCell = 16;             // cells up to this size sort in registers
 constexpr int kLdsSort = 2048;           // keys per LDS bitonic sort (block path)
@@ -84,8 +94,9 @@ __host__ __device__ constexpr uint32_t ctr_index(int k, int s) {  // s == kSlice
 }
 constexpr uint32_t kCtrWords = ctr_index(kClasses, 0);
 
-inline uint32_t scan_tiles_for(uint32_t ncell) {
-    return (ncell + kScanTile - 1) / kScanTile;
+inline uint32_t scan_tiles_for(uint32_t ncell, uint32_t shape) {
+    const uint32_t tile = (uint32_t)(kScanShapes[shape].threads * kScanShapes[shape].items);
+    return (ncell + tile - 1) / tile;
 }
 
 // Launchers (kernels.hip).  Each enqueues on `stream` and returns the launch

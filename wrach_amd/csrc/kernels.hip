@@ -1289,7 +1289,7 @@ __global__ __launch_bounds__(kCollideBlock) void canonical_kernel(FrameParams p,
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of the counts: one tile of kScanTile cells per block; status
+// Exclusive scan of the counts: one tile of THREADS x ITEMS cells per block; status
 // words are tagged with the frame's epoch so they never need clearing.
 
 constexpr unsigned long long kFlagAggregate = 1ull;
@@ -1349,34 +1349,37 @@ __device__ uint32_t look_back(const FrameBuffers &b, uint32_t tile, uint32_t epo
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, FrameBuffers b) {
+// THREADS x ITEMS counts per tile (kScanShapes); ITEMS is a multiple of 4.
+template <int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void scan_kernel(FrameParams p, FrameBuffers b) {
+    static_assert(ITEMS % 4 == 0 && THREADS % 64 == 0, "128-bit count loads, whole wavefronts");
     __shared__ uint32_t s_prefix;
-    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint32_t s_wave[THREADS / 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63, wave = tid >> 6;
     // Tiles follow block order: a block waits only on lower-numbered blocks,
     // which were dispatched before it.
     const uint32_t tile = blockIdx.x;
     const uint32_t epoch = *b.epoch & kEpochMask;
-    const uint32_t base = tile * kScanTile + tid * kScanItems;
+    const uint32_t base = tile * (THREADS * ITEMS) + tid * ITEMS;
 
-    // Load kScanItems counts and clear them for the next frame.
-    uint32_t v[kScanItems];
-    if (base + kScanItems <= p.ncell) {
+    // Load ITEMS counts and clear them for the next frame.
+    uint32_t v[ITEMS];
+    if (base + ITEMS <= p.ncell) {
         uint4 *q = reinterpret_cast<uint4 *>(b.cnt + base);
 #pragma unroll
-        for (int i = 0; i < kScanItems / 4; ++i) {
+        for (int i = 0; i < ITEMS / 4; ++i) {
             const uint4 w = q[i];
             v[4 * i] = w.x; v[4 * i + 1] = w.y; v[4 * i + 2] = w.z; v[4 * i + 3] = w.w;
         }
         // Clear only what holds counts: after the first frames most of the
         // grid is empty, and these stores are a third of the kernel's bytes.
 #pragma unroll
-        for (int i = 0; i < kScanItems / 4; ++i)
+        for (int i = 0; i < ITEMS / 4; ++i)
             if (v[4 * i] | v[4 * i + 1] | v[4 * i + 2] | v[4 * i + 3]) q[i] = make_uint4(0, 0, 0, 0);
     } else {
 #pragma unroll
-        for (int i = 0; i < kScanItems; ++i) {
+        for (int i = 0; i < ITEMS; ++i) {
             if (base + i < p.ncell) {
                 v[i] = b.cnt[base + i];
                 b.cnt[base + i] = 0;
@@ -1387,13 +1390,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
     }
     uint32_t mine = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) mine += v[i];
+    for (int i = 0; i < ITEMS; ++i) mine += v[i];
     const uint32_t incl = wave_inclusive_sum(mine);
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     uint32_t wave_off = 0, agg = 0;
 #pragma unroll
-    for (int w = 0; w < kScanThreads / 64; ++w) {
+    for (int w = 0; w < THREADS / 64; ++w) {
         const uint32_t ws = s_wave[w];
         if ((uint32_t)w < wave) wave_off += ws;
         agg += ws;
@@ -1421,46 +1424,47 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(FrameParams p, Frame
 
     // idx[1 + k] = start_k; idx[0] = 0; idx[C + 1] = N.
     uint32_t run = s_prefix + wave_off + incl - mine;
-    uint32_t out[kScanItems];
+    uint32_t out[ITEMS];
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         out[i] = run;
         run += v[i];
     }
     uint32_t *dst = b.idx + 1 + base;
-    if (base + kScanItems <= p.ncell) {
+    if (base + ITEMS <= p.ncell) {
         uint4 *q = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
-        for (int i = 0; i < kScanItems / 4; ++i)
+        for (int i = 0; i < ITEMS / 4; ++i)
             q[i] = make_uint4(out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
     } else {
 #pragma unroll
-        for (int i = 0; i < kScanItems; ++i)
+        for (int i = 0; i < ITEMS; ++i)
             if (base + i < p.ncell) dst[i] = out[i];
     }
     if (tile == 0 && tid == 0) {
         b.idx[0] = 0;
     }
     if (tile == 0)  // the cell kernel of this frame has finished with the lists
-        for (uint32_t i = tid; i < (uint32_t)(kClasses * (kSlices + 1)); i += kScanThreads)
+        for (uint32_t i = tid; i < (uint32_t)(kClasses * (kSlices + 1)); i += THREADS)
             b.class_ctr[i * kCtrStride] = 0;
     if (tile == b.scan_tiles - 1 && tid == 0) b.idx[p.ncell + 1] = s_prefix + agg;
 }
 
 // ---------------------------------------------------------------------------
 
-// kScatterPerLane particles per lane, strided across the wavefront: every
+// PER particles per lane (kScatterPerLane), strided across the wavefront: every
 // load of the dependent chain (kslot -> start offset -> destination) is in
 // flight for all of them at once.
+template <int PER>
 __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, FrameBuffers b) {
-    const uint32_t w0 = (blockIdx.x * kScatterBlock + (threadIdx.x & ~63u)) * kScatterPerLane;
+    const uint32_t w0 = (blockIdx.x * kScatterBlock + (threadIdx.x & ~63u)) * PER;
     const uint32_t ln = threadIdx.x & 63;
     if (blockIdx.x == 0 && threadIdx.x == 0) *b.epoch = *b.epoch + 1;  // the frame's scan has finished
-    uint2 ks[kScatterPerLane];
-    float2 q[kScatterPerLane], v[kScatterPerLane];
-    uint32_t start[kScatterPerLane];
+    uint2 ks[PER];
+    float2 q[PER], v[PER];
+    uint32_t start[PER];
 #pragma unroll
-    for (int j = 0; j < kScatterPerLane; ++j) {
+    for (int j = 0; j < PER; ++j) {
         const uint32_t i = w0 + 64 * j + ln;
         if (i < p.n) {
             ks[j] = b.kslot[i];
@@ -1469,10 +1473,10 @@ __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, F
         }
     }
 #pragma unroll
-    for (int j = 0; j < kScatterPerLane; ++j)
+    for (int j = 0; j < PER; ++j)
         if (w0 + 64 * j + ln < p.n) start[j] = b.idx[ks[j].x + 1];
 #pragma unroll
-    for (int j = 0; j < kScatterPerLane; ++j) {
+    for (int j = 0; j < PER; ++j) {
         const uint32_t i = w0 + 64 * j + ln;
         if (i < p.n) {
             const uint32_t dest = start[j] + (ks[j].y & kSrcMask);
@@ -1616,16 +1620,42 @@ hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStre
     return hipGetLastError();
 }
 
+template <int S>
+static hipError_t launch_scan_as(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                                 hipEvent_t start, hipEvent_t stop) {
+    constexpr ScanShape sh = kScanShapes[S];
+    return launch(scan_kernel<sh.threads, sh.items>, b.scan_tiles, sh.threads, stream, start, stop,
+                  p, b);
+}
+
 hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                        hipEvent_t start, hipEvent_t stop) {
-    return launch(scan_kernel, b.scan_tiles, kScanThreads, stream, start, stop, p, b);
+    static_assert(kNumScanShapes == 4, "one case per scan shape");
+    switch (b.scan_shape) {
+        case 1: return launch_scan_as<1>(p, b, stream, start, stop);
+        case 2: return launch_scan_as<2>(p, b, stream, start, stop);
+        case 3: return launch_scan_as<3>(p, b, stream, start, stop);
+        default: return launch_scan_as<0>(p, b, stream, start, stop);
+    }
+}
+
+template <int S>
+static hipError_t launch_scatter_as(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
+                                    hipEvent_t start, hipEvent_t stop) {
+    constexpr int per = kScatterPerLane[S];
+    constexpr uint32_t per_block = kScatterBlock * per;
+    return launch(scatter_kernel<per>, (p.n + per_block - 1) / per_block, kScatterBlock, stream,
+                  start, stop, p, b);
 }
 
 hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                           hipEvent_t start, hipEvent_t stop) {
-    const uint32_t per_block = kScatterBlock * kScatterPerLane;
-    return launch(scatter_kernel, (p.n + per_block - 1) / per_block, kScatterBlock, stream, start,
-                  stop, p, b);
+    static_assert(kNumScatterShapes == 3, "one case per scatter shape");
+    switch (b.scatter_shape) {
+        case 1: return launch_scatter_as<1>(p, b, stream, start, stop);
+        case 2: return launch_scatter_as<2>(p, b, stream, start, stop);
+        default: return launch_scatter_as<0>(p, b, stream, start, stop);
+    }
 }
 
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
