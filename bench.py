@@ -115,6 +115,9 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--autotune", type=int, default=0, metavar="FRAMES",
+                    help="run wrach_autotune over FRAMES frames per candidate after the warmup "
+                         "(untimed) and report the chosen launch shapes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -136,6 +139,8 @@ def main() -> int:
     world.set_graph(not args.no_graph)
     world.step(args.warmup)
     world.sync()
+    tuning = world.autotune(args.autotune) if args.autotune > 0 else world.tuning()
+    print(f"[bench] launch shapes: {json.dumps(tuning)}", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
     world.step(args.steps)
@@ -193,6 +198,8 @@ def main() -> int:
             "gravity": list(wl.config.gravity),
             "dt": wl.config.dt,
             "graph": not args.no_graph,
+            "launch_shapes": {k: v for k, v in tuning.items() if k != "frame_us"},
+            "autotuned": args.autotune > 0,
         },
         "roofline": {
             "bound": "hbm",
