@@ -169,6 +169,34 @@ int wrach_store_write_back(wrach_ctx *ctx);
 int wrach_load_packed(wrach_ctx *ctx, const uint32_t *indices, size_t n_indices,
                       const float *pos_xy, const float *vel_xy, uint32_t n_particles);
 
+/* A batch of worlds (no counterpart in the reference, which runs one world
+ * per Bevy App, runners/api/src/lib.rs:30-45): n_worlds independent worlds
+ * of one configuration, stepped together by wrach_step -- one launch per
+ * frame kernel for all of them over a flattened (world, cell) key space,
+ * world w owning cells [w*C, (w+1)*C) of the C = grid cells per world.
+ * Each world behaves exactly as a context of its own would.  The store is
+ * binned on the device (WRACH_FLAG_HOST_PACK is refused) and the capacity is
+ * n_worlds times one world's, shared.  wrach_read, wrach_capacity and
+ * wrach_canonical_device see the whole batch; wrach_add_particles,
+ * wrach_store_write_back and wrach_load_packed are for one world only
+ * (WRACH_E_INVALID on a batch).  wrach_create is wrach_create_batch with
+ * n_worlds = 1. */
+int wrach_create_batch(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out);
+
+/* wrach_add_particles for world `world` of a batch (0 for a single world):
+ * appends to the store (tagged with the world) and re-packs the batch. */
+int wrach_add_particles_to_world(wrach_ctx *ctx, uint32_t world, const float *pos_xy,
+                                 const float *vel_xy, uint32_t n);
+
+/* wrach_read of one world of a batch: its particles and its own offsets
+ * [0, start_0, ..., N_world] over its C + 2 entries. */
+int wrach_read_world(wrach_ctx *ctx, uint32_t world, float *pos_xy, float *vel_xy,
+                     size_t particle_cap, uint32_t *indices, size_t indices_cap,
+                     uint32_t *n_particles);
+
+/* Worlds in the context (1 unless made by wrach_create_batch). */
+int wrach_worlds(wrach_ctx *ctx, uint32_t *n_worlds);
+
 /* Launch shapes of the frame (no counterpart in the reference, whose passes
  * have fixed workgroup sizes).  Each field takes one of the listed values. */
 typedef struct wrach_tuning {

@@ -470,6 +470,71 @@ def test_autotune_keeps_the_state(oracle):
             w.set_tuning(cells_grid=0)
 
 
+def _batch_worlds(cs, W, H, seed):
+    """Worlds of uneven particle counts: uniform, empty, one particle,
+    crowded cells (a few hundred particles in three cells), and a small one."""
+    r = np.random.default_rng(seed)
+    out = []
+    for n in (300, 0, 1, 0, 57):
+        out.append((f32(r.uniform(0, [W, H], (n, 2))), f32(r.uniform(-1, 1, (n, 2)))))
+    crowd = np.repeat(f32([[10.5, 5.5], [30.2, 20.7], [47.9, 0.3]]), [260, 400, 90], 0)
+    crowd += f32(r.uniform(-0.4, 0.4, crowd.shape))
+    out[3] = (f32(np.clip(crowd, 0, [W, H])), f32(r.uniform(-1, 1, crowd.shape)))
+    return out
+
+
+@pytest.mark.parametrize("cs", [1, 3])
+def test_batch_of_worlds_matches_each_world_alone(oracle, cs):
+    """wrach_create_batch: worlds of uneven particle counts stepped together
+    over one flattened key space; each world bit for bit against the
+    single-world oracle, after the upload, after frames, after an append to
+    one world."""
+    W, H = 48, 32
+    say(f"[batch] 5 worlds {W}x{H} cs={cs}")
+    cfg = WrachConfig(dimensions=(W, H), cell_size=cs, gravity=(0.0, -0.1))
+    worlds = _batch_worlds(cs, W, H, 60 + cs)
+    with World(cfg, worlds=len(worlds)) as b:
+        for w, (p, v) in enumerate(worlds):
+            b.add_particles(p, v, world=w)
+        refs = [oracle_world(oracle, cfg, p, v) for p, v in worlds]
+        for w, (P, ref) in enumerate(refs):
+            assert_same_state(b.read_world(w), ref, f"world {w} upload")
+        b.step(21)
+        for w, (P, ref) in enumerate(refs):
+            assert_same_state(b.read_world(w), oracle.step(P, *ref, nsteps=21), f"world {w}, 21 frames")
+        # an append re-packs the batch from its store (state.rs:90-101, per world)
+        extra = f32([[5.5, 5.5], [6.5, 5.5], [40.25, 30.75]])
+        b.add_particles(extra, f32(np.zeros_like(extra)), world=1)
+        worlds[1] = (np.concatenate([worlds[1][0], extra]), np.concatenate([worlds[1][1], np.zeros_like(extra)]))
+        refs = [oracle_world(oracle, cfg, p, v) for p, v in worlds]
+        b.step(5)
+        for w, (P, ref) in enumerate(refs):
+            assert_same_state(b.read_world(w), oracle.step(P, *ref, nsteps=5), f"world {w} after the append")
+        assert b.read()[0][-1] == sum(int(ref[0][-1]) for _, ref in refs)  # the batch's offsets
+
+
+def test_batch_argument_errors(oracle):
+    say("[batch] errors")
+    cfg = WrachConfig(dimensions=(16, 16), cell_size=1)
+    with World(cfg, worlds=3) as b:
+        p = f32([[1.5, 1.5]])
+        with pytest.raises(WrachError) as e:
+            b.add_particles(p, p)
+        assert e.value.code == _lib.WRACH_E_INVALID
+        with pytest.raises(WrachError) as e:
+            b.add_particles(p, p, world=3)
+        assert e.value.code == _lib.WRACH_E_INVALID
+        with pytest.raises(WrachError):
+            b.store_write_back()
+        with pytest.raises(WrachError):
+            b.load_packed(np.zeros(3 * 17 * 17 + 2, np.uint32), np.zeros((0, 2)), np.zeros((0, 2)))
+        b.add_particles(p, p, world=2)
+        assert b.read_world(2)[1].shape == (1, 2) and b.read_world(0)[1].shape == (0, 2)
+    with pytest.raises(WrachError) as e:
+        World(WrachConfig(dimensions=(16, 16), cell_size=1, host_pack=True), worlds=2)
+    assert e.value.code == _lib.WRACH_E_INVALID
+
+
 def test_graph_replay_matches_plain_launches(oracle):
     """Replays of the captured frame graph, with eager work and readbacks in
     between, against plain stream launches."""

@@ -27,6 +27,10 @@ FRAME_KERNEL_NAMES = ("collide", "cells", "scan", "scatter")
 EXPORTS = (
     "wrach_config_default",
     "wrach_create",
+    "wrach_create_batch",
+    "wrach_add_particles_to_world",
+    "wrach_read_world",
+    "wrach_worlds",
     "wrach_destroy",
     "wrach_add_particles",
     "wrach_step",
@@ -128,6 +132,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_sync.argtypes = [vp]
     L.wrach_device_state.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
     L.wrach_canonical_device.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
+    L.wrach_create_batch.argtypes = [ctypes.POINTER(WrachConfigC), ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.wrach_add_particles_to_world.argtypes = [vp, ctypes.c_uint32, f32p, f32p, ctypes.c_uint32]
+    L.wrach_read_world.argtypes = [vp, ctypes.c_uint32, f32p, f32p, ctypes.c_size_t, u32p,
+                                   ctypes.c_size_t, u32p]
+    L.wrach_worlds.argtypes = [vp, u32p]
     L.wrach_store_write_back.argtypes = [vp]
     L.wrach_load_packed.argtypes = [vp, u32p, ctypes.c_size_t, f32p, f32p, ctypes.c_uint32]
     L.wrach_get_tuning.argtypes = [vp, ctypes.POINTER(TuningC)]

@@ -90,13 +90,20 @@ class WrachState:
 class World:
     """One simulated world on one GPU (a wrach_ctx)."""
 
-    def __init__(self, config: WrachConfig):
+    def __init__(self, config: WrachConfig, worlds: int = 1):
+        """worlds > 1: a batch of equal worlds stepped together
+        (wrach_create_batch); add_particles(..., world=w) and read_world(w)
+        address one of them."""
         L = _lib.load()
         self._L = L
         self.config = config
+        self.worlds = int(worlds)
         h = ctypes.c_void_p()
         cfg = config.to_c()
-        check(L.wrach_create(ctypes.byref(cfg), ctypes.byref(h)))
+        if self.worlds == 1:
+            check(L.wrach_create(ctypes.byref(cfg), ctypes.byref(h)))
+        else:
+            check(L.wrach_create_batch(ctypes.byref(cfg), self.worlds, ctypes.byref(h)))
         self._h = h
         cap, nidx = ctypes.c_uint32(), ctypes.c_uint32()
         check(L.wrach_capacity(h, ctypes.byref(cap), ctypes.byref(nidx)))
@@ -120,13 +127,30 @@ class World:
     def __exit__(self, *exc):
         self.close()
 
-    def add_particles(self, pos, vel) -> None:
+    def add_particles(self, pos, vel, world: int | None = None) -> None:
         pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 2)
         vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 2)
         if pos.shape != vel.shape:
             raise ValueError("positions and velocities differ in length")
-        check(self._L.wrach_add_particles(self._h, pos.ctypes.data_as(_f32p),
-                                          vel.ctypes.data_as(_f32p), len(pos)))
+        if world is None:
+            check(self._L.wrach_add_particles(self._h, pos.ctypes.data_as(_f32p),
+                                              vel.ctypes.data_as(_f32p), len(pos)))
+        else:
+            check(self._L.wrach_add_particles_to_world(self._h, int(world), pos.ctypes.data_as(_f32p),
+                                                       vel.ctypes.data_as(_f32p), len(pos)))
+
+    def read_world(self, world: int):
+        """One world of a batch: (indices, positions[N,2], velocities[N,2])."""
+        cells = (self.n_indices - 2) // self.worlds  # one world's grid
+        n = self.n_particles
+        pos = np.zeros((max(n, 1), 2), np.float32)
+        vel = np.zeros((max(n, 1), 2), np.float32)
+        idx = np.zeros(cells + 2, np.uint32)
+        got = ctypes.c_uint32()
+        check(self._L.wrach_read_world(self._h, int(world), pos.ctypes.data_as(_f32p),
+                                       vel.ctypes.data_as(_f32p), pos.shape[0],
+                                       idx.ctypes.data_as(_u32p), len(idx), ctypes.byref(got)))
+        return idx, pos[:got.value], vel[:got.value]
 
     def step(self, nframes: int = 1) -> None:
         check(self._L.wrach_step(self._h, int(nframes)))

@@ -94,6 +94,9 @@ struct wrach_ctx {
     uint2 *dks = nullptr;
     uint32_t *dvalid = nullptr;
     size_t dstore_n = 0, dstore_cap = 0;
+    uint32_t *dstore_world = nullptr;  // world of each store particle (a batch)
+    uint32_t worlds = 1;               // worlds of a batch (one grid each, keys one after another)
+    uint64_t world_cells = 0;          // cells of one world
     uint32_t cells_grid0 = 1;  // the cells kernel's grid chosen at creation
 };
 
@@ -150,6 +153,7 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->dstore_pos);
     (void)hipFree(c->dstore_vel);
     (void)hipFree(c->dks);
+    (void)hipFree(c->dstore_world);
     (void)hipFree(c->dvalid);
     (void)hipFree(c->buf.cells);
     (void)hipFree(c->buf.class_ctr);
@@ -293,6 +297,20 @@ int reserve_device_store(wrach_ctx *c, size_t total) {
         return fail(WRACH_E_DEVICE, std::string("device store: ") + hipGetErrorString(e));
     }
     swap_store(c, p, v, k, cap);
+    if (c->worlds > 1) {  // the world of each store particle, kept alongside
+        uint32_t *w = nullptr;
+        e = hipMalloc(&w, cap * sizeof(uint32_t));
+        if (e == hipSuccess && c->dstore_n)
+            e = hipMemcpyAsync(w, c->dstore_world, c->dstore_n * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(w);
+            return fail(WRACH_E_DEVICE, std::string("device store: ") + hipGetErrorString(e));
+        }
+        (void)hipFree(c->dstore_world);
+        c->dstore_world = w;
+    }
     return WRACH_OK;
 }
 
@@ -308,6 +326,8 @@ wrach::UploadParams store_params(const wrach_ctx *c, size_t n) {
     u.gx = c->settings.grid_dimensions[0];
     u.gy = c->settings.grid_dimensions[1];
     u.cap = c->capacity;
+    u.world = c->worlds > 1 ? c->dstore_world : nullptr;
+    u.world_cells = (uint32_t)c->world_cells;
     return u;
 }
 
@@ -315,7 +335,8 @@ wrach::UploadParams store_params(const wrach_ctx *c, size_t n) {
 // appended to the device store, which is binned on the device (cell and slot
 // per particle, the frame's scan, a scatter).  Nothing of the state changes
 // before the capacity check has passed.
-int add_device_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint32_t n) {
+int add_device_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint32_t n,
+                      uint32_t world) {
     const size_t total = c->dstore_n + n;
     // a store index and the run flag share a 32-bit source word
     if (total > (size_t)0x7fffffffu) return fail(WRACH_E_CAPACITY, "store exceeds 2^31 - 1 particles");
@@ -328,6 +349,9 @@ int add_device_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, ui
                                  hipMemcpyHostToDevice, c->stream));
         WRACH_HIP(hipMemcpyAsync(c->dstore_vel + c->dstore_n, vel_xy, (size_t)n * sizeof(float2),
                                  hipMemcpyHostToDevice, c->stream));
+        if (c->worlds > 1)
+            WRACH_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->dstore_world + c->dstore_n),
+                                        (int)world, n, c->stream));
     }
     const wrach::UploadParams u = store_params(c, total);
     WRACH_HIP(hipMemsetAsync(c->dvalid, 0, sizeof(uint32_t), c->stream));
@@ -371,11 +395,16 @@ void wrach_config_default(wrach_config *cfg) {
     cfg->device = 0;
 }
 
-int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
+namespace {
+
+int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     if (!cfg || !out) return fail(WRACH_E_INVALID, "wrach_create: null argument");
     *out = nullptr;
     if (cfg->cell_size == 0) return fail(WRACH_E_INVALID, "cell_size must be >= 1");
     if (cfg->flags & ~WRACH_FLAG_HOST_PACK) return fail(WRACH_E_INVALID, "unknown flags");
+    if (n_worlds == 0) return fail(WRACH_E_INVALID, "a batch holds at least one world");
+    if (n_worlds > 1 && (cfg->flags & WRACH_FLAG_HOST_PACK))
+        return fail(WRACH_E_INVALID, "a batch of worlds bins its store on the device");
     if (!std::isfinite(cfg->gravity[0]) || !std::isfinite(cfg->gravity[1]) || !std::isfinite(cfg->dt))
         return fail(WRACH_E_INVALID, "gravity and dt must be finite");
     int ndev = 0;
@@ -387,18 +416,23 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     int32_t tx, ty;
     cell_coord((float)cfg->width, (float)cfg->height, cs, tx, ty);
     const uint64_t gx = (uint64_t)tx + 1, gy = (uint64_t)ty + 1;
-    const uint64_t ncell = gx * gy;
+    const uint64_t world_cells = gx * gy;
+    const uint64_t ncell = world_cells * n_worlds;  // a batch: the worlds' grids one after another
     if (ncell > kMaxCells)
         return fail(WRACH_E_TOO_MANY_CELLS, "grid has more cells than the 32-bit offsets allow");
-    const uint32_t cap = max_particles_per_frame(ncell, cfg->cell_size);
-    if (cap == 0) return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity does not fit in u32");
+    const uint32_t world_cap = max_particles_per_frame(world_cells, cfg->cell_size);
+    if (world_cap == 0) return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity does not fit in u32");
+    const uint64_t cap64 = (uint64_t)world_cap * n_worlds;
     // Bit 31 of a particle's source index marks the start of a run.
-    if (cap > 0x7fffffffu)
+    if (cap64 > 0x7fffffffu)
         return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity exceeds 2^31 - 1");
+    const uint32_t cap = (uint32_t)cap64;
 
     wrach_ctx *c = new wrach_ctx();
     c->cfg = *cfg;
     c->host_pack = (cfg->flags & WRACH_FLAG_HOST_PACK) != 0;
+    c->worlds = n_worlds;
+    c->world_cells = world_cells;
     c->ncell = ncell;
     c->capacity = cap;
     c->settings.view_dimensions[0] = (float)cfg->width;
@@ -424,6 +458,7 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     fp.gy = (uint32_t)gy;
     fp.ncell = (uint32_t)ncell;
     fp.n = 0;
+    fp.world_cells = (uint32_t)world_cells;
 
     wrach::FrameBuffers &b = c->buf;
     b.scan_shape = 0;
@@ -499,21 +534,48 @@ int wrach_create(const wrach_config *cfg, wrach_ctx **out) {
     return WRACH_OK;
 }
 
+}  // namespace
+
+int wrach_create(const wrach_config *cfg, wrach_ctx **out) { return create_ctx(cfg, 1, out); }
+
+int wrach_create_batch(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
+    return create_ctx(cfg, n_worlds, out);
+}
+
 void wrach_destroy(wrach_ctx *ctx) { release(ctx); }
 
-int wrach_add_particles(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint32_t n) {
-    if (!c) return fail(WRACH_E_INVALID, "null context");
+namespace {
+
+int add_to_world(wrach_ctx *c, uint32_t world, const float *pos_xy, const float *vel_xy, uint32_t n) {
     if (n > 0 && (!pos_xy || !vel_xy)) return fail(WRACH_E_INVALID, "null particle arrays");
     for (uint32_t i = 0; i < 2 * n; ++i)
         if (std::isnan(pos_xy[i]) || std::isnan(vel_xy[i]))
             return fail(WRACH_E_NONFINITE, "NaN in particle " + std::to_string(i / 2));
     WRACH_HIP(hipSetDevice(c->cfg.device));
     WRACH_HIP(hipStreamSynchronize(c->stream));
-    const int rc = c->host_pack ? add_host_packed(c, pos_xy, vel_xy, n) : add_device_packed(c, pos_xy, vel_xy, n);
+    const int rc = c->host_pack ? add_host_packed(c, pos_xy, vel_xy, n)
+                                : add_device_packed(c, pos_xy, vel_xy, n, world);
     if (rc) return rc;
     c->canonical_valid = false;
     g_err.clear();
     return WRACH_OK;
+}
+
+}  // namespace
+
+int wrach_add_particles(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint32_t n) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (c->worlds > 1) return fail(WRACH_E_INVALID, "a batch of worlds: use wrach_add_particles_to_world");
+    return add_to_world(c, 0, pos_xy, vel_xy, n);
+}
+
+int wrach_add_particles_to_world(wrach_ctx *c, uint32_t world, const float *pos_xy, const float *vel_xy,
+                                 uint32_t n) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (world >= c->worlds)
+        return fail(WRACH_E_INVALID, "no world " + std::to_string(world) + " in a batch of " +
+                                         std::to_string(c->worlds));
+    return add_to_world(c, world, pos_xy, vel_xy, n);
 }
 
 int wrach_step(wrach_ctx *c, uint32_t nframes) {
@@ -638,6 +700,7 @@ int wrach_canonical_device(wrach_ctx *c, wrach_device_view *out) {
 
 int wrach_store_write_back(wrach_ctx *c) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (c->worlds > 1) return fail(WRACH_E_INVALID, "store write-back is for a single world");
     wrach_device_view view{};
     {
         const int rc = wrach_canonical_device(c, &view);
@@ -705,6 +768,7 @@ int wrach_store_write_back(wrach_ctx *c) {
 int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, const float *pos_xy,
                       const float *vel_xy, uint32_t n) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (c->worlds > 1) return fail(WRACH_E_INVALID, "a checkpoint restores a single world");
     if (!indices || (n > 0 && (!pos_xy || !vel_xy))) return fail(WRACH_E_INVALID, "null arrays");
     if (n_indices != c->ncell + 2)
         return fail(WRACH_E_INVALID, "indices must hold the grid cells + 2 offsets (" +
@@ -956,6 +1020,50 @@ int wrach_read(wrach_ctx *c, float *pos_xy, float *vel_xy, size_t particle_cap,
                                  hipMemcpyDeviceToHost, c->stream));
     WRACH_HIP(hipStreamSynchronize(c->stream));
     if (n_particles) *n_particles = n;
+    return WRACH_OK;
+}
+
+int wrach_read_world(wrach_ctx *c, uint32_t world, float *pos_xy, float *vel_xy, size_t particle_cap,
+                     uint32_t *indices, size_t indices_cap, uint32_t *n_particles) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (world >= c->worlds)
+        return fail(WRACH_E_INVALID, "no world " + std::to_string(world) + " in a batch of " +
+                                         std::to_string(c->worlds));
+    const size_t wc = (size_t)c->world_cells;
+    if (indices && indices_cap < wc + 2) return fail(WRACH_E_BUFFER, "indices array too small");
+    wrach_device_view v{};
+    {
+        const int rc = wrach_canonical_device(c, &v);
+        if (rc) return rc;
+    }
+    // The world's offsets: entries 1 + world * wc .. 1 + (world + 1) * wc of
+    // the batch's, less the first (the world's first particle).
+    std::vector<uint32_t> g(wc + 1);
+    WRACH_HIP(hipMemcpyAsync(g.data(), v.indices + 1 + (size_t)world * wc, (wc + 1) * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    const uint32_t off = g[0], n = g[wc] - g[0];
+    if ((pos_xy || vel_xy) && particle_cap < n)
+        return fail(WRACH_E_BUFFER, "particle arrays hold " + std::to_string(particle_cap) +
+                                        " particles, the world has " + std::to_string(n));
+    if (indices) {
+        indices[0] = 0;
+        for (size_t k = 0; k <= wc; ++k) indices[1 + k] = g[k] - off;
+    }
+    if (pos_xy && n)
+        WRACH_HIP(hipMemcpyAsync(pos_xy, v.positions + 2 * (size_t)off, (size_t)n * 2 * sizeof(float),
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (vel_xy && n)
+        WRACH_HIP(hipMemcpyAsync(vel_xy, v.velocities + 2 * (size_t)off, (size_t)n * 2 * sizeof(float),
+                                 hipMemcpyDeviceToHost, c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (n_particles) *n_particles = n;
+    return WRACH_OK;
+}
+
+int wrach_worlds(wrach_ctx *c, uint32_t *n_worlds) {
+    if (!c || !n_worlds) return fail(WRACH_E_INVALID, "null argument");
+    *n_worlds = c->worlds;
     return WRACH_OK;
 }
 

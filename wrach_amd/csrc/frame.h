@@ -21,8 +21,11 @@ struct FrameParams {
     float gdx, gdy;
     float dt;
     uint32_t gx, gy;     // grid_dimensions
-    uint32_t ncell;      // gx * gy
-    uint32_t n;          // particles_in_frame_count
+    uint32_t ncell;      // gx * gy, times the worlds of a batch
+    uint32_t n;          // particles_in_frame_count (all worlds of a batch)
+    // A batch of equal worlds: world w owns cell keys [w * world_cells,
+    // (w + 1) * world_cells); one world: world_cells == ncell.
+    uint32_t world_cells;
 };
 
 // Device buffers of one world.  `idx` has ncell + 2 entries and is placed 3
@@ -126,6 +129,8 @@ struct UploadParams {
     float cs;           // cell side
     uint32_t gx, gy;    // grid dimensions
     uint32_t cap;       // packed-array capacity
+    const uint32_t *world;  // a batch: world of each store particle (else null)
+    uint32_t world_cells;   // cells of one world (a world's keys start at world * world_cells)
 };
 // Cell and slot of every store particle, counts into b.cnt, inside-grid
 // total into u.valid (zeroed by the caller).

@@ -114,19 +114,36 @@ __device__ __forceinline__ void cell_xy(const FrameParams &p, float x, float y, 
     ky = (uint32_t)fmaxf(fy, 0.0f);
 }
 
-__device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, float y) {
+// First cell key of the world that holds cell key c: 0 for one world; in a
+// batch of equal worlds, the world's offset in the flattened key space (a
+// particle never leaves its world: the walls keep it inside).
+__device__ __forceinline__ uint32_t world_base(const FrameParams &p, uint32_t c) {
+    if (p.world_cells == p.ncell) return 0u;
+    return c / p.world_cells * p.world_cells;
+}
+
+// Grid coordinates of cell key c inside its world, and the world's base.
+__device__ __forceinline__ void cell_coords(const FrameParams &p, uint32_t c, uint32_t &base,
+                                            uint32_t &cx, uint32_t &cy) {
+    base = world_base(p, c);
+    const uint32_t l = c - base;
+    cy = l / p.gx;
+    cx = l - cy * p.gx;
+}
+
+__device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, float y, uint32_t base) {
     uint32_t kx, ky;
     cell_xy(p, x, y, kx, ky);
-    return ky * p.gx + kx;
+    return base + ky * p.gx + kx;
 }
 
 // Writes one finished particle at canonical index `at` and claims a slot of
 // its own (a run of one) in the new cell.
 __device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b, uint32_t at,
-                                     float px, float py, float vx, float vy) {
+                                     float px, float py, float vx, float vy, uint32_t base) {
     b.pos_out[at] = make_float2(px, py);
     b.vel_out[at] = make_float2(vx, vy);
-    const uint32_t k = cell_key(p, px, py);
+    const uint32_t k = cell_key(p, px, py, base);
     const uint32_t slot = atomicAdd(&b.cnt[k], 1u);
     b.kslot[at] = make_uint2(k, slot | kRunFlag);
 }
@@ -134,10 +151,10 @@ __device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b
 // Neighbour direction of a new cell relative to the old one: 0..8 for the
 // 3x3 neighbourhood (row-major from (-1,-1)), 9 for anything farther.
 __device__ __forceinline__ uint32_t direction(const FrameParams &p, float x, float y, uint32_t cx,
-                                              uint32_t cy, uint32_t &key) {
+                                              uint32_t cy, uint32_t base, uint32_t &key) {
     uint32_t kx, ky;
     cell_xy(p, x, y, kx, ky);
-    key = ky * p.gx + kx;
+    key = base + ky * p.gx + kx;
     const int dx = (int)kx - (int)cx, dy = (int)ky - (int)cy;
     if (dx < -1 || dx > 1 || dy < -1 || dy > 1) return 9;
     return (uint32_t)((dy + 1) * 3 + (dx + 1));
@@ -240,13 +257,13 @@ __device__ __forceinline__ void sort_by_src(uint32_t (&key)[N], float (&px)[N], 
 
 template <bool STEP>
 __device__ __forceinline__ void one_particle(const FrameParams &p, const FrameBuffers &b,
-                                             uint32_t s) {
+                                             uint32_t c, uint32_t s) {
     const float2 q = b.pos[s];
     const float2 v = b.vel[s];
     if (STEP) {
         float px = q.x, py = q.y, vx = v.x, vy = v.y;
         finish_particle(p, px, py, vx, vy);
-        emit(p, b, s, px, py, vx, vy);
+        emit(p, b, s, px, py, vx, vy, world_base(p, c));
     } else {
         b.pos_out[s] = q;
         b.vel_out[s] = v;
@@ -283,7 +300,8 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
             for (int j = i + 1; j < M; ++j)
                 if (j < m) collide_pair(px[i], py[i], px[j], py[j]);
         // Finish every particle; direction of its new cell from this one.
-        const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+        uint32_t base, cx, cy;
+        cell_coords(p, c, base, cx, cy);
         uint32_t dir[NB], nkey[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -291,7 +309,7 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
             nkey[j] = 0;
             if (j < (int)n) {
                 finish_particle(p, px[j], py[j], vx[j], vy[j]);
-                dir[j] = direction(p, px[j], py[j], cx, cy, nkey[j]);
+                dir[j] = direction(p, px[j], py[j], cx, cy, base, nkey[j]);
             }
         }
         // One run per occupied direction: count, then all atomics at once.
@@ -307,7 +325,7 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
             first[d] = 0;
             if (count[d]) {
                 const uint32_t tx = cx + d % 3 - 1, ty = cy + d / 3 - 1;
-                first[d] = atomicAdd(&b.cnt[ty * p.gx + tx], count[d]);
+                first[d] = atomicAdd(&b.cnt[base + ty * p.gx + tx], count[d]);
             }
         }
 #pragma unroll
@@ -453,7 +471,8 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
         __shared__ uint32_t s_wcnt[kWavesPerBlock][9], s_run[9], s_base[9];
         const uint32_t lane = tid & 63, wave = tid >> 6;
         const unsigned long long lt = lanemask_lt();
-        const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+        uint32_t wbase, cx, cy;
+        cell_coords(p, c, wbase, cx, cy);
         uint2 *tmp = reinterpret_cast<const void *>(perm) == reinterpret_cast<const void *>(b.kslot + s)
                          ? reinterpret_cast<uint2 *>(b.sortbuf + s)
                          : b.kslot + s;
@@ -472,7 +491,7 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
                 finish_particle(p, q.x, q.y, v.x, v.y);
                 b.pos_out[s + r] = q;
                 b.vel_out[s + r] = v;
-                dir = direction(p, q.x, q.y, cx, cy, key);
+                dir = direction(p, q.x, q.y, cx, cy, wbase, key);
             }
             uint32_t rin = 0;
 #pragma unroll
@@ -501,7 +520,7 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
         // One atomic per occupied direction, then the slots.
         if (tid < 9) {
             const uint32_t tx = cx + tid % 3 - 1, ty = cy + tid / 3 - 1;
-            s_base[tid] = s_run[tid] ? atomicAdd(&b.cnt[ty * p.gx + tx], s_run[tid]) : 0u;
+            s_base[tid] = s_run[tid] ? atomicAdd(&b.cnt[wbase + ty * p.gx + tx], s_run[tid]) : 0u;
         }
         __syncthreads();
         for (uint32_t r = tid; r < n; r += kCollideBlock) {
@@ -509,7 +528,7 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
             if (t.x & kRunFlag) {
                 b.kslot[s + r] = make_uint2(t.x & kSrcMask, t.y);
             } else {
-                const uint32_t key = (cy + t.x / 3 - 1) * p.gx + (cx + t.x % 3 - 1);
+                const uint32_t key = wbase + (cy + t.x / 3 - 1) * p.gx + (cx + t.x % 3 - 1);
                 b.kslot[s + r] = make_uint2(key, (s_base[t.x] + t.y) | (t.y == 0 ? kRunFlag : 0u));
             }
         }
@@ -588,7 +607,7 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
             const float2 w = b.vel[st[j]];
             px[j] = q.x; py[j] = q.y; vx[j] = w.x; vy[j] = w.y;
             finish_particle(p, px[j], py[j], vx[j], vy[j]);
-            key[j] = cell_key(p, px[j], py[j]);
+            key[j] = cell_key(p, px[j], py[j], world_base(p, cid[j]));
         }
     }
 #pragma unroll
@@ -781,11 +800,12 @@ __device__ __forceinline__ void ranked_group(const FrameParams &p, const FrameBu
     collide_group<G>(px, py, j, n < (uint32_t)kMaxInCell ? n : (uint32_t)kMaxInCell, gbase);
 
     const bool has = j < n;
-    const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+    uint32_t wbase, cx, cy;
+    cell_coords(p, c, wbase, cx, cy);
     uint32_t dir = 10, nkey = 0;
     if (has) {
         finish_particle(p, px, py, vx, vy);
-        dir = direction(p, px, py, cx, cy, nkey);
+        dir = direction(p, px, py, cx, cy, wbase, nkey);
         b.pos_out[s + j] = make_float2(px, py);
         b.vel_out[s + j] = make_float2(vx, vy);
     }
@@ -952,7 +972,8 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
                                           const uint32_t (&rk)[kWideChunks]) {
     const uint32_t lane = lane_id();
     const unsigned long long lt = lanemask_lt();
-    const uint32_t cy = c / p.gx, cx = c - cy * p.gx;
+    uint32_t wbase, cx, cy;
+    cell_coords(p, c, wbase, cx, cy);
     uint32_t *code = L.code;
     uint32_t fkey[kWideChunks];  // a farther mover's new key, or ~0
 #pragma unroll
@@ -965,7 +986,7 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
             b.pos_out[s + rk[q]] = make_float2(px, py);
             b.vel_out[s + rk[q]] = make_float2(vx, vy);
             uint32_t key;
-            const uint32_t dir = direction(p, px, py, cx, cy, key);
+            const uint32_t dir = direction(p, px, py, cx, cy, wbase, key);
             code[rk[q]] = dir < 9 ? (kRunFlag | (dir << 27)) : 0u;
             if (dir == 9) fkey[q] = key;
         }
@@ -999,7 +1020,7 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
     uint32_t got = 0, tkey = 0;
     if (lane < 9) {
         const uint32_t tx = cx + lane % 3 - 1, ty = cy + lane / 3 - 1;
-        tkey = ty * p.gx + tx;
+        tkey = wbase + ty * p.gx + tx;
         if (mine > 0) got = atomicAdd(&b.cnt[tkey], mine);
     }
     uint32_t fslot[kWideChunks];
@@ -1268,7 +1289,7 @@ __global__ __launch_bounds__(kCollideBlock) void canonical_kernel(FrameParams p,
         uint32_t n = b.idx[c + 2] - s;
         clamp_range(p, s, n);
         if (n == 1) {
-            one_particle<false>(p, b, s);
+            one_particle<false>(p, b, c, s);
         } else if (n == 2) {
             small_cell<2, false>(p, b, c, s, n);
         } else if (n > 2 && n <= 4) {
@@ -1532,7 +1553,10 @@ __global__ __launch_bounds__(256) void upload_bin_kernel(UploadParams u, FrameBu
     if (i < u.n) {
         uint2 ks = make_uint2(0xffffffffu, 0u);
         in = store_cell(u, u.pos[i], ks.x);
-        if (in) ks.y = atomicAdd(&b.cnt[ks.x], 1u);
+        if (in) {
+            if (u.world) ks.x += u.world[i] * u.world_cells;  // its world's grid in a batch
+            ks.y = atomicAdd(&b.cnt[ks.x], 1u);
+        }
         u.ks[i] = ks;
     }
     const unsigned long long m = __ballot(in);
