@@ -513,6 +513,24 @@ def test_batch_of_worlds_matches_each_world_alone(oracle, cs):
         assert b.read()[0][-1] == sum(int(ref[0][-1]) for _, ref in refs)  # the batch's offsets
 
 
+def test_many_worlds_batch_full_size(oracle):
+    """The 512-world workload's 169 worlds of side 256 (11.2M cells, 2.65M
+    particles) as one batch: the first, a middle and the last world bit for
+    bit against the single-world oracle after 10 frames; the batch's
+    particle total against the sum of the worlds'."""
+    ws = [w for w in workloads.many_worlds() if w.config.dimensions[0] == 256]
+    say(f"[batch] many-worlds side 256: {len(ws)} worlds")
+    with World(ws[0].config, worlds=len(ws)) as b:
+        for k, w in enumerate(ws):
+            b.add_particles(w.pos, w.vel, world=k)
+        assert b.n_particles == sum(w.n for w in ws)
+        b.step(10)
+        for k in (0, len(ws) // 2, len(ws) - 1):
+            P, ref = oracle_world(oracle, ws[k].config, ws[k].pos, ws[k].vel)
+            assert_same_state(b.read_world(k), oracle.step(P, *ref, nsteps=10, nthreads=THREADS),
+                              f"world {k} of {len(ws)}")
+
+
 def test_batch_argument_errors(oracle):
     say("[batch] errors")
     cfg = WrachConfig(dimensions=(16, 16), cell_size=1)
