@@ -98,6 +98,7 @@ struct wrach_ctx {
     uint32_t worlds = 1;               // worlds of a batch (one grid each, keys one after another)
     uint64_t world_cells = 0;          // cells of one world
     uint32_t cells_grid0 = 1;  // the cells kernel's grid chosen at creation
+    uint64_t chunks = 0;       // chunks of kChunkCells cells
 };
 
 namespace {
@@ -149,6 +150,9 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.sortbuf);
     (void)hipFree(c->buf.cnt);
     (void)hipFree(c->buf.tile_state);
+    (void)hipFree(c->buf.chunk_next);
+    (void)hipFree(c->buf.chunk_live);
+    (void)hipFree(c->buf.chunk_prefix);
     (void)hipFree(c->buf.epoch);
     (void)hipFree(c->dstore_pos);
     (void)hipFree(c->dstore_vel);
@@ -229,6 +233,8 @@ int add_host_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint
                                  hipMemcpyHostToDevice, c->stream));
     }
     WRACH_HIP(wrach::launch_iota(c->buf.src, packed, c->stream));
+    // every offset came from the host: no chunk's offsets are stale
+    WRACH_HIP(hipMemsetAsync(c->buf.chunk_live, 1, c->chunks, c->stream));
     WRACH_HIP(hipStreamSynchronize(c->stream));
     if (c->fp.n != packed) free_graphs(c);
     c->fp.n = packed;
@@ -510,6 +516,10 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     // status words for the smallest scan tile of kScanShapes
     const size_t max_tiles = (ncell + wrach::kMinScanTile - 1) / wrach::kMinScanTile;
     WRACH_HIP_C(dmalloc(&b.tile_state, max_tiles));
+    c->chunks = (ncell + wrach::kChunkCells - 1) / wrach::kChunkCells;
+    WRACH_HIP_C(dmalloc(&b.chunk_next, c->chunks));
+    WRACH_HIP_C(dmalloc(&b.chunk_live, c->chunks));
+    WRACH_HIP_C(dmalloc(&b.chunk_prefix, c->chunks));
     WRACH_HIP_C(dmalloc(&b.epoch, 1));
     WRACH_HIP_C(dmalloc(&b.cells, ncells_total));
     WRACH_HIP_C(dmalloc(&b.class_ctr, wrach::kCtrWords));
@@ -521,6 +531,9 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     WRACH_HIP_C(hipMemsetAsync(b.vel_out, 0, cap * sizeof(float2), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.cnt, 0, (ncell + 16) * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.tile_state, 0, max_tiles * sizeof(unsigned long long), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.chunk_next, 0, c->chunks, c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.chunk_live, 0, c->chunks, c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.chunk_prefix, 0, c->chunks * sizeof(uint32_t), c->stream));
     {
         static const uint32_t first_epoch = 1;  // status words start at 0, never valid
         WRACH_HIP_C(hipMemcpyAsync(b.epoch, &first_epoch, sizeof first_epoch, hipMemcpyHostToDevice,
@@ -673,6 +686,8 @@ int wrach_settings(wrach_ctx *c, wrach_world_settings *out) {
 
 int wrach_device_state(wrach_ctx *c, wrach_device_view *out) {
     if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(wrach::launch_materialize(c->fp, c->buf, c->stream));  // every offset, on the stream
     out->indices = c->buf.idx;
     out->positions = reinterpret_cast<const float *>(c->buf.pos);
     out->velocities = reinterpret_cast<const float *>(c->buf.vel);
@@ -685,8 +700,10 @@ int wrach_device_state(wrach_ctx *c, wrach_device_view *out) {
 int wrach_canonical_device(wrach_ctx *c, wrach_device_view *out) {
     if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
     WRACH_HIP(hipSetDevice(c->cfg.device));
-    if (!c->canonical_valid && c->fp.n > 0) {
-        WRACH_HIP(wrach::launch_canonical(c->fp, c->buf, c->stream));
+    if (!c->canonical_valid) {
+        // the offsets of chunks the last scan found empty, then the order
+        WRACH_HIP(wrach::launch_materialize(c->fp, c->buf, c->stream));
+        if (c->fp.n > 0) WRACH_HIP(wrach::launch_canonical(c->fp, c->buf, c->stream));
         c->canonical_valid = true;
     }
     out->indices = c->buf.idx;
@@ -804,6 +821,7 @@ int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, c
                                  c->stream));
     }
     WRACH_HIP(wrach::launch_iota(c->buf.src, n, c->stream));
+    WRACH_HIP(hipMemsetAsync(c->buf.chunk_live, 1, c->chunks, c->stream));  // offsets from the host
     if (c->host_pack) {
         c->store_pos.assign(pos_xy, pos_xy + 2 * (size_t)n);
         c->store_vel.assign(vel_xy, vel_xy + 2 * (size_t)n);
@@ -897,13 +915,16 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
     // source words (counts, list counters and scan words are clean between
     // frames).
     const size_t nidx = c->ncell + 2;
-    void *s_idx = nullptr, *s_pos = nullptr, *s_vel = nullptr, *s_src = nullptr;
+    void *s_idx = nullptr, *s_pos = nullptr, *s_vel = nullptr, *s_src = nullptr, *s_live = nullptr,
+         *s_pre = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto cleanup = [&]() {
         (void)hipFree(s_idx);
         (void)hipFree(s_pos);
         (void)hipFree(s_vel);
         (void)hipFree(s_src);
+        (void)hipFree(s_live);
+        (void)hipFree(s_pre);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
     };
@@ -914,12 +935,16 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
     if (e == hipSuccess) e = hipMalloc(&s_pos, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&s_vel, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&s_src, (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s_live, c->chunks);
+    if (e == hipSuccess) e = hipMalloc(&s_pre, c->chunks * sizeof(uint32_t));
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) e = copy(s_idx, c->buf.idx, nidx * sizeof(uint32_t));
     if (e == hipSuccess) e = copy(s_pos, c->buf.pos, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = copy(s_vel, c->buf.vel, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = copy(s_src, c->buf.src, (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess) e = copy(s_live, c->buf.chunk_live, c->chunks);
+    if (e == hipSuccess) e = copy(s_pre, c->buf.chunk_prefix, c->chunks * sizeof(uint32_t));
     if (e != hipSuccess) {
         cleanup();
         return fail(WRACH_E_DEVICE, std::string("autotune snapshot: ") + hipGetErrorString(e));
@@ -929,6 +954,8 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
         if (r == hipSuccess) r = copy(c->buf.pos, s_pos, (size_t)n * sizeof(float2));
         if (r == hipSuccess) r = copy(c->buf.vel, s_vel, (size_t)n * sizeof(float2));
         if (r == hipSuccess) r = copy(c->buf.src, s_src, (size_t)n * sizeof(uint32_t));
+        if (r == hipSuccess) r = copy(c->buf.chunk_live, s_live, c->chunks);
+        if (r == hipSuccess) r = copy(c->buf.chunk_prefix, s_pre, c->chunks * sizeof(uint32_t));
         if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
         return r;
     };

@@ -40,6 +40,13 @@ struct FrameBuffers {
     unsigned long long *sortbuf;  // merge-sort ping-pong for very full cells
     uint32_t *cnt;        // per-cell counters, zero between frames
     unsigned long long *tile_state;  // scan look-back words
+    // Per chunk of kChunkCells cells: chunk_next is set by the first slot
+    // claimed in the chunk during a frame (the scan visits only those and
+    // clears it); chunk_live is what the last scan found; chunk_prefix the
+    // offset at the chunk's first cell.  idx entries of a chunk that is not
+    // live are left as they were until a readback fills them in.
+    uint8_t *chunk_next, *chunk_live;
+    uint32_t *chunk_prefix;
     uint32_t *epoch;      // frame counter tagging the scan's status words; the scatter advances it
     uint4 *cells;         // queued cells (s, n, cell, 0), one region per size class
     uint32_t *class_ctr;  // fill of every slice and overflow list (ctr_index); reset by the scan
@@ -53,6 +60,8 @@ struct FrameBuffers {
 };
 
 constexpr int kCollideBlock = 256;
+constexpr uint32_t kChunkShift = 10;  // a chunk: the cells of one collide block
+constexpr uint32_t kChunkCells = 1u << kChunkShift;
 // Launch shapes the autotuner (wrach_autotune) chooses among; entry 0 of
 // each table is the default.  Scan tiles: threads x counts per thread
 // (1024 x 8 is the fastest in the 1M-particle frame).  Scatter: particles
@@ -116,6 +125,8 @@ hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t 
 hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
+// Fills the idx entries of chunks the last scan found empty (before a readback).
+hipError_t launch_materialize(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream);
 
 // Upload binning on the device: the store (every particle added, in

@@ -137,6 +137,14 @@ __device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, floa
     return base + ky * p.gx + kx;
 }
 
+// Claims `count` consecutive slots in cell k.  The frame's first claim in a
+// chunk marks it, so that the scan visits only chunks that hold particles.
+__device__ __forceinline__ uint32_t claim(const FrameBuffers &b, uint32_t k, uint32_t count) {
+    const uint32_t old = atomicAdd(&b.cnt[k], count);
+    if (old == 0) b.chunk_next[k >> kChunkShift] = 1;
+    return old;
+}
+
 // Writes one finished particle at canonical index `at` and claims a slot of
 // its own (a run of one) in the new cell.
 __device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b, uint32_t at,
@@ -144,7 +152,7 @@ __device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b
     b.pos_out[at] = make_float2(px, py);
     b.vel_out[at] = make_float2(vx, vy);
     const uint32_t k = cell_key(p, px, py, base);
-    const uint32_t slot = atomicAdd(&b.cnt[k], 1u);
+    const uint32_t slot = claim(b, k, 1u);
     b.kslot[at] = make_uint2(k, slot | kRunFlag);
 }
 
@@ -325,11 +333,11 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
             first[d] = 0;
             if (count[d]) {
                 const uint32_t tx = cx + d % 3 - 1, ty = cy + d / 3 - 1;
-                first[d] = atomicAdd(&b.cnt[base + ty * p.gx + tx], count[d]);
+                first[d] = claim(b, base + ty * p.gx + tx, count[d]);
             }
         }
 #pragma unroll
-        for (int j = 0; j < NB; ++j) far[j] = dir[j] == 9 ? atomicAdd(&b.cnt[nkey[j]], 1u) : 0u;
+        for (int j = 0; j < NB; ++j) far[j] = dir[j] == 9 ? claim(b, nkey[j], 1u) : 0u;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             if (j < (int)n) {
@@ -506,7 +514,7 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
                 for (uint32_t w = 0; w < wave; ++w) off += s_wcnt[w][dir];
                 tmp[r] = make_uint2(dir, off + rin);
             } else if (dir == 9) {
-                const uint32_t slot = atomicAdd(&b.cnt[key], 1u);
+                const uint32_t slot = claim(b, key, 1u);
                 tmp[r] = make_uint2(key | kRunFlag, slot | kRunFlag);
             }
             __syncthreads();
@@ -520,7 +528,7 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
         // One atomic per occupied direction, then the slots.
         if (tid < 9) {
             const uint32_t tx = cx + tid % 3 - 1, ty = cy + tid / 3 - 1;
-            s_base[tid] = s_run[tid] ? atomicAdd(&b.cnt[wbase + ty * p.gx + tx], s_run[tid]) : 0u;
+            s_base[tid] = s_run[tid] ? claim(b, wbase + ty * p.gx + tx, s_run[tid]) : 0u;
         }
         __syncthreads();
         for (uint32_t r = tid; r < n; r += kCollideBlock) {
@@ -578,6 +586,13 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
     // cells (a few cache lines).  A cell's end is the next lane's start; lane
     // 63 reads it (idx has ncell + 2 entries, entry c + 1 exists for every
     // c <= ncell).
+    //
+    // A block's cells are one chunk.  A chunk the last scan found empty holds
+    // no particle and its offsets are stale: the block ends.  One world: the
+    // flag is read beside the offsets (no added latency); a batch, whose
+    // worlds are mostly empty grid: before them (no wasted loads).
+    const bool sparse = p.world_cells != p.ncell;
+    if (sparse && !b.chunk_live[blockIdx.x]) return;  // the whole block: no barrier follows
     const uint32_t w0 = (blockIdx.x * kCollideBlock + (threadIdx.x & ~63u)) * kCellsPerLane;
     const uint32_t ln = threadIdx.x & 63;
     uint32_t st[kCellsPerLane], nn[kCellsPerLane], cid[kCellsPerLane];
@@ -586,10 +601,33 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
         cid[j] = w0 + 64 * j + ln;
         st[j] = cid[j] <= p.ncell ? b.idx[cid[j] + 1] : p.n;
     }
+    if (!sparse) {
+        // A vector load (the lane term is 0 but opaque), queued behind the
+        // offsets: waiting for it waits for nothing they do not.
+        const uint32_t live = b.chunk_live[blockIdx.x + (lane_id() >> 6)];
+        if (!__builtin_amdgcn_readfirstlane(live)) return;
+    }
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
         uint32_t end = (uint32_t)__shfl_down((int)st[j], 1, 64);
-        if (ln == 63) end = cid[j] + 1 <= p.ncell ? b.idx[cid[j] + 2] : p.n;
+        if (ln == 63) {
+            // The next cell's start; past the chunk's last cell it lies in
+            // the next chunk, whose offsets are stale if that chunk is empty
+            // (its prefix is not).  The three loads go out together.
+            const uint32_t nx = cid[j] + 1;
+            if (nx >= p.ncell) {
+                end = p.n;
+            } else {
+                const uint32_t nc = nx >> kChunkShift;
+                const uint32_t at = b.idx[nx + 1];
+                if (nc != blockIdx.x) {
+                    const uint32_t lv = b.chunk_live[nc], pre = b.chunk_prefix[nc];
+                    end = lv ? at : pre;
+                } else {
+                    end = at;
+                }
+            }
+        }
         nn[j] = cid[j] < p.ncell ? end - st[j] : 0u;
         clamp_range(p, st[j], nn[j]);
     }
@@ -612,7 +650,7 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
     }
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j)
-        if (nn[j] == 1) slot[j] = atomicAdd(&b.cnt[key[j]], 1u);
+        if (nn[j] == 1) slot[j] = claim(b, key[j], 1u);
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
         if (nn[j] == 1) {
@@ -823,7 +861,7 @@ __device__ __forceinline__ void ranked_group(const FrameParams &p, const FrameBu
     const uint32_t leader = near ? (uint32_t)__builtin_ctzll(same) : lane;
     uint32_t got = 0;
     if (has && (!near || rin == 0))
-        got = atomicAdd(&b.cnt[nkey], near ? (uint32_t)__popcll(same) : 1u);
+        got = claim(b, nkey, near ? (uint32_t)__popcll(same) : 1u);
     const uint32_t base = (uint32_t)__shfl((int)got, (int)leader, 64);
     if (has) {
         const uint32_t slot = near ? (base + rin) | (rin == 0 ? kRunFlag : 0u) : (got | kRunFlag);
@@ -1021,12 +1059,12 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
     if (lane < 9) {
         const uint32_t tx = cx + lane % 3 - 1, ty = cy + lane / 3 - 1;
         tkey = wbase + ty * p.gx + tx;
-        if (mine > 0) got = atomicAdd(&b.cnt[tkey], mine);
+        if (mine > 0) got = claim(b, tkey, mine);
     }
     uint32_t fslot[kWideChunks];
 #pragma unroll
     for (int q = 0; q < kWideChunks; ++q)
-        fslot[q] = fkey[q] != 0xffffffffu ? atomicAdd(&b.cnt[fkey[q]], 1u) : 0u;
+        fslot[q] = fkey[q] != 0xffffffffu ? claim(b, fkey[q], 1u) : 0u;
     // Per direction: (new key, first slot), then the slots.
     if (lane < 9) L.dir_tbl[lane] = make_uint2(tkey, got);
     wave_sync();
@@ -1383,10 +1421,18 @@ __global__ __launch_bounds__(THREADS) void scan_kernel(FrameParams p, FrameBuffe
     const uint32_t tile = blockIdx.x;
     const uint32_t epoch = *b.epoch & kEpochMask;
     const uint32_t base = tile * (THREADS * ITEMS) + tid * ITEMS;
+    // A chunk in which no slot was claimed this frame holds only zero counts:
+    // its counts are not read and its offsets not written (the readback fills
+    // them in from chunk_prefix).
+    const uint32_t chunk = base >> kChunkShift;
+    const bool live = base < p.ncell && b.chunk_next[chunk] != 0;
 
     // Load ITEMS counts and clear them for the next frame.
     uint32_t v[ITEMS];
-    if (base + ITEMS <= p.ncell) {
+    if (!live) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) v[i] = 0;
+    } else if (base + ITEMS <= p.ncell) {
         uint4 *q = reinterpret_cast<uint4 *>(b.cnt + base);
 #pragma unroll
         for (int i = 0; i < ITEMS / 4; ++i) {
@@ -1452,7 +1498,8 @@ __global__ __launch_bounds__(THREADS) void scan_kernel(FrameParams p, FrameBuffe
         run += v[i];
     }
     uint32_t *dst = b.idx + 1 + base;
-    if (base + ITEMS <= p.ncell) {
+    if (!live) {
+    } else if (base + ITEMS <= p.ncell) {
         uint4 *q = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
         for (int i = 0; i < ITEMS / 4; ++i)
@@ -1461,6 +1508,11 @@ __global__ __launch_bounds__(THREADS) void scan_kernel(FrameParams p, FrameBuffe
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
             if (base + i < p.ncell) dst[i] = out[i];
+    }
+    if ((base & (kChunkCells - 1)) == 0 && base < p.ncell) {  // the chunk's first thread
+        b.chunk_prefix[chunk] = out[0];
+        b.chunk_live[chunk] = live ? 1 : 0;
+        b.chunk_next[chunk] = 0;  // every thread of the chunk read it before the barriers above
     }
     if (tile == 0 && tid == 0) {
         b.idx[0] = 0;
@@ -1501,10 +1553,24 @@ __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, F
         const uint32_t i = w0 + 64 * j + ln;
         if (i < p.n) {
             const uint32_t dest = start[j] + (ks[j].y & kSrcMask);
-            b.pos[dest] = q[j];
-            b.vel[dest] = v[j];
-            b.src[dest] = i | (ks[j].y & kRunFlag);
+            if (dest < p.n) {  // always, with a consistent frame
+                b.pos[dest] = q[j];
+                b.vel[dest] = v[j];
+                b.src[dest] = i | (ks[j].y & kRunFlag);
+            }
         }
+    }
+}
+
+// Offsets of the chunks the last scan found empty: every entry is the offset
+// at the chunk's first cell.
+__global__ __launch_bounds__(256) void materialize_kernel(FrameParams p, FrameBuffers b) {
+    const uint32_t chunk = blockIdx.x;
+    if (b.chunk_live[chunk]) return;
+    const uint32_t v = b.chunk_prefix[chunk];
+    for (uint32_t i = threadIdx.x; i < kChunkCells; i += 256) {
+        const uint32_t c = chunk * kChunkCells + i;
+        if (c < p.ncell) b.idx[1 + c] = v;
     }
 }
 
@@ -1555,7 +1621,7 @@ __global__ __launch_bounds__(256) void upload_bin_kernel(UploadParams u, FrameBu
         in = store_cell(u, u.pos[i], ks.x);
         if (in) {
             if (u.world) ks.x += u.world[i] * u.world_cells;  // its world's grid in a batch
-            ks.y = atomicAdd(&b.cnt[ks.x], 1u);
+            ks.y = claim(b, ks.x, 1u);
         }
         u.ks[i] = ks;
     }
@@ -1680,6 +1746,13 @@ hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream
         case 2: return launch_scatter_as<2>(p, b, stream, start, stop);
         default: return launch_scatter_as<0>(p, b, stream, start, stop);
     }
+}
+
+hipError_t launch_materialize(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
+    const uint32_t chunks = (p.ncell + kChunkCells - 1) / kChunkCells;
+    if (chunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(materialize_kernel, dim3(chunks), dim3(256), 0, stream, p, b);
+    return hipGetLastError();
 }
 
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
