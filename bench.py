@@ -130,6 +130,9 @@ def bench_many_worlds(args) -> int:
         for k, w in enumerate(group):
             b.add_particles(w.pos, w.vel, world=k)
         b.set_graph(not args.no_graph)
+        if args.autotune > 0:
+            b.step(args.warmup)
+            b.autotune(args.autotune)
         batches.append((side, group, b))
     upload_s = time.perf_counter() - t_up
     n = sum(w.n for w in ws)
@@ -157,6 +160,7 @@ def bench_many_worlds(args) -> int:
         ms = b.step_profiled(min(args.profile_steps, 50))
         per_batch[str(side)] = {"worlds": len(group), "particles": sum(w.n for w in group),
                                 "cells": len(group) * group[0].cells,
+                                "launch_shapes": {k: v for k, v in b.tuning().items() if k != "frame_us"},
                                 "kernel_avg_us": {k: 1e3 * v / min(args.profile_steps, 50)
                                                   for k, v in ms.items()}}
         b.close()

@@ -142,7 +142,10 @@ int wrach_settings(wrach_ctx *ctx, wrach_world_settings *out);
 /* Blocks until all submitted frames have finished. */
 int wrach_sync(wrach_ctx *ctx);
 
-/* Device pointers of the live state (no copy, no synchronisation). */
+/* Device pointers of the live state (no copy, no synchronisation).  The
+ * frames leave the offsets of empty 1024-cell chunks as they were; this call
+ * enqueues their fill on the context's stream, so work ordered after it on
+ * that stream sees every offset. */
 int wrach_device_state(wrach_ctx *ctx, wrach_device_view *out);
 
 /* Writes the canonical (in-cell source order) copy of the state into
@@ -201,7 +204,8 @@ int wrach_worlds(wrach_ctx *ctx, uint32_t *n_worlds);
  * have fixed workgroup sizes).  Each field takes one of the listed values. */
 typedef struct wrach_tuning {
     uint32_t scan_threads;     /* threads per scan tile: 1024 or 512 */
-    uint32_t scan_items;       /* counts per thread: (1024, 8), (512, 16), (1024, 4), (512, 8) */
+    uint32_t scan_items;       /* counts per thread: (1024, 8), (512, 16), (1024, 4), (512, 8),
+                                  (1024, 16), (1024, 32) */
     uint32_t scatter_per_lane; /* particles per lane of the scatter: 2, 1 or 4 */
     uint32_t cells_grid;       /* blocks of the queued-cell kernel, 1..4x the default */
     float frame_us;            /* wrach_autotune: the chosen shapes' frame time; else 0 */

@@ -415,7 +415,7 @@ def test_store_write_back_then_append(oracle, host):
         assert_same_state(w.read(), oracle.step(P, *ref, nsteps=5), "5 frames after the append")
 
 
-SCAN_SHAPES = [(1024, 8), (512, 16), (1024, 4), (512, 8)]
+SCAN_SHAPES = [(1024, 8), (512, 16), (1024, 4), (512, 8), (1024, 16), (1024, 32)]
 
 
 @pytest.mark.parametrize("scan", SCAN_SHAPES, ids=lambda s: f"scan{s[0]}x{s[1]}")

@@ -69,8 +69,10 @@ constexpr uint32_t kChunkCells = 1u << kChunkShift;
 struct ScanShape {
     int threads, items;
 };
-constexpr ScanShape kScanShapes[] = {{1024, 8}, {512, 16}, {1024, 4}, {512, 8}};
-constexpr int kNumScanShapes = 4;
+// The last two, 16k and 32k cells per tile, are for large grids that are
+// mostly empty (a batch of worlds), where the number of tiles sets the time.
+constexpr ScanShape kScanShapes[] = {{1024, 8}, {512, 16}, {1024, 4}, {512, 8}, {1024, 16}, {1024, 32}};
+constexpr int kNumScanShapes = 6;
 constexpr uint32_t kMinScanTile = 4096;  // smallest threads x items above (sizes tile_state)
 constexpr int kScatterBlock = 256;
 constexpr int kScatterPerLane[] = {2, 1, 4};

@@ -1720,11 +1720,13 @@ static hipError_t launch_scan_as(const FrameParams &p, const FrameBuffers &b, hi
 
 hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                        hipEvent_t start, hipEvent_t stop) {
-    static_assert(kNumScanShapes == 4, "one case per scan shape");
+    static_assert(kNumScanShapes == 6, "one case per scan shape");
     switch (b.scan_shape) {
         case 1: return launch_scan_as<1>(p, b, stream, start, stop);
         case 2: return launch_scan_as<2>(p, b, stream, start, stop);
         case 3: return launch_scan_as<3>(p, b, stream, start, stop);
+        case 4: return launch_scan_as<4>(p, b, stream, start, stop);
+        case 5: return launch_scan_as<5>(p, b, stream, start, stop);
         default: return launch_scan_as<0>(p, b, stream, start, stop);
     }
 }
