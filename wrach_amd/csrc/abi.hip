@@ -1069,6 +1069,8 @@ int wrach_read_world(wrach_ctx *c, uint32_t world, float *pos_xy, float *vel_xy,
     WRACH_HIP(hipMemcpyAsync(g.data(), v.indices + 1 + (size_t)world * wc, (wc + 1) * sizeof(uint32_t),
                              hipMemcpyDeviceToHost, c->stream));
     WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (g[0] > g[wc] || g[wc] > c->fp.n)
+        return fail(WRACH_E_STATE, "world " + std::to_string(world) + ": offsets outside the particles");
     const uint32_t off = g[0], n = g[wc] - g[0];
     if ((pos_xy || vel_xy) && particle_cap < n)
         return fail(WRACH_E_BUFFER, "particle arrays hold " + std::to_string(particle_cap) +

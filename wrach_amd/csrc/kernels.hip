@@ -577,8 +577,18 @@ __device__ __forceinline__ uint32_t size_class(uint32_t n) {
     return k < (uint32_t)kWaveClass ? k : (uint32_t)kWaveClass;
 }
 
-__global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ uint32_t s_cnt[kClasses], s_base[kClasses], s_in[kClasses], s_ovf[kClasses];
+static_assert(kCollideBlock * kCellsPerLane == (int)kChunkCells, "a collide block covers one chunk");
+
+struct CollideLds {
+    uint32_t cnt[kClasses], base[kClasses], in[kClasses], ovf[kClasses];
+};
+
+// One chunk (kCollideCells cells) by one block.  BATCH: the caller has
+// checked that the last scan found the chunk live.
+template <bool BATCH>
+__device__ __forceinline__ void collide_chunk(const FrameParams &p, const FrameBuffers &b,
+                                              uint32_t chunk, CollideLds &sl) {
+    uint32_t *s_cnt = sl.cnt, *s_base = sl.base, *s_in = sl.in, *s_ovf = sl.ovf;
     if (threadIdx.x < (uint32_t)kClasses) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     // Lane l of a wavefront takes cells w0 + l, w0 + 64 + l, ... of the
@@ -587,13 +597,10 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
     // 63 reads it (idx has ncell + 2 entries, entry c + 1 exists for every
     // c <= ncell).
     //
-    // A block's cells are one chunk.  A chunk the last scan found empty holds
-    // no particle and its offsets are stale: the block ends.  One world: the
-    // flag is read beside the offsets (no added latency); a batch, whose
-    // worlds are mostly empty grid: before them (no wasted loads).
-    const bool sparse = p.world_cells != p.ncell;
-    if (sparse && !b.chunk_live[blockIdx.x]) return;  // the whole block: no barrier follows
-    const uint32_t w0 = (blockIdx.x * kCollideBlock + (threadIdx.x & ~63u)) * kCellsPerLane;
+    // A chunk the last scan found empty holds no particle and its offsets
+    // are stale: the block skips it.  One world: the flag is read beside the
+    // offsets (no added latency); a batch has checked it before.
+    const uint32_t w0 = (chunk * kCollideBlock + (threadIdx.x & ~63u)) * kCellsPerLane;
     const uint32_t ln = threadIdx.x & 63;
     uint32_t st[kCellsPerLane], nn[kCellsPerLane], cid[kCellsPerLane];
 #pragma unroll
@@ -601,11 +608,11 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
         cid[j] = w0 + 64 * j + ln;
         st[j] = cid[j] <= p.ncell ? b.idx[cid[j] + 1] : p.n;
     }
-    if (!sparse) {
+    if (!BATCH) {
         // A vector load (the lane term is 0 but opaque), queued behind the
         // offsets: waiting for it waits for nothing they do not.
-        const uint32_t live = b.chunk_live[blockIdx.x + (lane_id() >> 6)];
-        if (!__builtin_amdgcn_readfirstlane(live)) return;
+        const uint32_t live = b.chunk_live[chunk + (lane_id() >> 6)];
+        if (!__builtin_amdgcn_readfirstlane(live)) return;  // the whole block: no barrier follows
     }
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
@@ -620,7 +627,7 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
             } else {
                 const uint32_t nc = nx >> kChunkShift;
                 const uint32_t at = b.idx[nx + 1];
-                if (nc != blockIdx.x) {
+                if (nc != chunk) {
                     const uint32_t lv = b.chunk_live[nc], pre = b.chunk_prefix[nc];
                     end = lv ? at : pre;
                 } else {
@@ -697,6 +704,36 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
                 if (o < b.class_cap[k]) b.cells[b.class_off[k] + kSlices * b.slice_cap[k] + o] = ent;
             }
         }
+    }
+}
+
+__global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
+    __shared__ CollideLds sl;
+    if (p.world_cells == p.ncell) {  // one world: a block per chunk
+        collide_chunk<false>(p, b, blockIdx.x, sl);
+        return;
+    }
+    // A batch, whose worlds are mostly empty grid: a block per 64 chunks,
+    // visiting the ones the last scan found live (one flag per lane).
+    __shared__ unsigned long long s_live;
+    const uint32_t first = blockIdx.x * 64;
+    const uint32_t chunks = (p.ncell + kChunkCells - 1) >> kChunkShift;
+    if (threadIdx.x < 64) {
+        const uint32_t c = first + threadIdx.x;
+        const unsigned long long m = __ballot(c < chunks && b.chunk_live[c] != 0);
+        if (threadIdx.x == 0) s_live = m;
+    }
+    __syncthreads();
+    // The mask in scalar registers: readfirstlane returns an int, so each
+    // half goes through uint32_t before widening (an int with bit 31 set
+    // would sign-extend into the upper half).
+    const unsigned long long live = s_live;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)live);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(live >> 32));
+    for (unsigned long long m = ((unsigned long long)hi << 32) | (unsigned long long)lo; m; m &= m - 1) {
+        const uint32_t chunk = first + (uint32_t)__builtin_ctzll(m);
+        if (chunk >= chunks) break;  // never: the ballot only sets bits of chunks below `chunks`
+        collide_chunk<true>(p, b, chunk, sl);
     }
 }
 
@@ -1683,9 +1720,9 @@ static hipError_t launch(K kernel, uint32_t grid, uint32_t block, hipStream_t st
 
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                           hipEvent_t start, hipEvent_t stop) {
-    const uint32_t per_block = kCollideBlock * kCellsPerLane;
-    return launch(collide_kernel, (p.ncell + per_block - 1) / per_block, kCollideBlock, stream,
-                  start, stop, p, b);
+    const uint32_t chunks = (p.ncell + kChunkCells - 1) / kChunkCells;
+    const uint32_t grid = p.world_cells == p.ncell ? chunks : (chunks + 63) / 64;
+    return launch(collide_kernel, grid, kCollideBlock, stream, start, stop, p, b);
 }
 
 hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
