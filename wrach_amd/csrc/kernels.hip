@@ -709,32 +709,15 @@ __device__ __forceinline__ void collide_chunk(const FrameParams &p, const FrameB
 
 __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
     __shared__ CollideLds sl;
-    if (p.world_cells == p.ncell) {  // one world: a block per chunk
+    // A block per chunk.  A batch, whose worlds are mostly empty grid, reads
+    // the chunk's flag before its offsets (no wasted loads); one world reads
+    // it beside them (no added latency).
+    if (p.world_cells == p.ncell) {
         collide_chunk<false>(p, b, blockIdx.x, sl);
         return;
     }
-    // A batch, whose worlds are mostly empty grid: a block per 64 chunks,
-    // visiting the ones the last scan found live (one flag per lane).
-    __shared__ unsigned long long s_live;
-    const uint32_t first = blockIdx.x * 64;
-    const uint32_t chunks = (p.ncell + kChunkCells - 1) >> kChunkShift;
-    if (threadIdx.x < 64) {
-        const uint32_t c = first + threadIdx.x;
-        const unsigned long long m = __ballot(c < chunks && b.chunk_live[c] != 0);
-        if (threadIdx.x == 0) s_live = m;
-    }
-    __syncthreads();
-    // The mask in scalar registers: readfirstlane returns an int, so each
-    // half goes through uint32_t before widening (an int with bit 31 set
-    // would sign-extend into the upper half).
-    const unsigned long long live = s_live;
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)live);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(live >> 32));
-    for (unsigned long long m = ((unsigned long long)hi << 32) | (unsigned long long)lo; m; m &= m - 1) {
-        const uint32_t chunk = first + (uint32_t)__builtin_ctzll(m);
-        if (chunk >= chunks) break;  // never: the ballot only sets bits of chunks below `chunks`
-        collide_chunk<true>(p, b, chunk, sl);
-    }
+    if (!b.chunk_live[blockIdx.x]) return;  // the whole block: no barrier follows
+    collide_chunk<true>(p, b, blockIdx.x, sl);
 }
 
 // Wave-private LDS of a cell wave of the wide path (below).
@@ -1721,8 +1704,7 @@ static hipError_t launch(K kernel, uint32_t grid, uint32_t block, hipStream_t st
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                           hipEvent_t start, hipEvent_t stop) {
     const uint32_t chunks = (p.ncell + kChunkCells - 1) / kChunkCells;
-    const uint32_t grid = p.world_cells == p.ncell ? chunks : (chunks + 63) / 64;
-    return launch(collide_kernel, grid, kCollideBlock, stream, start, stop, p, b);
+    return launch(collide_kernel, chunks, kCollideBlock, stream, start, stop, p, b);
 }
 
 hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
