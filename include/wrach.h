@@ -226,6 +226,35 @@ int wrach_set_tuning(wrach_ctx *ctx, const wrach_tuning *tuning);
  * world evolves).  *out (may be NULL) receives it.  Synchronises. */
 int wrach_autotune(wrach_ctx *ctx, uint32_t frames, wrach_tuning *out);
 
+/* Solvers.  WRACH_SOLVER_R (the default) is the reference's step:
+ * collisions within one cell (shaders/physics/src/cell.rs:52-75).
+ * WRACH_SOLVER_S is an extension with no counterpart in the reference: each
+ * frame first runs up to max_passes separation passes over the 3x3 cells
+ * around every particle, then the reference step.  A pass is Jacobi: every
+ * particle i moves by the sum, over the other particles j in the 3x3 cells
+ * around its frame-start cell, of the reference's pair displacement
+ * -(p_j - p_i) * (0.5 * (1 - d)) / d for d <= 1 (particles.rs:65-92; d == 0
+ * counts as 0.0001).  The terms are rounded to multiples of 2^-32 and summed
+ * exactly, so the result does not depend on the order of particles in a
+ * cell.  A pass runs only while the largest overlap 1 - d at its start
+ * exceeds overlap_tol.  A particle whose 3x3 cells hold more than
+ * WRACH_STENCIL_BUDGET particles is not displaced by the passes. */
+#define WRACH_SOLVER_R 0u
+#define WRACH_SOLVER_S 1u
+#define WRACH_MAX_PASSES 64u
+#define WRACH_STENCIL_BUDGET 4096u
+
+/* Selects the solver for the following frames.  WRACH_E_INVALID for an
+ * unknown mode, max_passes > WRACH_MAX_PASSES, a negative, infinite or NaN
+ * tolerance, or WRACH_SOLVER_S on a batch of worlds.  max_passes = 0 makes
+ * WRACH_SOLVER_S the reference's step. */
+int wrach_set_solver(wrach_ctx *ctx, uint32_t mode, uint32_t max_passes, float overlap_tol);
+
+/* The last WRACH_SOLVER_S frame's largest overlap before and after its
+ * passes, and the number of passes that ran (zeros before the first such
+ * frame).  Synchronises the stream. */
+int wrach_solver_stats(wrach_ctx *ctx, float *overlap_before, float *overlap_after, uint32_t *passes);
+
 /* 1 (default): frames are replayed from a captured hipGraph; 0: plain
  * stream launches. */
 int wrach_set_graph(wrach_ctx *ctx, int enable);

@@ -376,3 +376,114 @@ int wo_step_mt(const wo_params *p, uint32_t *idx, float *pos, float *vel,
     free(sh.key); free(th); free(args);
     return sh.bad ? -1 : 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Mode S: separation passes over the 3x3 cells around each particle.     */
+/* An extension with no counterpart in the reference, whose collisions     */
+/* are same-cell only (shaders/physics/src/cell.rs:60-69).                 */
+
+/* Cell (cx, cy) of a particle from its frame-start position, as the
+ * re-bin keys it (wo_cell_key); -1 outside the grid. */
+static int64_t stencil_cell(const wo_params *p, const float *pcell, uint64_t i)
+{
+    return wo_cell_key(p, pcell[2 * i], pcell[2 * i + 1]);
+}
+
+/* Particles in the 3x3 cells around cell k (clipped at the grid). */
+static uint64_t stencil_count(const wo_params *p, const uint32_t *idx, int64_t k)
+{
+    int64_t gx = p->grid_dimensions[0], gy = p->grid_dimensions[1];
+    int64_t cx = k % gx, cy = k / gx;
+    uint64_t total = 0;
+    for (int64_t y = cy - 1; y <= cy + 1; ++y) {
+        if (y < 0 || y >= gy) continue;
+        for (int64_t x = cx - 1; x <= cx + 1; ++x) {
+            if (x < 0 || x >= gx) continue;
+            int64_t m = y * gx + x;
+            total += idx[m + 2] - idx[m + 1];
+        }
+    }
+    return total;
+}
+
+/* One pair of the pass: particles.rs:65-92's distance, cutoff, 0 -> 0.0001
+ * and force, for the displacement of i (returned through ex, ey: the
+ * reference subtracts (p_j - p_i) * force from p_i) and the overlap 1 - d. */
+static int stencil_pair(float xi, float yi, float xj, float yj, float *ex, float *ey,
+                        float *overlap)
+{
+    float dx = xi - xj;
+    float dy = yi - yj;
+    float sq = dx * dx;
+    sq = sq + dy * dy;
+    float d = sqrtf(sq);
+    if (d > 1.0f) return 0;
+    if (d == 0.0f) d = 0.0001f;
+    *overlap = 1.0f - d;
+    float force = (0.5f * (1.0f - d)) / d;
+    *ex = (xj - xi) * force;
+    *ey = (yj - yi) * force;
+    return 1;
+}
+
+float wo_stencil_pass(const wo_params *p, const uint32_t *idx, const float *pcell,
+                      const float *pin, float *pout, int move)
+{
+    uint64_t n = p->particles_in_frame_count;
+    int64_t gx = p->grid_dimensions[0], gy = p->grid_dimensions[1];
+    float worst = 0.0f;
+    for (uint64_t i = 0; i < n; ++i) {
+        float xi = pin[2 * i], yi = pin[2 * i + 1];
+        int64_t k = stencil_cell(p, pcell, i);
+        int64_t ax = 0, ay = 0;  /* the displacement in WO_STENCIL_SCALE units */
+        if (k >= 0 && stencil_count(p, idx, k) <= WO_STENCIL_BUDGET) {
+            int64_t cx = k % gx, cy = k / gx;
+            for (int64_t y = cy - 1; y <= cy + 1; ++y) {
+                if (y < 0 || y >= gy) continue;
+                for (int64_t x = cx - 1; x <= cx + 1; ++x) {
+                    if (x < 0 || x >= gx) continue;
+                    int64_t m = y * gx + x;
+                    for (uint64_t j = idx[m + 1]; j < idx[m + 2]; ++j) {
+                        if (j == i) continue;
+                        float ex, ey, ov;
+                        if (!stencil_pair(xi, yi, pin[2 * j], pin[2 * j + 1], &ex, &ey, &ov))
+                            continue;
+                        if (ov > worst) worst = ov;
+                        ax -= llrintf(ex * WO_STENCIL_SCALE);
+                        ay -= llrintf(ey * WO_STENCIL_SCALE);
+                    }
+                }
+            }
+        }
+        if (move) {
+            pout[2 * i] = xi + (float)ax * (1.0f / WO_STENCIL_SCALE);
+            pout[2 * i + 1] = yi + (float)ay * (1.0f / WO_STENCIL_SCALE);
+        }
+    }
+    return worst;
+}
+
+int wo_step_s(const wo_params *p, uint32_t *idx, float *pos, float *vel, float *scratch,
+              uint32_t max_passes, float overlap_tol, float overlap[2], uint32_t *passes)
+{
+    uint64_t n = p->particles_in_frame_count;
+    float *a = (float *)malloc(sizeof(float) * 2 * (n ? n : 1));
+    float *b = (float *)malloc(sizeof(float) * 2 * (n ? n : 1));
+    memcpy(a, pos, sizeof(float) * 2 * n);
+    uint32_t ran = 0;
+    float ov = wo_stencil_pass(p, idx, pos, a, NULL, 0);
+    overlap[0] = ov;
+    for (uint32_t k = 0; k < max_passes; ++k) {
+        if (!(ov > overlap_tol)) break;     /* the pass would start within tolerance */
+        wo_stencil_pass(p, idx, pos, a, b, 1);
+        float *t = a; a = b; b = t;
+        ov = wo_stencil_pass(p, idx, pos, a, NULL, 0);
+        ran = k + 1;
+    }
+    overlap[1] = ov;
+    *passes = ran;
+    memcpy(pos, a, sizeof(float) * 2 * n);
+    free(a);
+    free(b);
+    return wo_step(p, idx, pos, vel, scratch);
+}

@@ -32,6 +32,13 @@
  *   wo_rebin             brute force (Python stable sort by cell), invariants
  *                        (permutation, sortedness, counts)
  *   wo_step_mt           equality with wo_step (serial) on every tested case
+ *   wo_stencil_pass      brute force (numpy all-pairs Jacobi, float64 sums) on
+ *                        random worlds; closed form for one overlapping pair
+ *                        (separation 1 after a pass); invariance under a
+ *                        permutation of the particles inside their cells;
+ *                        overlap equal to the all-pairs maximum
+ *   wo_step_s            zero passes equal wo_step; a tolerance above the
+ *                        first overlap runs no pass
  */
 #ifndef WRACH_ORACLE_H
 #define WRACH_ORACLE_H
@@ -129,6 +136,37 @@ int wo_step(const wo_params *p, uint32_t *indices, float *pos, float *vel,
  * Output identical to wo_step.  Scratch: 4*N floats. */
 int wo_step_mt(const wo_params *p, uint32_t *indices, float *pos, float *vel,
                float *scratch, int nthreads);
+
+/* Mode S (an extension; the reference collides within one cell only,
+ * shaders/physics/src/cell.rs:60-69): before the reference step, separation
+ * passes over the 3x3 cells around each particle.
+ *
+ * A pass is Jacobi: every particle i takes, from every other particle j in
+ * the 3x3 cells around its frame-start cell (wo_cell_key of its position
+ * when the frame began), the pair rule of wo_collide_cell
+ * (particles.rs:65-92): skip d > 1, d == 0 -> 0.0001, force
+ * (0.5 * (1 - d)) / d, p_i -= (p_j - p_i) * force.  The terms are rounded to
+ * multiples of 1/WO_STENCIL_SCALE (llrintf, nearest-even) and summed in
+ * int64, exactly, so the displacement does not depend on the order of the
+ * particles in a cell; p_i' = p_i + (float)sum / WO_STENCIL_SCALE.  A
+ * particle whose 3x3 cells hold more than WO_STENCIL_BUDGET particles
+ * (itself included) takes no displacement and measures no overlap: a bound
+ * on the work per particle.  With cell sides >= 1 every pair closer than 1
+ * lies in the 3x3 cells, so one pass equals the all-pairs Jacobi pass. */
+#define WO_STENCIL_SCALE 4294967296.0f
+#define WO_STENCIL_BUDGET 4096
+
+/* One pass over the packed state: positions pin, frame-start positions
+ * pcell (the cells), indices idx of the frame.  move: write pout.  Returns
+ * the largest overlap 1 - d over the pairs it visits (0 if none). */
+float wo_stencil_pass(const wo_params *p, const uint32_t *indices, const float *pcell,
+                      const float *pin, float *pout, int move);
+
+/* A mode S frame: up to max_passes passes, each run only while the overlap
+ * at its start exceeds overlap_tol, then wo_step.  overlap[0]: before the
+ * passes, overlap[1]: after them; *passes: the passes that ran. */
+int wo_step_s(const wo_params *p, uint32_t *indices, float *pos, float *vel, float *scratch,
+              uint32_t max_passes, float overlap_tol, float overlap[2], uint32_t *passes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,154 @@
+"""Mode S (separation passes over the 3x3 cells around each particle, then
+the reference step) on the HIP path vs the oracle's wo_step_s, through the
+C ABI (needs a GPU).
+
+Mode S is an extension with no counterpart in the reference.  Its passes sum
+each particle's pair terms in 2^-32 fixed point, so the device's in-cell
+order does not matter and the bar is the same as mode R's: indices exact,
+positions and velocities bit for bit, and the overlap statistics equal.
+"""
+import numpy as np
+import pytest
+
+from wrach_amd import World, WrachConfig, WrachError, workloads
+from wrach_amd import _lib
+from tests.test_gpu_parity import assert_same_state, f32, oracle_world, say
+
+pytestmark = pytest.mark.gpu
+
+
+def run_stencil(oracle, cfg, pos, vel, frames, passes, tol=0.0, every=2, label="", graph=True):
+    say(f"[stencil] {label}: N={len(pos)} dims={cfg.dimensions} cs={cfg.cell_size} passes={passes} "
+        f"tol={tol} frames={frames} graph={graph}")
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        w.set_graph(graph)
+        w.set_solver("S", passes, tol)
+        done = 0
+        while done < frames:
+            k = min(every, frames - done)
+            w.step(k)
+            *ref, st = oracle.step_s(P, *ref, max_passes=passes, overlap_tol=tol, nsteps=k)
+            done += k
+            assert_same_state(w.read(), tuple(ref), f"{label} frame {done}")
+            got = w.solver_stats()
+            assert got == st, f"{label} frame {done}: stats {got} against {st}"
+        return st
+
+
+def uniform(n, w, h, seed):
+    rng = np.random.default_rng(seed)
+    pos = f32(np.stack([rng.uniform(0, w, n), rng.uniform(0, h / 2, n)], 1))
+    vel = f32(rng.uniform(-1, 1, (n, 2)))
+    return pos, vel
+
+
+@pytest.mark.parametrize("passes", [1, 2, 3])
+def test_stencil_uniform_16k(oracle, passes):
+    pos, vel = uniform(16384, 512, 512, 11)
+    cfg = WrachConfig(dimensions=(512, 512), cell_size=1, gravity=(0.0, -0.1))
+    st = run_stencil(oracle, cfg, pos, vel, 8, passes, label="uniform-16k")
+    assert st[2] == passes and st[1] < st[0]
+
+
+def test_stencil_cell_size_3(oracle):
+    pos, vel = uniform(6000, 240, 180, 12)
+    cfg = WrachConfig(dimensions=(240, 180), cell_size=3)
+    run_stencil(oracle, cfg, pos, vel, 6, 2, label="cs3")
+
+
+def test_stencil_dense_clusters(oracle):
+    """Clusters of 10-60 particles per cell (mode R's group and wide classes
+    beside the passes), and every lane visiting hundreds of neighbours."""
+    rng = np.random.default_rng(13)
+    centres = rng.uniform(5, 59, (40, 2))
+    pos = np.concatenate([c + rng.normal(0, 0.6, (rng.integers(10, 60), 2)) for c in centres])
+    pos = f32(np.clip(pos, 0, 63.99))
+    vel = f32(rng.normal(0, 0.3, pos.shape))
+    cfg = WrachConfig(dimensions=(64, 64), cell_size=1, gravity=(0.0, -0.05))
+    run_stencil(oracle, cfg, pos, vel, 6, 2, label="clusters")
+
+
+def test_stencil_tolerance_stops_the_passes(oracle):
+    """A tolerance between the first and the best overlap: the passes stop
+    early, after the count the oracle stops at."""
+    pos, vel = uniform(4000, 128, 128, 14)
+    cfg = WrachConfig(dimensions=(128, 128), cell_size=1)
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    *_, st8 = oracle.step_s(P, *ref, max_passes=8)
+    tol = (st8[0] + st8[1]) / 2
+    st = run_stencil(oracle, cfg, pos, vel, 4, 8, tol=tol, every=1, label="tolerance")
+    assert 0 < st[2] < 8 or st[0] <= tol
+
+
+def test_stencil_budget_guard(oracle):
+    """A cell of 4,200 particles: its lanes and their neighbours' exceed the
+    4,096-particle budget and take no displacement; the rest of the world
+    is separated as usual."""
+    rng = np.random.default_rng(15)
+    crowd = f32(np.full((4200, 2), 20.5) + rng.uniform(-0.45, 0.45, (4200, 2)))
+    rest, _ = uniform(800, 128, 128, 16)
+    pos = np.concatenate([crowd, rest])
+    vel = f32(rng.uniform(-0.2, 0.2, pos.shape))
+    cfg = WrachConfig(dimensions=(128, 128), cell_size=1)  # capacity 18,305 (cells x 1.1)
+    run_stencil(oracle, cfg, pos, vel, 3, 1, every=1, label="budget")
+
+
+def test_stencil_plain_launches_match_graphs(oracle):
+    pos, vel = uniform(8000, 256, 256, 17)
+    cfg = WrachConfig(dimensions=(256, 256), cell_size=1, gravity=(0.0, -0.1))
+    run_stencil(oracle, cfg, pos, vel, 20, 3, every=10, label="plain launches", graph=False)
+    run_stencil(oracle, cfg, pos, vel, 20, 3, every=10, label="graph of 16 frames", graph=True)
+
+
+def test_stencil_zero_passes_and_back_to_mode_r(oracle):
+    """Mode S with no passes is the reference's step; switching back to R
+    mid-run continues the reference's step."""
+    pos, vel = uniform(8000, 256, 256, 18)
+    cfg = WrachConfig(dimensions=(256, 256), cell_size=1, gravity=(0.0, -0.1))
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        w.set_solver("S", 0)
+        w.step(5)
+        ref = oracle.step(P, *ref, nsteps=5)
+        assert_same_state(w.read(), ref, "zero passes")
+        w.set_solver("S", 2)
+        w.step(3)
+        *ref, _ = oracle.step_s(P, *ref, max_passes=2, nsteps=3)
+        assert_same_state(w.read(), tuple(ref), "two passes")
+        w.set_solver("R")
+        w.step(4)
+        ref = oracle.step(P, *ref, nsteps=4)
+        assert_same_state(w.read(), ref, "back to R")
+
+
+def test_stencil_argument_errors(oracle):
+    say("[stencil] errors")
+    cfg = WrachConfig(dimensions=(32, 32), cell_size=1)
+    with World(cfg) as w:
+        for args in [("S", _lib.WRACH_MAX_PASSES + 1, 0.0), ("S", 1, float("nan")), ("S", 1, -1.0),
+                     ("S", 1, float("inf"))]:
+            with pytest.raises(WrachError) as e:
+                w.set_solver(*args)
+            assert e.value.code == _lib.WRACH_E_INVALID, args
+        with pytest.raises(WrachError) as e:
+            _lib.check(w._L.wrach_set_solver(w._h, 7, 1, 0.0))
+        assert e.value.code == _lib.WRACH_E_INVALID
+        w.add_particles(f32([[3.5, 3.5], [4.1, 3.5]]), f32([[0, 0], [0, 0]]))
+        w.set_solver("S", 1)
+        w.step(1)
+        assert w.solver_stats()[2] == 1
+    with World(cfg, worlds=2) as b:
+        with pytest.raises(WrachError) as e:
+            b.set_solver("S", 1)
+        assert e.value.code == _lib.WRACH_E_INVALID
+        b.set_solver("R")
+
+
+def test_stencil_lattice_1m_full_size(oracle):
+    """The 1,000,000-particle lattice, 2048x1024, with two passes per frame:
+    exact against the oracle for 3 frames."""
+    wl = workloads.lattice_1m()
+    run_stencil(oracle, wl.config, wl.pos, wl.vel, 3, 2, every=3, label=wl.name)

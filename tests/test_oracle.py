@@ -300,3 +300,113 @@ def test_regression_fixture(oracle):
         assert np.array_equal(idx, z["idx"][k])
         assert np.array_equal(bits(pos), bits(z["pos"][k]))
         assert np.array_equal(bits(vel), bits(z["vel"][k]))
+
+
+# -------------------------------------------------- mode S (an extension)
+
+def _packed_world(oracle, n, side, cs, seed, spread=1.0):
+    rng = np.random.default_rng(seed)
+    pos = f32(rng.uniform(0, side * spread, (n, 2)))
+    vel = f32(rng.uniform(-1, 1, (n, 2)))
+    idx, p, v = oracle.pack(pos, vel, (0, 0, side, side), float(cs))
+    P = oracle.params(side, side, cs, len(p))
+    return P, idx, p, v
+
+
+def _brute_force_pass(pin):
+    """All-pairs Jacobi pass in numpy: the pair rule in float32 operation by
+    operation, the sum of a particle's terms in float64."""
+    x, y = pin[:, 0], pin[:, 1]
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    sq = (dx * dx).astype(F) + (dy * dy).astype(F)
+    d = np.sqrt(sq.astype(F)).astype(F)
+    near = d <= F(1)
+    np.fill_diagonal(near, False)
+    d = np.where(d == 0, F(0.0001), d).astype(F)
+    force = ((F(0.5) * (F(1) - d)).astype(F) / d).astype(F)
+    ex = ((x[None, :] - x[:, None]).astype(F) * force).astype(F)
+    ey = ((y[None, :] - y[:, None]).astype(F) * force).astype(F)
+    disp = np.stack([-(np.where(near, ex, 0).astype(np.float64)).sum(1),
+                     -(np.where(near, ey, 0).astype(np.float64)).sum(1)], 1)
+    ov = np.where(near, F(1) - d, 0).max() if near.any() else 0.0
+    return pin.astype(np.float64) + disp, float(ov)
+
+
+def test_stencil_pair_separates_to_one(oracle):
+    """Two particles 0.6 apart in neighbouring cells: each moves 0.2 away
+    from the other, so the pair ends exactly 1 apart, midpoint kept."""
+    idx, p, _ = oracle.pack(f32([[2.5, 2.5], [3.1, 2.5]]), f32([[0, 0], [0, 0]]), (0, 0, 8, 8), 1.0)
+    P = oracle.params(8, 8, 1, 2)
+    out, ov = oracle.stencil_pass(P, idx, p, p)
+    assert out[1, 0] - out[0, 0] == F(1.0)
+    assert abs((out[0, 0] + out[1, 0]) / 2 - 2.8) < 1e-6 and np.all(out[:, 1] == F(2.5))
+    dx = F(2.5) - F(3.1)
+    assert ov == float(F(1) - np.sqrt(dx * dx + F(0) * F(0)))
+
+
+@pytest.mark.parametrize("cs", [1, 3])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_stencil_pass_equals_all_pairs_brute_force(oracle, cs, seed):
+    """With cell sides >= 1 every pair closer than 1 shares the 3x3 cells,
+    so the pass must equal the all-pairs Jacobi pass: positions to the
+    fixed-point rounding (2^-33 per term), the overlap exactly."""
+    P, idx, p, _ = _packed_world(oracle, 400, 24, cs, seed)
+    out, ov = oracle.stencil_pass(P, idx, p, p)
+    ref, ref_ov = _brute_force_pass(p)
+    assert np.abs(out - ref).max() < 2e-6
+    assert ov == ref_ov
+    assert np.abs(out - p).max() > 0.01  # the world has overlaps to resolve
+
+
+def test_stencil_pass_ignores_order_inside_cells(oracle):
+    """The fixed-point sum makes the pass independent of the order of the
+    particles within their cells: a permuted packed layout gives every
+    particle bit-identical output."""
+    P, idx, p, _ = _packed_world(oracle, 600, 16, 1, 3)
+    rng = np.random.default_rng(7)
+    perm = np.arange(len(p))
+    for c in range(len(idx) - 2):
+        s, e = int(idx[c + 1]), int(idx[c + 2])
+        perm[s:e] = s + rng.permutation(e - s)
+    q = p[perm]
+    out, ov = oracle.stencil_pass(P, idx, p, p)
+    out2, ov2 = oracle.stencil_pass(P, idx, q, q)
+    assert np.array_equal(bits(out2), bits(out[perm])) and ov == ov2
+
+
+def test_stencil_budget_bounds_the_work(oracle):
+    """A particle whose 3x3 cells hold more than the budget (4096) takes no
+    displacement; a sparse particle beside the crowd still does."""
+    crowd = np.full((4200, 2), 5.5, np.float32) + f32(np.random.default_rng(0).uniform(-0.4, 0.4, (4200, 2)))
+    pos = np.concatenate([crowd, f32([[20.5, 20.5], [21.2, 20.5]])])
+    idx, p, _ = oracle.pack(pos, np.zeros_like(pos), (0, 0, 32, 32), 1.0)
+    P = oracle.params(32, 32, 1, len(p))
+    out, _ = oracle.stencil_pass(P, idx, p, p)
+    in_crowd = np.abs(p[:, 0] - 5.5) < 1
+    assert np.array_equal(out[in_crowd], p[in_crowd])
+    assert out[~in_crowd][1, 0] - out[~in_crowd][0, 0] == F(1.0)
+
+
+def test_step_s_zero_passes_is_the_reference_step(oracle):
+    P, idx, p, v = _packed_world(oracle, 500, 20, 1, 5)
+    a = oracle.step(P, idx, p, v, nsteps=3)
+    b = oracle.step_s(P, idx, p, v, max_passes=0, nsteps=3)
+    for x, y in zip(a, b[:3]):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    assert b[3][2] == 0
+
+
+def test_step_s_passes_and_tolerance(oracle):
+    """Passes reduce the largest overlap; a tolerance at or above the first
+    overlap runs none; the frame after the passes is the reference step on
+    the separated positions."""
+    P, idx, p, v = _packed_world(oracle, 500, 20, 1, 6)
+    _, _, _, (before, after, ran) = oracle.step_s(P, idx, p, v, max_passes=4)
+    assert ran == 4 and after < before
+    _, _, _, st = oracle.step_s(P, idx, p, v, max_passes=4, overlap_tol=before)
+    assert st == (before, before, 0)
+    moved, _ = oracle.stencil_pass(P, idx, p, p)
+    i2, p2, v2, st1 = oracle.step_s(P, idx, p, v, max_passes=1)
+    r = oracle.step(P, idx, moved, v)
+    assert st1[2] == 1 and np.array_equal(bits(p2), bits(r[1])) and np.array_equal(i2, r[0])

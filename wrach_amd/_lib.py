@@ -20,6 +20,10 @@ WRACH_E_DEVICE = -5
 WRACH_E_BUFFER = -6
 WRACH_E_STATE = -7
 WRACH_FLAG_HOST_PACK = 0x1
+WRACH_SOLVER_R = 0
+WRACH_SOLVER_S = 1
+WRACH_MAX_PASSES = 64
+WRACH_STENCIL_BUDGET = 4096
 NUM_FRAME_KERNELS = 4
 FRAME_KERNEL_NAMES = ("collide", "cells", "scan", "scatter")
 
@@ -46,6 +50,8 @@ EXPORTS = (
     "wrach_set_tuning",
     "wrach_autotune",
     "wrach_set_graph",
+    "wrach_set_solver",
+    "wrach_solver_stats",
     "wrach_step_profiled",
     "wrach_last_error",
 )
@@ -143,6 +149,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_set_tuning.argtypes = [vp, ctypes.POINTER(TuningC)]
     L.wrach_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(TuningC)]
     L.wrach_set_graph.argtypes = [vp, ctypes.c_int]
+    L.wrach_set_solver.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]
+    L.wrach_solver_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                     ctypes.POINTER(ctypes.c_uint32)]
     L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.wrach_last_error.argtypes = []
     L.wrach_last_error.restype = ctypes.c_char_p

@@ -198,6 +198,22 @@ class World:
     def sync(self) -> None:
         check(self._L.wrach_sync(self._h))
 
+    def set_solver(self, mode: str = "R", max_passes: int = 0, overlap_tol: float = 0.0) -> None:
+        """wrach_set_solver: "R", the reference's step, or "S", up to
+        max_passes separation passes over the 3x3 cells around each particle
+        (each while the largest overlap exceeds overlap_tol), then that step."""
+        modes = {"R": _lib.WRACH_SOLVER_R, "S": _lib.WRACH_SOLVER_S}
+        if mode not in modes:
+            raise ValueError(f"unknown solver {mode!r}")
+        check(self._L.wrach_set_solver(self._h, modes[mode], int(max_passes), float(overlap_tol)))
+
+    def solver_stats(self):
+        """(overlap before the passes, overlap after them, passes run) of
+        the last mode S frame (wrach_solver_stats)."""
+        b, a, k = ctypes.c_float(), ctypes.c_float(), ctypes.c_uint32()
+        check(self._L.wrach_solver_stats(self._h, ctypes.byref(b), ctypes.byref(a), ctypes.byref(k)))
+        return b.value, a.value, k.value
+
     def set_graph(self, enable: bool) -> None:
         check(self._L.wrach_set_graph(self._h, 1 if enable else 0))
 

@@ -99,6 +99,13 @@ struct wrach_ctx {
     uint64_t world_cells = 0;          // cells of one world
     uint32_t cells_grid0 = 1;  // the cells kernel's grid chosen at creation
     uint64_t chunks = 0;       // chunks of kChunkCells cells
+    // Solver: WRACH_SOLVER_R, or WRACH_SOLVER_S with up to `passes`
+    // separation passes before the reference step (stencil_* kernels).
+    uint32_t solver = WRACH_SOLVER_R;
+    uint32_t passes = 0;
+    float overlap_tol = 0.0f;
+    uint32_t *scell = nullptr;   // frame-start cell of each particle (capacity entries)
+    uint32_t *swords = nullptr;  // wrach::kStencilWords words
 };
 
 namespace {
@@ -111,7 +118,40 @@ void free_graphs(wrach_ctx *c) {
     c->graph_one = c->graph_many = nullptr;
 }
 
+// Mode S's passes: P0 = pos; pass k reads pos or pos_out (k even or odd)
+// and writes the other (pos_out is free until the collide kernel writes
+// it); an odd count leaves the result in pos_out and a copy brings it back.
+int enqueue_stencil(wrach_ctx *c) {
+    if (c->solver != WRACH_SOLVER_S || c->passes == 0 || c->fp.n == 0) return WRACH_OK;
+    wrach::StencilArgs a{};
+    a.cell = c->scell;
+    a.words = c->swords;
+    a.tol = c->overlap_tol;
+    WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->stream));
+    a.pin = c->buf.pos;
+    a.first = 1;
+    a.in_slot = 0;
+    WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
+    a.first = 0;
+    for (uint32_t k = 0; k < c->passes; ++k) {
+        float2 *in = k % 2 == 0 ? c->buf.pos : c->buf.pos_out;
+        float2 *out = k % 2 == 0 ? c->buf.pos_out : c->buf.pos;
+        a.pin = in;
+        a.pout = out;
+        a.in_slot = k;
+        a.pass = k;
+        WRACH_HIP(wrach::launch_stencil_move(c->fp, c->buf, a, c->stream));
+        a.pin = out;
+        a.in_slot = k + 1;
+        WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
+    }
+    if (c->passes % 2) WRACH_HIP(wrach::launch_copy_pos(c->buf.pos, c->buf.pos_out, c->fp.n, c->stream));
+    return WRACH_OK;
+}
+
 int enqueue_frame(wrach_ctx *c) {
+    const int rc = enqueue_stencil(c);
+    if (rc) return rc;
     WRACH_HIP(wrach::launch_collide(c->fp, c->buf, c->stream));
     WRACH_HIP(wrach::launch_cells(c->fp, c->buf, c->stream));
     WRACH_HIP(wrach::launch_scan(c->fp, c->buf, c->stream));
@@ -161,6 +201,8 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->dvalid);
     (void)hipFree(c->buf.cells);
     (void)hipFree(c->buf.class_ctr);
+    (void)hipFree(c->scell);
+    (void)hipFree(c->swords);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -632,6 +674,8 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
         hipError_t (*const launch[K])(const wrach::FrameParams &, const wrach::FrameBuffers &,
                                       hipStream_t, hipEvent_t, hipEvent_t) = {
             wrach::launch_collide, wrach::launch_cells, wrach::launch_scan, wrach::launch_scatter};
+        rc = enqueue_stencil(c);  // mode S's passes, untimed
+        if (rc) break;
         hipError_t e = hipSuccess;
         // The launch stamps both events with the kernel's own start and end.
         for (int k = 0; k < K && e == hipSuccess; ++k)
@@ -662,6 +706,51 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
 int wrach_sync(wrach_ctx *c) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
     WRACH_HIP(hipStreamSynchronize(c->stream));
+    return WRACH_OK;
+}
+
+static_assert(WRACH_MAX_PASSES == wrach::kMaxStencilPasses, "one pass limit");
+static_assert(WRACH_STENCIL_BUDGET == wrach::kStencilBudget, "one budget");
+
+int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float overlap_tol) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (mode != WRACH_SOLVER_R && mode != WRACH_SOLVER_S) return fail(WRACH_E_INVALID, "no such solver");
+    if (mode == WRACH_SOLVER_S) {
+        if (max_passes > wrach::kMaxStencilPasses)
+            return fail(WRACH_E_INVALID, "more than " + std::to_string(wrach::kMaxStencilPasses) + " passes");
+        if (!(overlap_tol >= 0.0f) || std::isinf(overlap_tol))
+            return fail(WRACH_E_INVALID, "overlap tolerance must be finite and >= 0");
+        if (c->worlds > 1) return fail(WRACH_E_INVALID, "the stencil solver steps one world, not a batch");
+    }
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (mode == WRACH_SOLVER_S && !c->scell) {
+        WRACH_HIP(dmalloc(&c->scell, c->capacity ? c->capacity : 1));
+        if (!c->swords) WRACH_HIP(dmalloc(&c->swords, wrach::kStencilWords));
+    }
+    if (c->swords) WRACH_HIP(hipMemset(c->swords, 0, wrach::kStencilWords * sizeof(uint32_t)));
+    free_graphs(c);  // captured with the previous solver
+    c->solver = mode;
+    c->passes = mode == WRACH_SOLVER_S ? max_passes : 0;
+    c->overlap_tol = mode == WRACH_SOLVER_S ? overlap_tol : 0.0f;
+    g_err.clear();
+    return WRACH_OK;
+}
+
+int wrach_solver_stats(wrach_ctx *c, float *overlap_before, float *overlap_after, uint32_t *passes) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    uint32_t w[wrach::kStencilWords] = {};
+    if (c->swords) {
+        WRACH_HIP(hipSetDevice(c->cfg.device));
+        WRACH_HIP(hipStreamSynchronize(c->stream));
+        WRACH_HIP(hipMemcpy(w, c->swords, sizeof w, hipMemcpyDeviceToHost));
+    }
+    float before, after;
+    std::memcpy(&before, &w[0], sizeof before);
+    std::memcpy(&after, &w[c->passes], sizeof after);
+    if (overlap_before) *overlap_before = before;
+    if (overlap_after) *overlap_after = after;
+    if (passes) *passes = w[wrach::kStencilPassesWord];
     return WRACH_OK;
 }
 
@@ -847,11 +936,10 @@ int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, c
 namespace {
 
 // Candidate grids of the cells kernel around the creation default g0.
-std::vector<uint32_t> cells_grid_candidates(uint32_t g0) {
-    std::vector<uint32_t> out;
+void cells_grid_candidates(uint32_t g0, std::vector<uint32_t> &out) {
+    out.clear();
     for (const uint32_t g : {g0, std::max(1u, g0 * 3 / 4), std::max(1u, g0 / 2), g0 + g0 / 2})
         if (std::find(out.begin(), out.end(), g) == out.end()) out.push_back(g);
-    return out;
 }
 
 wrach_tuning current_tuning(const wrach_ctx *c) {
@@ -985,7 +1073,7 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
         else if (knob == 1)
             for (int i = 0; i < wrach::kNumScatterShapes; ++i) cand.push_back((uint32_t)i);
         else
-            cand = cells_grid_candidates(c->cells_grid0);
+            cells_grid_candidates(c->cells_grid0, cand);
         uint32_t best = knob == 0 ? scan : knob == 1 ? scatter : grid;
         float best_here = -1.0f;
         for (const uint32_t v : cand) {

@@ -156,4 +156,29 @@ hipError_t launch_store_outside(const UploadParams &u, float2 *dpos, float2 *dve
 // run of its own (src = store index | run flag).
 hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, hipStream_t stream);
 
+// Mode S: separation passes over the 3x3 cells around each particle before
+// the reference step (the kernels' comment has the rule).  `words` holds the
+// largest overlap (float bits) after 0..max_passes passes, then the number
+// of passes that ran, for the last frame.
+constexpr uint32_t kMaxStencilPasses = 64;
+constexpr uint32_t kStencilPassesWord = kMaxStencilPasses + 1;
+constexpr uint32_t kStencilWords = kMaxStencilPasses + 2;
+constexpr uint32_t kStencilBudget = 4096;  // most particles one lane visits in its 3x3 cells
+struct StencilArgs {
+    const float2 *pin;  // positions entering the kernel
+    float2 *pout;       // a move's output
+    uint32_t *cell;     // each particle's cell at the frame's start
+    uint32_t *words;    // kStencilWords words
+    uint32_t in_slot;   // a measure: the word it raises; a move: the word of its start
+    uint32_t first;     // a measure of the frame-start positions (writes `cell`)
+    uint32_t pass;      // a move's number, from 0
+    float tol;          // a move runs while the overlap at its start exceeds this
+};
+hipError_t launch_stencil_measure(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
+                                  hipStream_t stream);
+hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
+                               hipStream_t stream);
+hipError_t launch_stencil_clear(uint32_t *words, hipStream_t stream);
+hipError_t launch_copy_pos(float2 *dst, const float2 *src, uint32_t n, hipStream_t stream);
+
 }  // namespace wrach

@@ -1686,6 +1686,155 @@ __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, Fra
     b.src[dest] = i | kRunFlag;
 }
 
+// ---------------------------------------------------------------------------
+// Mode S: separation passes over the 3x3 cells around each particle, before
+// the reference step (an extension: the reference collides within one cell,
+// shaders/physics/src/cell.rs:60-69).  A pass is Jacobi -- every particle
+// reads the positions the pass started from -- and a particle's pair terms
+// are rounded to multiples of 2^-32 and summed in int64, exactly, so its
+// displacement does not depend on the order of the particles inside a cell
+// (which the device layout keeps canonical only through `src`).
+
+constexpr float kStencilScale = 4294967296.0f;            // 2^32
+constexpr float kStencilUnit = 2.3283064365386963e-10f;   // 2^-32
+
+// Offsets [s, e) of cell k's particles.  A cell of a chunk the last scan
+// found empty holds none (its offsets are stale); the last cell of a live
+// chunk takes its end from the next chunk's prefix when that one is empty.
+__device__ __forceinline__ void cell_span(const FrameParams &p, const FrameBuffers &b, uint32_t k,
+                                          uint32_t &s, uint32_t &e) {
+    const uint32_t ch = k >> kChunkShift;
+    s = e = 0;
+    if (!b.chunk_live[ch]) return;
+    s = b.idx[k + 1];
+    const uint32_t nx = k + 1;
+    if (nx >= p.ncell) {
+        e = p.n;
+    } else {
+        const uint32_t nc = nx >> kChunkShift;
+        e = (nc == ch || b.chunk_live[nc]) ? b.idx[nx + 1] : b.chunk_prefix[nc];
+    }
+    if (s > p.n) s = p.n;
+    if (e > p.n) e = p.n;
+    if (e < s) e = s;
+}
+
+// collide_pair's rule for particle i alone: whether j is within reach, the
+// overlap 1 - d, and (p_j - p_i) * force, which the reference subtracts
+// from p_i (particles.rs:65-92).
+__device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float yj, float &ex,
+                                             float &ey, float &ov) {
+    const float dx = xi - xj;
+    const float dy = yi - yj;
+    float sq = dx * dx;
+    sq = sq + dy * dy;
+    float d = sqrtf(sq);
+    if (d > 1.0f) return false;
+    if (d == 0.0f) d = 0.0001f;
+    ov = 1.0f - d;
+    const float f = (0.5f * (1.0f - d)) / d;
+    ex = (xj - xi) * f;
+    ey = (yj - yi) * f;
+    return true;
+}
+
+// Particle i (in cell k at the frame's start) against the 3x3 cells around
+// k: the largest overlap, and with MOVE the displacement in 2^-32 units.
+// Over kStencilBudget particles in those cells: neither (a bound on the
+// work of one lane).
+template <bool MOVE>
+__device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
+                                                  const float2 *pin, uint32_t i, uint32_t k,
+                                                  float xi, float yi, long long &ax,
+                                                  long long &ay) {
+    const int cx = (int)(k % p.gx), cy = (int)(k / p.gx);
+    const int gx = (int)p.gx, gy = (int)p.gy;
+    uint32_t s[9], e[9];
+    uint64_t total = 0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int x = cx + t % 3 - 1, y = cy + t / 3 - 1;
+        s[t] = e[t] = 0;
+        if (x >= 0 && y >= 0 && x < gx && y < gy) cell_span(p, b, (uint32_t)(y * gx + x), s[t], e[t]);
+        total += e[t] - s[t];
+    }
+    float worst = 0.0f;
+    if (total > kStencilBudget) return worst;
+    for (int t = 0; t < 9; ++t) {
+        for (uint32_t j = s[t]; j < e[t]; ++j) {
+            if (j == i) continue;
+            const float2 q = pin[j];
+            float ex, ey, ov;
+            if (!stencil_pair(xi, yi, q.x, q.y, ex, ey, ov)) continue;
+            worst = fmaxf(worst, ov);
+            if (MOVE) {
+                ax -= __float2ll_rn(ex * kStencilScale);
+                ay -= __float2ll_rn(ey * kStencilScale);
+            }
+        }
+    }
+    return worst;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// The largest overlap at positions a.pin, raised into a.words[a.in_slot]
+// (non-negative floats order as their bits).  a.first: the frame-start
+// positions, whose cells every pass of the frame uses.
+__global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, FrameBuffers b,
+                                                              StencilArgs a) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    float worst = 0.0f;
+    if (i < p.n) {
+        const float2 q = a.pin[i];
+        uint32_t k;
+        if (a.first) {
+            k = cell_key(p, q.x, q.y, 0u);
+            a.cell[i] = k;
+        } else {
+            k = a.cell[i];
+        }
+        long long ax = 0, ay = 0;
+        if (k < p.ncell) worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay);
+    }
+    worst = wave_max(worst);
+    if (lane_id() == 0 && worst > 0.0f) atomicMax(&a.words[a.in_slot], __float_as_uint(worst));
+}
+
+// Pass a.pass: a.pin -> a.pout, run only while the overlap at its start
+// (a.words[a.in_slot]) exceeds the tolerance; otherwise a copy.
+__global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameBuffers b,
+                                                           StencilArgs a) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.n) return;
+    const float2 q = a.pin[i];
+    if (!(__uint_as_float(a.words[a.in_slot]) > a.tol)) {
+        a.pout[i] = q;
+        return;
+    }
+    if (i == 0) a.words[kStencilPassesWord] = a.pass + 1;
+    long long ax = 0, ay = 0;
+    const uint32_t k = a.cell[i];
+    if (k < p.ncell) (void)stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay);
+    float2 r;
+    r.x = q.x + __ll2float_rn(ax) * kStencilUnit;
+    r.y = q.y + __ll2float_rn(ay) * kStencilUnit;
+    a.pout[i] = r;
+}
+
+__global__ __launch_bounds__(64) void stencil_clear_kernel(uint32_t *w) {
+    for (uint32_t i = threadIdx.x; i < kStencilWords; i += 64) w[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void copy_pos_kernel(float2 *dst, const float2 *src, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
 
 // The frame launchers take optional start/stop events, which hipExtLaunchKernel
@@ -1799,6 +1948,31 @@ hipError_t launch_store_outside(const UploadParams &u, float2 *dpos, float2 *dve
 hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, hipStream_t stream) {
     const uint32_t blocks = u.n ? (u.n + 255) / 256 : 1;  // block 0 advances the epoch
     hipLaunchKernelGGL(upload_scatter_kernel, dim3(blocks), dim3(256), 0, stream, u, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_stencil_measure(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
+                                  hipStream_t stream) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(stencil_measure_kernel, dim3((p.n + 255) / 256), dim3(256), 0, stream, p, b, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
+                               hipStream_t stream) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(stencil_move_kernel, dim3((p.n + 255) / 256), dim3(256), 0, stream, p, b, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stencil_clear(uint32_t *words, hipStream_t stream) {
+    hipLaunchKernelGGL(stencil_clear_kernel, dim3(1), dim3(64), 0, stream, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_pos(float2 *dst, const float2 *src, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_pos_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dst, src, n);
     return hipGetLastError();
 }
 
