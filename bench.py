@@ -206,6 +206,9 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=0, metavar="FRAMES",
                     help="run wrach_autotune over FRAMES frames per candidate after the warmup "
                          "(untimed) and report the chosen launch shapes")
+    ap.add_argument("--passes", type=int, default=0, metavar="K",
+                    help="mode S: K separation passes over the 3x3 cells around each particle "
+                         "before the reference step (an extension; default 0, the reference's step)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -228,6 +231,8 @@ def main() -> int:
     world = World(wl.config)
     world.add_particles(wl.pos, wl.vel)
     world.set_graph(not args.no_graph)
+    if args.passes > 0:
+        world.set_solver("S", args.passes)
     world.step(args.warmup)
     world.sync()
     tuning = world.autotune(args.autotune) if args.autotune > 0 else world.tuning()
@@ -239,6 +244,12 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     value = n * args.steps / elapsed
     print(f"[bench] timed {args.steps} frames in {elapsed * 1e3:.2f} ms", file=sys.stderr, flush=True)
+    solver = {"mode": "R"}
+    if args.passes > 0:
+        before, after, ran = world.solver_stats()
+        solver = {"mode": "S", "passes": args.passes, "overlap_tol": 0.0,
+                  "last_frame_overlap": [before, after], "last_frame_passes": ran}
+        print(f"[bench] mode S: {json.dumps(solver)}", file=sys.stderr, flush=True)
 
     # Per-kernel durations: kernel-stamped HIP events, plain launches.
     split_before = particles_by_kernel(world.read()[0])
@@ -263,7 +274,7 @@ def main() -> int:
     print(f"[bench] upload ms: {json.dumps(upload)}", file=sys.stderr, flush=True)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.passes == 0:
         cpu = cpu_baseline(wl)
         print(f"[bench] cpu baseline {cpu['value']:.3e} p-steps/s", file=sys.stderr, flush=True)
 
@@ -291,6 +302,7 @@ def main() -> int:
             "graph": not args.no_graph,
             "launch_shapes": {k: v for k, v in tuning.items() if k != "frame_us"},
             "autotuned": args.autotune > 0,
+            "solver": solver,
         },
         "roofline": {
             "bound": "hbm",

@@ -231,12 +231,12 @@ int wrach_autotune(wrach_ctx *ctx, uint32_t frames, wrach_tuning *out);
  * WRACH_SOLVER_S is an extension with no counterpart in the reference: each
  * frame first runs up to max_passes separation passes over the 3x3 cells
  * around every particle, then the reference step.  A pass is Jacobi: every
- * particle i moves by the sum, over the other particles j in the 3x3 cells
- * around its frame-start cell, of the reference's pair displacement
- * -(p_j - p_i) * (0.5 * (1 - d)) / d for d <= 1 (particles.rs:65-92; d == 0
- * counts as 0.0001).  The terms are rounded to multiples of 2^-32 and summed
- * exactly, so the result does not depend on the order of particles in a
- * cell.  A pass runs only while the largest overlap 1 - d at its start
+ * particle i moves by the mean, over the other particles j in the 3x3 cells
+ * around its frame-start cell with d <= 1, of the reference's pair
+ * displacement -(p_j - p_i) * (0.5 * (1 - d)) / d (particles.rs:65-92;
+ * d == 0 counts as 0.0001).  The terms are rounded to multiples of 2^-32 and
+ * summed exactly, so the result does not depend on the order of particles
+ * in a cell.  A pass runs only while the largest overlap 1 - d at its start
  * exceeds overlap_tol.  A particle whose 3x3 cells hold more than
  * WRACH_STENCIL_BUDGET particles is not displaced by the passes. */
 #define WRACH_SOLVER_R 0u

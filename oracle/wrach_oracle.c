@@ -436,6 +436,7 @@ float wo_stencil_pass(const wo_params *p, const uint32_t *idx, const float *pcel
         float xi = pin[2 * i], yi = pin[2 * i + 1];
         int64_t k = stencil_cell(p, pcell, i);
         int64_t ax = 0, ay = 0;  /* the displacement in WO_STENCIL_SCALE units */
+        uint32_t pairs = 0;      /* overlapping neighbours */
         if (k >= 0 && stencil_count(p, idx, k) <= WO_STENCIL_BUDGET) {
             int64_t cx = k % gx, cy = k / gx;
             for (int64_t y = cy - 1; y <= cy + 1; ++y) {
@@ -449,6 +450,7 @@ float wo_stencil_pass(const wo_params *p, const uint32_t *idx, const float *pcel
                         if (!stencil_pair(xi, yi, pin[2 * j], pin[2 * j + 1], &ex, &ey, &ov))
                             continue;
                         if (ov > worst) worst = ov;
+                        pairs += 1;
                         ax -= llrintf(ex * WO_STENCIL_SCALE);
                         ay -= llrintf(ey * WO_STENCIL_SCALE);
                     }
@@ -456,8 +458,14 @@ float wo_stencil_pass(const wo_params *p, const uint32_t *idx, const float *pcel
             }
         }
         if (move) {
-            pout[2 * i] = xi + (float)ax * (1.0f / WO_STENCIL_SCALE);
-            pout[2 * i + 1] = yi + (float)ay * (1.0f / WO_STENCIL_SCALE);
+            float dx = (float)ax * (1.0f / WO_STENCIL_SCALE);
+            float dy = (float)ay * (1.0f / WO_STENCIL_SCALE);
+            if (pairs > 1) {  /* the mean of the pair corrections */
+                dx = dx / (float)pairs;
+                dy = dy / (float)pairs;
+            }
+            pout[2 * i] = xi + dx;
+            pout[2 * i + 1] = yi + dy;
         }
     }
     return worst;

@@ -148,7 +148,10 @@ int wo_step_mt(const wo_params *p, uint32_t *indices, float *pos, float *vel,
  * (0.5 * (1 - d)) / d, p_i -= (p_j - p_i) * force.  The terms are rounded to
  * multiples of 1/WO_STENCIL_SCALE (llrintf, nearest-even) and summed in
  * int64, exactly, so the displacement does not depend on the order of the
- * particles in a cell; p_i' = p_i + (float)sum / WO_STENCIL_SCALE.  A
+ * particles in a cell.  The displacement is their mean (constraint averaging
+ * of position-based dynamics, which bounds a particle's move to 0.5 per
+ * pass): with m > 1 overlapping neighbours, D = (float)sum / WO_STENCIL_SCALE
+ * then D / (float)m, each a rounded f32 operation; p_i' = p_i + D.  A
  * particle whose 3x3 cells hold more than WO_STENCIL_BUDGET particles
  * (itself included) takes no displacement and measures no overlap: a bound
  * on the work per particle.  With cell sides >= 1 every pair closer than 1

@@ -315,7 +315,7 @@ def _packed_world(oracle, n, side, cs, seed, spread=1.0):
 
 def _brute_force_pass(pin):
     """All-pairs Jacobi pass in numpy: the pair rule in float32 operation by
-    operation, the sum of a particle's terms in float64."""
+    operation, the mean of a particle's terms in float64."""
     x, y = pin[:, 0], pin[:, 1]
     dx = x[:, None] - x[None, :]
     dy = y[:, None] - y[None, :]
@@ -329,6 +329,7 @@ def _brute_force_pass(pin):
     ey = ((y[None, :] - y[:, None]).astype(F) * force).astype(F)
     disp = np.stack([-(np.where(near, ex, 0).astype(np.float64)).sum(1),
                      -(np.where(near, ey, 0).astype(np.float64)).sum(1)], 1)
+    disp /= np.maximum(near.sum(1), 1)[:, None]  # the mean over the overlapping neighbours
     ov = np.where(near, F(1) - d, 0).max() if near.any() else 0.0
     return pin.astype(np.float64) + disp, float(ov)
 

@@ -1739,14 +1739,14 @@ __device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float
 }
 
 // Particle i (in cell k at the frame's start) against the 3x3 cells around
-// k: the largest overlap, and with MOVE the displacement in 2^-32 units.
-// Over kStencilBudget particles in those cells: neither (a bound on the
-// work of one lane).
+// k: the largest overlap, and with MOVE the sum of its pair terms in 2^-32
+// units and their number.  Over kStencilBudget particles in those cells:
+// neither (a bound on the work of one lane).
 template <bool MOVE>
 __device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
                                                   const float2 *pin, uint32_t i, uint32_t k,
                                                   float xi, float yi, long long &ax,
-                                                  long long &ay) {
+                                                  long long &ay, uint32_t &pairs) {
     const int cx = (int)(k % p.gx), cy = (int)(k / p.gx);
     const int gx = (int)p.gx, gy = (int)p.gy;
     uint32_t s[9], e[9];
@@ -1768,6 +1768,7 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
             if (!stencil_pair(xi, yi, q.x, q.y, ex, ey, ov)) continue;
             worst = fmaxf(worst, ov);
             if (MOVE) {
+                pairs += 1;
                 ax -= __float2ll_rn(ex * kStencilScale);
                 ay -= __float2ll_rn(ey * kStencilScale);
             }
@@ -1780,6 +1781,20 @@ __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
     return v;
+}
+
+// Raises *w to the block's largest v.  One atomic per block, and none when
+// the word already holds as much (every block hitting one word serialises).
+__device__ __forceinline__ void block_raise_max(uint32_t *w, float v) {
+    __shared__ float s_max[4];
+    v = wave_max(v);
+    if (lane_id() == 0) s_max[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+        if (m > 0.0f && __float_as_uint(m) > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(w, __float_as_uint(m));
+    }
 }
 
 // The largest overlap at positions a.pin, raised into a.words[a.in_slot]
@@ -1799,10 +1814,10 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
             k = a.cell[i];
         }
         long long ax = 0, ay = 0;
-        if (k < p.ncell) worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay);
+        uint32_t pairs = 0;
+        if (k < p.ncell) worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs);
     }
-    worst = wave_max(worst);
-    if (lane_id() == 0 && worst > 0.0f) atomicMax(&a.words[a.in_slot], __float_as_uint(worst));
+    block_raise_max(&a.words[a.in_slot], worst);
 }
 
 // Pass a.pass: a.pin -> a.pout, run only while the overlap at its start
@@ -1818,12 +1833,18 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
     }
     if (i == 0) a.words[kStencilPassesWord] = a.pass + 1;
     long long ax = 0, ay = 0;
+    uint32_t pairs = 0;
     const uint32_t k = a.cell[i];
-    if (k < p.ncell) (void)stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay);
-    float2 r;
-    r.x = q.x + __ll2float_rn(ax) * kStencilUnit;
-    r.y = q.y + __ll2float_rn(ay) * kStencilUnit;
-    a.pout[i] = r;
+    if (k < p.ncell) (void)stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs);
+    // The mean of the pair corrections (constraint averaging): a particle
+    // moves at most 0.5 per pass.
+    float dx = __ll2float_rn(ax) * kStencilUnit;
+    float dy = __ll2float_rn(ay) * kStencilUnit;
+    if (pairs > 1) {
+        dx = dx / (float)pairs;
+        dy = dy / (float)pairs;
+    }
+    a.pout[i] = make_float2(q.x + dx, q.y + dy);
 }
 
 __global__ __launch_bounds__(64) void stencil_clear_kernel(uint32_t *w) {
