@@ -1698,25 +1698,35 @@ __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, Fra
 constexpr float kStencilScale = 4294967296.0f;            // 2^32
 constexpr float kStencilUnit = 2.3283064365386963e-10f;   // 2^-32
 
-// Offsets [s, e) of cell k's particles.  A cell of a chunk the last scan
-// found empty holds none (its offsets are stale); the last cell of a live
-// chunk takes its end from the next chunk's prefix when that one is empty.
-__device__ __forceinline__ void cell_span(const FrameParams &p, const FrameBuffers &b, uint32_t k,
-                                          uint32_t &s, uint32_t &e) {
+// First particle of cell k (k == ncell: the end).  Cells of a chunk the last
+// scan found empty all start at the chunk's prefix (their idx entries are
+// stale); a live chunk's are in idx.
+__device__ __forceinline__ uint32_t cell_start(const FrameParams &p, const FrameBuffers &b,
+                                               uint32_t k) {
+    if (k >= p.ncell) return p.n;
     const uint32_t ch = k >> kChunkShift;
-    s = e = 0;
-    if (!b.chunk_live[ch]) return;
-    s = b.idx[k + 1];
-    const uint32_t nx = k + 1;
-    if (nx >= p.ncell) {
-        e = p.n;
-    } else {
-        const uint32_t nc = nx >> kChunkShift;
-        e = (nc == ch || b.chunk_live[nc]) ? b.idx[nx + 1] : b.chunk_prefix[nc];
+    const uint32_t s = b.chunk_live[ch] ? b.idx[k + 1] : b.chunk_prefix[ch];
+    return s < p.n ? s : p.n;
+}
+
+// The particles of the 3x3 cells around (cx, cy) as three ranges, one per
+// row cy - 1 .. cy + 1: the cells of one row are contiguous in the packed
+// layout, and [start(x0), start(x1 + 1)) covers columns x0..x1 (past the
+// row's last column, start(x1 + 1) is the next row's first cell, which
+// begins where the row ends).
+__device__ __forceinline__ void stencil_rows(const FrameParams &p, const FrameBuffers &b, int cx,
+                                             int cy, uint32_t rs[3], uint32_t re[3]) {
+    const int gx = (int)p.gx, gy = (int)p.gy;
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gx ? cx + 1 : gx - 1;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int y = cy + t - 1;
+        rs[t] = re[t] = 0;
+        if (y < 0 || y >= gy) continue;
+        rs[t] = cell_start(p, b, (uint32_t)(y * gx + x0));
+        re[t] = cell_start(p, b, (uint32_t)(y * gx + x1 + 1));
+        if (re[t] < rs[t]) re[t] = rs[t];
     }
-    if (s > p.n) s = p.n;
-    if (e > p.n) e = p.n;
-    if (e < s) e = s;
 }
 
 // collide_pair's rule for particle i alone: whether j is within reach, the
@@ -1728,6 +1738,11 @@ __device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float
     const float dy = yi - yj;
     float sq = dx * dx;
     sq = sq + dy * dy;
+    // sqrtf(sq) > 1 exactly when sq > 1 + 2^-23: sqrtf(1 + 2^-23) rounds to
+    // 1 and the root of the next float above it does not.  Most neighbours of
+    // a crowded cell are out of reach; they skip the root and the division
+    // with the same outcome.
+    if (sq > 1.00000011920928955078125f) return false;
     float d = sqrtf(sq);
     if (d > 1.0f) return false;
     if (d == 0.0f) d = 0.0001f;
@@ -1738,30 +1753,22 @@ __device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float
     return true;
 }
 
-// Particle i (in cell k at the frame's start) against the 3x3 cells around
-// k: the largest overlap, and with MOVE the sum of its pair terms in 2^-32
-// units and their number.  Over kStencilBudget particles in those cells:
-// neither (a bound on the work of one lane).
+// Particle i (in cell (cx, cy) at the frame's start) against the 3x3 cells
+// around it: the largest overlap and, with MOVE, the sum of its pair terms
+// in 2^-32 units and their number.  Over kStencilBudget particles in those
+// cells: neither (a bound on the work of one lane).
 template <bool MOVE>
 __device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
                                                   const float2 *pin, uint32_t i, uint32_t k,
                                                   float xi, float yi, long long &ax,
                                                   long long &ay, uint32_t &pairs) {
-    const int cx = (int)(k % p.gx), cy = (int)(k / p.gx);
-    const int gx = (int)p.gx, gy = (int)p.gy;
-    uint32_t s[9], e[9];
-    uint64_t total = 0;
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int x = cx + t % 3 - 1, y = cy + t / 3 - 1;
-        s[t] = e[t] = 0;
-        if (x >= 0 && y >= 0 && x < gx && y < gy) cell_span(p, b, (uint32_t)(y * gx + x), s[t], e[t]);
-        total += e[t] - s[t];
-    }
+    uint32_t rs[3], re[3];
+    stencil_rows(p, b, (int)(k % p.gx), (int)(k / p.gx), rs, re);
+    const uint64_t total = (uint64_t)(re[0] - rs[0]) + (re[1] - rs[1]) + (re[2] - rs[2]);
     float worst = 0.0f;
     if (total > kStencilBudget) return worst;
-    for (int t = 0; t < 9; ++t) {
-        for (uint32_t j = s[t]; j < e[t]; ++j) {
+    for (int t = 0; t < 3; ++t) {
+        for (uint32_t j = rs[t]; j < re[t]; ++j) {
             if (j == i) continue;
             const float2 q = pin[j];
             float ex, ey, ov;
@@ -1821,7 +1828,8 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
 }
 
 // Pass a.pass: a.pin -> a.pout, run only while the overlap at its start
-// (a.words[a.in_slot]) exceeds the tolerance; otherwise a copy.
+// (a.words[a.in_slot], the same for the whole grid) exceeds the tolerance;
+// otherwise a copy.
 __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameBuffers b,
                                                            StencilArgs a) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
