@@ -149,6 +149,21 @@ class WrachAPI {
                                 reinterpret_cast<const float *>(d.velocities.data()), n));
         check(wrach_settings(ctx_, &state_.shader_settings));
     }
+    // Not in the reference: the solver (WRACH_SOLVER_R, or WRACH_SOLVER_S with
+    // separation passes over the 3x3 cells before the reference step) and
+    // the last mode S frame's overlap before and after its passes.
+    void set_solver(uint32_t mode, uint32_t max_passes = 0, float overlap_tol = 0.0f) {
+        check(wrach_set_solver(ctx_, mode, max_passes, overlap_tol));
+    }
+    struct SolverStats {
+        float overlap_before = 0.0f, overlap_after = 0.0f;
+        uint32_t passes = 0;
+    };
+    SolverStats solver_stats() {
+        SolverStats st;
+        check(wrach_solver_stats(ctx_, &st.overlap_before, &st.overlap_after, &st.passes));
+        return st;
+    }
     wrach_ctx *raw() { return ctx_; }
 
   private:

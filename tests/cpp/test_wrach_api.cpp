@@ -109,6 +109,26 @@ static void checkpoint_and_write_back(bool host_pack) {
     CHECK(a.velocities == b.velocities);
 }
 
+// Mode S through the mirror: two particles 0.6 apart in neighbouring cells
+// (which mode R never separates) end a pass exactly 1 apart before the
+// step moves them; with no velocity and no gravity the frame keeps that.
+static void stencil_solver() {
+    wrach::WrachConfig cfg;
+    cfg.dimensions = {16, 16};
+    cfg.cell_size = 1;
+    wrach::WrachAPI a(cfg);
+    a.add_particles({{{5.5f, 5.5f}, {0.0f, 0.0f}}, {{6.1f, 5.5f}, {0.0f, 0.0f}}});
+    a.set_solver(WRACH_SOLVER_S, 2, 0.0f);
+    a.tick();
+    CHECK(a.positions[1].first - a.positions[0].first == 1.0f);
+    const auto st = a.solver_stats();
+    CHECK(st.passes == 1);  // the second pass starts with no overlap left
+    CHECK(st.overlap_after == 0.0f && st.overlap_before > 0.39f && st.overlap_before < 0.41f);
+    a.set_solver(WRACH_SOLVER_R);
+    a.tick();
+    CHECK(a.positions[1].first - a.positions[0].first == 1.0f);
+}
+
 int main() {
     api_returns_data();
     prefix_sum_for_small_arrays();
@@ -116,6 +136,7 @@ int main() {
     capacity_error();
     checkpoint_and_write_back(false);
     checkpoint_and_write_back(true);
+    stencil_solver();
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);
     return failures ? 1 : 0;
 }
