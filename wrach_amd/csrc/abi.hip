@@ -724,9 +724,17 @@ int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float ove
     }
     WRACH_HIP(hipSetDevice(c->cfg.device));
     WRACH_HIP(hipStreamSynchronize(c->stream));
+    // Each buffer on its own: a failed allocation leaves the solver as it was,
+    // and the next call allocates what is still missing.
     if (mode == WRACH_SOLVER_S && !c->scell) {
-        WRACH_HIP(dmalloc(&c->scell, c->capacity ? c->capacity : 1));
-        if (!c->swords) WRACH_HIP(dmalloc(&c->swords, wrach::kStencilWords));
+        uint32_t *cell = nullptr;
+        WRACH_HIP(dmalloc(&cell, c->capacity ? c->capacity : 1));
+        c->scell = cell;
+    }
+    if (mode == WRACH_SOLVER_S && !c->swords) {
+        uint32_t *words = nullptr;
+        WRACH_HIP(dmalloc(&words, wrach::kStencilWords));
+        c->swords = words;
     }
     if (c->swords) WRACH_HIP(hipMemset(c->swords, 0, wrach::kStencilWords * sizeof(uint32_t)));
     free_graphs(c);  // captured with the previous solver
