@@ -152,3 +152,27 @@ def test_stencil_lattice_1m_full_size(oracle):
     exact against the oracle for 3 frames."""
     wl = workloads.lattice_1m()
     run_stencil(oracle, wl.config, wl.pos, wl.vel, 3, 2, every=3, label=wl.name)
+
+
+def test_stencil_with_autotune_and_profiled_frames(oracle):
+    """The autotuner's trial frames and the event-timed frames run mode S's
+    passes too: the world continues bit for bit after autotuning (which
+    restores the state) and after profiled frames (which step it)."""
+    pos, vel = uniform(12000, 384, 256, 19)
+    cfg = WrachConfig(dimensions=(384, 256), cell_size=1, gravity=(0.0, -0.1))
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        w.set_solver("S", 2)
+        w.step(4)
+        *ref, _ = oracle.step_s(P, *ref, max_passes=2, nsteps=4)
+        w.autotune(2)
+        assert_same_state(w.read(), tuple(ref), "after autotune")
+        ms = w.step_profiled(3)
+        assert all(v > 0 for v in ms.values())
+        *ref, st = oracle.step_s(P, *ref, max_passes=2, nsteps=3)
+        assert_same_state(w.read(), tuple(ref), "after profiled frames")
+        assert w.solver_stats() == st
+        w.step(5)
+        *ref, _ = oracle.step_s(P, *ref, max_passes=2, nsteps=5)
+        assert_same_state(w.read(), tuple(ref), "graph frames after")
