@@ -246,9 +246,11 @@ def main() -> int:
     print(f"[bench] timed {args.steps} frames in {elapsed * 1e3:.2f} ms", file=sys.stderr, flush=True)
     solver = {"mode": "R"}
     if args.passes > 0:
-        before, after, ran = world.solver_stats()
+        before, after, ran, (sb, nb, sa, na) = world.solver_stats()
         solver = {"mode": "S", "passes": args.passes, "overlap_tol": 0.0,
-                  "last_frame_overlap": [before, after], "last_frame_passes": ran}
+                  "last_frame_max_overlap": [before, after], "last_frame_passes": ran,
+                  "last_frame_mean_overlap": [sb / 2**24 / nb if nb else 0.0, sa / 2**24 / na if na else 0.0],
+                  "last_frame_overlapping_pairs": [nb // 2, na // 2]}
         print(f"[bench] mode S: {json.dumps(solver)}", file=sys.stderr, flush=True)
 
     # Per-kernel durations: kernel-stamped HIP events, plain launches.

@@ -250,10 +250,24 @@ int wrach_autotune(wrach_ctx *ctx, uint32_t frames, wrach_tuning *out);
  * WRACH_SOLVER_S the reference's step. */
 int wrach_set_solver(wrach_ctx *ctx, uint32_t mode, uint32_t max_passes, float overlap_tol);
 
-/* The last WRACH_SOLVER_S frame's largest overlap before and after its
- * passes, and the number of passes that ran (zeros before the first such
- * frame).  Synchronises the stream. */
-int wrach_solver_stats(wrach_ctx *ctx, float *overlap_before, float *overlap_after, uint32_t *passes);
+/* The last WRACH_SOLVER_S frame's overlaps (1 - d over the pairs its passes
+ * visit) before and after its passes.  The sums are exact: each overlap is
+ * rounded to a multiple of 2^-24 and they are added as integers, every pair
+ * counted from both of its particles, so the mean overlap is
+ * overlap_sum / 2^24 / pairs.  Zeros before the first such frame. */
+typedef struct wrach_solver_report {
+    float max_overlap_before;     /* largest 1 - d */
+    float max_overlap_after;
+    uint32_t passes;              /* passes that ran */
+    uint32_t reserved;
+    uint64_t overlap_sum_before;  /* sum of 1 - d, in units of 2^-24 */
+    uint64_t pairs_before;        /* overlapping pairs, each counted twice */
+    uint64_t overlap_sum_after;
+    uint64_t pairs_after;
+} wrach_solver_report;
+
+/* Fills *out for the last WRACH_SOLVER_S frame.  Synchronises the stream. */
+int wrach_solver_stats(wrach_ctx *ctx, wrach_solver_report *out);
 
 /* 1 (default): frames are replayed from a captured hipGraph; 0: plain
  * stream launches. */

@@ -155,14 +155,10 @@ class WrachAPI {
     void set_solver(uint32_t mode, uint32_t max_passes = 0, float overlap_tol = 0.0f) {
         check(wrach_set_solver(ctx_, mode, max_passes, overlap_tol));
     }
-    struct SolverStats {
-        float overlap_before = 0.0f, overlap_after = 0.0f;
-        uint32_t passes = 0;
-    };
-    SolverStats solver_stats() {
-        SolverStats st;
-        check(wrach_solver_stats(ctx_, &st.overlap_before, &st.overlap_after, &st.passes));
-        return st;
+    wrach_solver_report solver_stats() {
+        wrach_solver_report r{};
+        check(wrach_solver_stats(ctx_, &r));
+        return r;
     }
     wrach_ctx *raw() { return ctx_; }
 

@@ -72,10 +72,11 @@ def lib():
         L.wo_step.restype = ctypes.c_int
         L.wo_step_mt.argtypes = [ctypes.POINTER(Params), _u32p, _f32p, _f32p, _f32p, ctypes.c_int]
         L.wo_step_mt.restype = ctypes.c_int
-        L.wo_stencil_pass.argtypes = [ctypes.POINTER(Params), _u32p, _f32p, _f32p, _f32p, ctypes.c_int]
+        L.wo_stencil_pass.argtypes = [ctypes.POINTER(Params), _u32p, _f32p, _f32p, _f32p, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint64)]
         L.wo_stencil_pass.restype = ctypes.c_float
         L.wo_step_s.argtypes = [ctypes.POINTER(Params), _u32p, _f32p, _f32p, _f32p, ctypes.c_uint32,
-                                ctypes.c_float, _f32p, _u32p]
+                                ctypes.c_float, _f32p, _u32p, ctypes.POINTER(ctypes.c_uint64)]
         L.wo_step_s.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -212,14 +213,27 @@ def stencil_pass(P: Params, idx, pcell, pin, move: bool = True):
     pin = np.ascontiguousarray(pin, np.float32)
     pout = np.zeros_like(pin)
     ov = lib().wo_stencil_pass(ctypes.byref(P), _p(idx, _u32p), _p(pcell, _f32p), _p(pin, _f32p),
-                               _p(pout, _f32p), 1 if move else 0)
+                               _p(pout, _f32p), 1 if move else 0, None)
     return (pout if move else None), float(ov)
+
+
+def stencil_overlap_sums(P: Params, idx, pcell, pin):
+    """(sum of the overlaps 1 - d in 2^-24 units, overlapping pairs) over
+    the pairs one pass visits, each pair counted from both particles."""
+    idx = np.ascontiguousarray(idx, np.uint32)
+    pcell = np.ascontiguousarray(pcell, np.float32)
+    pin = np.ascontiguousarray(pin, np.float32)
+    sums = np.zeros(2, np.uint64)
+    lib().wo_stencil_pass(ctypes.byref(P), _p(idx, _u32p), _p(pcell, _f32p), _p(pin, _f32p),
+                          None, 0, _p(sums, ctypes.POINTER(ctypes.c_uint64)))
+    return int(sums[0]), int(sums[1])
 
 
 def step_s(P: Params, idx, pos, vel, max_passes: int, overlap_tol: float = 0.0, nsteps: int = 1):
     """nsteps mode S frames: separation passes, then the reference step.
-    Returns (idx, pos, vel, stats of the last frame: overlap before and
-    after the passes and the passes run)."""
+    Returns (idx, pos, vel, stats of the last frame: largest overlap before
+    and after the passes, the passes run, and the overlap sums in 2^-24
+    units and pair counts before and after)."""
     idx = np.array(idx, np.uint32)
     pos = np.array(pos, np.float32)
     vel = np.array(vel, np.float32)
@@ -227,12 +241,13 @@ def step_s(P: Params, idx, pos, vel, max_passes: int, overlap_tol: float = 0.0, 
     scratch = np.zeros(4 * max(n, 1), np.float32)
     ov = np.zeros(2, np.float32)
     ran = np.zeros(1, np.uint32)
+    sums = np.zeros(4, np.uint64)
     stats = None
     for _ in range(nsteps):
         rc = lib().wo_step_s(ctypes.byref(P), _p(idx, _u32p), _p(pos, _f32p), _p(vel, _f32p),
                              _p(scratch, _f32p), int(max_passes), float(overlap_tol),
-                             _p(ov, _f32p), _p(ran, _u32p))
+                             _p(ov, _f32p), _p(ran, _u32p), _p(sums, ctypes.POINTER(ctypes.c_uint64)))
         if rc != 0:
             raise ValueError("a particle left the grid")
-        stats = (float(ov[0]), float(ov[1]), int(ran[0]))
+        stats = (float(ov[0]), float(ov[1]), int(ran[0]), tuple(int(x) for x in sums))
     return idx, pos, vel, stats

@@ -427,11 +427,12 @@ static int stencil_pair(float xi, float yi, float xj, float yj, float *ex, float
 }
 
 float wo_stencil_pass(const wo_params *p, const uint32_t *idx, const float *pcell,
-                      const float *pin, float *pout, int move)
+                      const float *pin, float *pout, int move, uint64_t sums[2])
 {
     uint64_t n = p->particles_in_frame_count;
     int64_t gx = p->grid_dimensions[0], gy = p->grid_dimensions[1];
     float worst = 0.0f;
+    if (sums) sums[0] = sums[1] = 0;
     for (uint64_t i = 0; i < n; ++i) {
         float xi = pin[2 * i], yi = pin[2 * i + 1];
         int64_t k = stencil_cell(p, pcell, i);
@@ -450,6 +451,10 @@ float wo_stencil_pass(const wo_params *p, const uint32_t *idx, const float *pcel
                         if (!stencil_pair(xi, yi, pin[2 * j], pin[2 * j + 1], &ex, &ey, &ov))
                             continue;
                         if (ov > worst) worst = ov;
+                        if (sums) {
+                            sums[0] += (uint64_t)llrintf(ov * WO_OVERLAP_SCALE);
+                            sums[1] += 1;
+                        }
                         pairs += 1;
                         ax -= llrintf(ex * WO_STENCIL_SCALE);
                         ay -= llrintf(ey * WO_STENCIL_SCALE);
@@ -472,20 +477,23 @@ float wo_stencil_pass(const wo_params *p, const uint32_t *idx, const float *pcel
 }
 
 int wo_step_s(const wo_params *p, uint32_t *idx, float *pos, float *vel, float *scratch,
-              uint32_t max_passes, float overlap_tol, float overlap[2], uint32_t *passes)
+              uint32_t max_passes, float overlap_tol, float overlap[2], uint32_t *passes,
+              uint64_t sums[4])
 {
     uint64_t n = p->particles_in_frame_count;
     float *a = (float *)malloc(sizeof(float) * 2 * (n ? n : 1));
     float *b = (float *)malloc(sizeof(float) * 2 * (n ? n : 1));
     memcpy(a, pos, sizeof(float) * 2 * n);
     uint32_t ran = 0;
-    float ov = wo_stencil_pass(p, idx, pos, a, NULL, 0);
+    float ov = wo_stencil_pass(p, idx, pos, a, NULL, 0, sums);
     overlap[0] = ov;
+    sums[2] = sums[0];
+    sums[3] = sums[1];
     for (uint32_t k = 0; k < max_passes; ++k) {
         if (!(ov > overlap_tol)) break;     /* the pass would start within tolerance */
-        wo_stencil_pass(p, idx, pos, a, b, 1);
+        wo_stencil_pass(p, idx, pos, a, b, 1, NULL);
         float *t = a; a = b; b = t;
-        ov = wo_stencil_pass(p, idx, pos, a, NULL, 0);
+        ov = wo_stencil_pass(p, idx, pos, a, NULL, 0, sums + 2);
         ran = k + 1;
     }
     overlap[1] = ov;

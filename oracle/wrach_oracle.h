@@ -158,18 +158,24 @@ int wo_step_mt(const wo_params *p, uint32_t *indices, float *pos, float *vel,
  * lies in the 3x3 cells, so one pass equals the all-pairs Jacobi pass. */
 #define WO_STENCIL_SCALE 4294967296.0f
 #define WO_STENCIL_BUDGET 4096
+/* Overlaps are summed as multiples of 1/WO_OVERLAP_SCALE (llrintf). */
+#define WO_OVERLAP_SCALE 16777216.0f
 
 /* One pass over the packed state: positions pin, frame-start positions
  * pcell (the cells), indices idx of the frame.  move: write pout.  Returns
- * the largest overlap 1 - d over the pairs it visits (0 if none). */
+ * the largest overlap 1 - d over the pairs it visits (0 if none); sums (if
+ * not NULL): the sum of those overlaps in 1/WO_OVERLAP_SCALE units and the
+ * number of pairs, each pair counted from both of its particles. */
 float wo_stencil_pass(const wo_params *p, const uint32_t *indices, const float *pcell,
-                      const float *pin, float *pout, int move);
+                      const float *pin, float *pout, int move, uint64_t sums[2]);
 
 /* A mode S frame: up to max_passes passes, each run only while the overlap
  * at its start exceeds overlap_tol, then wo_step.  overlap[0]: before the
- * passes, overlap[1]: after them; *passes: the passes that ran. */
+ * passes, overlap[1]: after them; *passes: the passes that ran; sums: the
+ * overlap sums and pair counts of wo_stencil_pass before and after. */
 int wo_step_s(const wo_params *p, uint32_t *indices, float *pos, float *vel, float *scratch,
-              uint32_t max_passes, float overlap_tol, float overlap[2], uint32_t *passes);
+              uint32_t max_passes, float overlap_tol, float overlap[2], uint32_t *passes,
+              uint64_t sums[4]);
 
 #ifdef __cplusplus
 }

@@ -123,7 +123,10 @@ static void stencil_solver() {
     CHECK(a.positions[1].first - a.positions[0].first == 1.0f);
     const auto st = a.solver_stats();
     CHECK(st.passes == 1);  // the second pass starts with no overlap left
-    CHECK(st.overlap_after == 0.0f && st.overlap_before > 0.39f && st.overlap_before < 0.41f);
+    CHECK(st.max_overlap_after == 0.0f && st.max_overlap_before > 0.39f && st.max_overlap_before < 0.41f);
+    CHECK(st.pairs_before == 2 && st.pairs_after == 2);  // the pair, from both sides, now touching
+    CHECK(st.overlap_sum_before == 2 * 6710888u);  // 1 - d = 0.4000001: 6,710,888 units of 2^-24, twice
+    CHECK(st.overlap_sum_after == 0);
     a.set_solver(WRACH_SOLVER_R);
     a.tick();
     CHECK(a.positions[1].first - a.positions[0].first == 1.0f);

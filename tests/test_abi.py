@@ -54,11 +54,13 @@ def test_struct_layouts_match_ctypes():
     #include <stddef.h>
     #include "wrach.h"
     int main(void) {
-      printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(wrach_config),
+      printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(wrach_config),
         offsetof(wrach_config, cell_size), offsetof(wrach_config, boundaries_as_dimensions),
         offsetof(wrach_config, gravity), offsetof(wrach_config, dt),
         offsetof(wrach_config, flags), offsetof(wrach_config, device),
-        sizeof(wrach_world_settings), sizeof(wrach_device_view), sizeof(wrach_tuning));
+        sizeof(wrach_world_settings), sizeof(wrach_device_view), sizeof(wrach_tuning),
+        sizeof(wrach_solver_report), offsetof(wrach_solver_report, overlap_sum_before),
+        offsetof(wrach_solver_report, pairs_after));
       return 0;
     }
     """
@@ -73,7 +75,8 @@ def test_struct_layouts_match_ctypes():
     want = [ctypes.sizeof(C), C.cell_size.offset, C.boundaries_as_dimensions.offset,
             C.gravity.offset, C.dt.offset, C.flags.offset, C.device.offset,
             ctypes.sizeof(_lib.WorldSettingsC), ctypes.sizeof(_lib.DeviceViewC),
-            ctypes.sizeof(_lib.TuningC)]
+            ctypes.sizeof(_lib.TuningC), ctypes.sizeof(_lib.SolverReportC),
+            _lib.SolverReportC.overlap_sum_before.offset, _lib.SolverReportC.pairs_after.offset]
     assert got == want
     assert got[7] == 32  # ShaderWorldSettings is 32 bytes (config_shader.rs:15-29)
 

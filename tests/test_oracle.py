@@ -334,6 +334,20 @@ def _brute_force_pass(pin):
     return pin.astype(np.float64) + disp, float(ov)
 
 
+def _brute_force_overlap_sums(pin):
+    """Sum of 1 - d in 2^-24 units (float32 product, round half to even)
+    and count, over ordered pairs i != j with d <= 1 (d == 0 as 0.0001)."""
+    x, y = pin[:, 0], pin[:, 1]
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    d = np.sqrt((dx * dx).astype(F) + (dy * dy).astype(F)).astype(F)
+    near = d <= F(1)
+    np.fill_diagonal(near, False)
+    d = np.where(d == 0, F(0.0001), d).astype(F)
+    units = np.rint(((F(1) - d).astype(F) * F(16777216.0)).astype(F)).astype(np.int64)
+    return int(units[near].sum()), int(near.sum())
+
+
 def test_stencil_pair_separates_to_one(oracle):
     """Two particles 0.6 apart in neighbouring cells: each moves 0.2 away
     from the other, so the pair ends exactly 1 apart, midpoint kept."""
@@ -357,6 +371,7 @@ def test_stencil_pass_equals_all_pairs_brute_force(oracle, cs, seed):
     ref, ref_ov = _brute_force_pass(p)
     assert np.abs(out - ref).max() < 2e-6
     assert ov == ref_ov
+    assert oracle.stencil_overlap_sums(P, idx, p, p) == _brute_force_overlap_sums(p)
     assert np.abs(out - p).max() > 0.01  # the world has overlaps to resolve
 
 
@@ -403,10 +418,11 @@ def test_step_s_passes_and_tolerance(oracle):
     overlap runs none; the frame after the passes is the reference step on
     the separated positions."""
     P, idx, p, v = _packed_world(oracle, 500, 20, 1, 6)
-    _, _, _, (before, after, ran) = oracle.step_s(P, idx, p, v, max_passes=4)
+    _, _, _, (before, after, ran, sums) = oracle.step_s(P, idx, p, v, max_passes=4)
     assert ran == 4 and after < before
+    assert sums[0] / sums[1] > sums[2] / sums[3]  # the mean overlap falls too
     _, _, _, st = oracle.step_s(P, idx, p, v, max_passes=4, overlap_tol=before)
-    assert st == (before, before, 0)
+    assert st[:3] == (before, before, 0) and st[3][:2] == st[3][2:]
     moved, _ = oracle.stencil_pass(P, idx, p, p)
     i2, p2, v2, st1 = oracle.step_s(P, idx, p, v, max_passes=1)
     r = oracle.step(P, idx, moved, v)

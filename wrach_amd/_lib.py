@@ -81,6 +81,19 @@ class WorldSettingsC(ctypes.Structure):
     ]
 
 
+class SolverReportC(ctypes.Structure):
+    _fields_ = [
+        ("max_overlap_before", ctypes.c_float),
+        ("max_overlap_after", ctypes.c_float),
+        ("passes", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("overlap_sum_before", ctypes.c_uint64),
+        ("pairs_before", ctypes.c_uint64),
+        ("overlap_sum_after", ctypes.c_uint64),
+        ("pairs_after", ctypes.c_uint64),
+    ]
+
+
 class TuningC(ctypes.Structure):
     _fields_ = [
         ("scan_threads", ctypes.c_uint32),
@@ -150,8 +163,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(TuningC)]
     L.wrach_set_graph.argtypes = [vp, ctypes.c_int]
     L.wrach_set_solver.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]
-    L.wrach_solver_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
-                                     ctypes.POINTER(ctypes.c_uint32)]
+    L.wrach_solver_stats.argtypes = [vp, ctypes.POINTER(SolverReportC)]
     L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.wrach_last_error.argtypes = []
     L.wrach_last_error.restype = ctypes.c_char_p

@@ -208,11 +208,14 @@ class World:
         check(self._L.wrach_set_solver(self._h, modes[mode], int(max_passes), float(overlap_tol)))
 
     def solver_stats(self):
-        """(overlap before the passes, overlap after them, passes run) of
-        the last mode S frame (wrach_solver_stats)."""
-        b, a, k = ctypes.c_float(), ctypes.c_float(), ctypes.c_uint32()
-        check(self._L.wrach_solver_stats(self._h, ctypes.byref(b), ctypes.byref(a), ctypes.byref(k)))
-        return b.value, a.value, k.value
+        """The last mode S frame (wrach_solver_stats): (largest overlap
+        before the passes, after them, passes run, (overlap sum in 2^-24
+        units, pairs) before + the same after).  The mean overlap is
+        sum / 2**24 / pairs."""
+        r = _lib.SolverReportC()
+        check(self._L.wrach_solver_stats(self._h, ctypes.byref(r)))
+        return (r.max_overlap_before, r.max_overlap_after, r.passes,
+                (r.overlap_sum_before, r.pairs_before, r.overlap_sum_after, r.pairs_after))
 
     def set_graph(self, enable: bool) -> None:
         check(self._L.wrach_set_graph(self._h, 1 if enable else 0))

@@ -106,6 +106,7 @@ struct wrach_ctx {
     float overlap_tol = 0.0f;
     uint32_t *scell = nullptr;   // frame-start cell of each particle (capacity entries)
     uint32_t *swords = nullptr;  // wrach::kStencilWords words
+    unsigned long long *ssums = nullptr;  // wrach::kStencilSumEntries entries
 };
 
 namespace {
@@ -126,11 +127,13 @@ int enqueue_stencil(wrach_ctx *c) {
     wrach::StencilArgs a{};
     a.cell = c->scell;
     a.words = c->swords;
+    a.sums = c->ssums;
     a.tol = c->overlap_tol;
-    WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->stream));
+    WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->ssums, c->stream));
     a.pin = c->buf.pos;
     a.first = 1;
     a.in_slot = 0;
+    a.stat = 1;  // the overlap sums before the passes
     WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
     a.first = 0;
     for (uint32_t k = 0; k < c->passes; ++k) {
@@ -143,6 +146,7 @@ int enqueue_stencil(wrach_ctx *c) {
         WRACH_HIP(wrach::launch_stencil_move(c->fp, c->buf, a, c->stream));
         a.pin = out;
         a.in_slot = k + 1;
+        a.stat = k + 1 == c->passes ? 2 : 0;  // the last measure: the sums after them
         WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
     }
     if (c->passes % 2) WRACH_HIP(wrach::launch_copy_pos(c->buf.pos, c->buf.pos_out, c->fp.n, c->stream));
@@ -203,6 +207,7 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.class_ctr);
     (void)hipFree(c->scell);
     (void)hipFree(c->swords);
+    (void)hipFree(c->ssums);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -736,7 +741,13 @@ int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float ove
         WRACH_HIP(dmalloc(&words, wrach::kStencilWords));
         c->swords = words;
     }
+    if (mode == WRACH_SOLVER_S && !c->ssums) {
+        unsigned long long *sums = nullptr;
+        WRACH_HIP(dmalloc(&sums, wrach::kStencilSumEntries));
+        c->ssums = sums;
+    }
     if (c->swords) WRACH_HIP(hipMemset(c->swords, 0, wrach::kStencilWords * sizeof(uint32_t)));
+    if (c->ssums) WRACH_HIP(hipMemset(c->ssums, 0, wrach::kStencilSumEntries * sizeof(unsigned long long)));
     free_graphs(c);  // captured with the previous solver
     c->solver = mode;
     c->passes = mode == WRACH_SOLVER_S ? max_passes : 0;
@@ -745,20 +756,33 @@ int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float ove
     return WRACH_OK;
 }
 
-int wrach_solver_stats(wrach_ctx *c, float *overlap_before, float *overlap_after, uint32_t *passes) {
-    if (!c) return fail(WRACH_E_INVALID, "null context");
+int wrach_solver_stats(wrach_ctx *c, wrach_solver_report *out) {
+    if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
     uint32_t w[wrach::kStencilWords] = {};
-    if (c->swords) {
+    std::vector<unsigned long long> sums(wrach::kStencilSumEntries, 0ull);
+    if (c->swords && c->ssums) {
         WRACH_HIP(hipSetDevice(c->cfg.device));
         WRACH_HIP(hipStreamSynchronize(c->stream));
         WRACH_HIP(hipMemcpy(w, c->swords, sizeof w, hipMemcpyDeviceToHost));
+        WRACH_HIP(hipMemcpy(sums.data(), c->ssums, sums.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost));
     }
-    float before, after;
-    std::memcpy(&before, &w[0], sizeof before);
-    std::memcpy(&after, &w[c->passes], sizeof after);
-    if (overlap_before) *overlap_before = before;
-    if (overlap_after) *overlap_after = after;
-    if (passes) *passes = w[wrach::kStencilPassesWord];
+    wrach_solver_report r{};
+    std::memcpy(&r.max_overlap_before, &w[0], sizeof r.max_overlap_before);
+    std::memcpy(&r.max_overlap_after, &w[c->passes], sizeof r.max_overlap_after);
+    r.passes = w[wrach::kStencilPassesWord];
+    uint64_t acc[4] = {0, 0, 0, 0};
+    for (uint32_t which = 0; which < 2; ++which)
+        for (uint32_t slot = 0; slot < wrach::kStencilSumSlots; ++slot) {
+            const size_t e = ((size_t)which * wrach::kStencilSumSlots + slot) * wrach::kStencilSumStride;
+            acc[2 * which] += sums[e];
+            acc[2 * which + 1] += sums[e + 1];
+        }
+    r.overlap_sum_before = acc[0];
+    r.pairs_before = acc[1];
+    r.overlap_sum_after = acc[2];
+    r.pairs_after = acc[3];
+    *out = r;
     return WRACH_OK;
 }
 

@@ -164,11 +164,22 @@ constexpr uint32_t kMaxStencilPasses = 64;
 constexpr uint32_t kStencilPassesWord = kMaxStencilPasses + 1;
 constexpr uint32_t kStencilWords = kMaxStencilPasses + 2;
 constexpr uint32_t kStencilBudget = 4096;  // most particles one lane visits in its 3x3 cells
+// Overlap sums (1 - d in 2^-24 units) and pair counts of the frame's first
+// and last measure, striped over kStencilSumSlots counters 128 bytes apart
+// (block b adds to slot b % kStencilSumSlots): entry
+// ((which * kStencilSumSlots + slot) * kStencilSumStride) holds the sum,
+// the next one the count.
+constexpr uint32_t kStencilSumSlots = 64;
+constexpr uint32_t kStencilSumStride = 16;  // u64 entries between slots
+constexpr uint32_t kStencilSumEntries = 2 * kStencilSumSlots * kStencilSumStride;
+constexpr float kOverlapScale = 16777216.0f;  // 2^24
 struct StencilArgs {
     const float2 *pin;  // positions entering the kernel
     float2 *pout;       // a move's output
     uint32_t *cell;     // each particle's cell at the frame's start
     uint32_t *words;    // kStencilWords words
+    unsigned long long *sums;  // kStencilSumEntries entries
+    uint32_t stat;      // a measure: 0, or 1 / 2 to add its overlaps to the first / last sums
     uint32_t in_slot;   // a measure: the word it raises; a move: the word of its start
     uint32_t first;     // a measure of the frame-start positions (writes `cell`)
     uint32_t pass;      // a move's number, from 0
@@ -178,7 +189,7 @@ hipError_t launch_stencil_measure(const FrameParams &p, const FrameBuffers &b, c
                                   hipStream_t stream);
 hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
                                hipStream_t stream);
-hipError_t launch_stencil_clear(uint32_t *words, hipStream_t stream);
+hipError_t launch_stencil_clear(uint32_t *words, unsigned long long *sums, hipStream_t stream);
 hipError_t launch_copy_pos(float2 *dst, const float2 *src, uint32_t n, hipStream_t stream);
 
 }  // namespace wrach

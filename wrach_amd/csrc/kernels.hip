@@ -1754,14 +1754,16 @@ __device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float
 }
 
 // Particle i (in cell (cx, cy) at the frame's start) against the 3x3 cells
-// around it: the largest overlap and, with MOVE, the sum of its pair terms
-// in 2^-32 units and their number.  Over kStencilBudget particles in those
-// cells: neither (a bound on the work of one lane).
+// around it: the largest overlap, the number of overlapping pairs, and with
+// MOVE the sum of its pair terms in 2^-32 units, without it the sum of the
+// overlaps in 2^-24 units.  Over kStencilBudget particles in those cells:
+// none of these (a bound on the work of one lane).
 template <bool MOVE>
 __device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
                                                   const float2 *pin, uint32_t i, uint32_t k,
                                                   float xi, float yi, long long &ax,
-                                                  long long &ay, uint32_t &pairs) {
+                                                  long long &ay, uint32_t &pairs,
+                                                  unsigned long long &osum) {
     uint32_t rs[3], re[3];
     stencil_rows(p, b, (int)(k % p.gx), (int)(k / p.gx), rs, re);
     const uint64_t total = (uint64_t)(re[0] - rs[0]) + (re[1] - rs[1]) + (re[2] - rs[2]);
@@ -1774,10 +1776,12 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
             float ex, ey, ov;
             if (!stencil_pair(xi, yi, q.x, q.y, ex, ey, ov)) continue;
             worst = fmaxf(worst, ov);
+            pairs += 1;
             if (MOVE) {
-                pairs += 1;
                 ax -= __float2ll_rn(ex * kStencilScale);
                 ay -= __float2ll_rn(ey * kStencilScale);
+            } else {
+                osum += (unsigned long long)__float2ll_rn(ov * kOverlapScale);
             }
         }
     }
@@ -1790,17 +1794,36 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// Raises *w to the block's largest v.  One atomic per block, and none when
-// the word already holds as much (every block hitting one word serialises).
-__device__ __forceinline__ void block_raise_max(uint32_t *w, float v) {
+// Raises *w to the block's largest v -- one atomic per block, and none when
+// the word already holds as much (every block hitting one word serialises)
+// -- and, with sums, adds the block's overlap sum and pair count to its
+// slot (blockIdx % kStencilSumSlots) of them.
+__device__ __forceinline__ void block_report(uint32_t *w, float v, unsigned long long *sums,
+                                             unsigned long long osum, unsigned long long ocnt) {
     __shared__ float s_max[4];
+    __shared__ unsigned long long s_sum[4], s_cnt[4];
     v = wave_max(v);
-    if (lane_id() == 0) s_max[threadIdx.x >> 6] = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        osum += __shfl_xor(osum, o, 64);
+        ocnt += __shfl_xor(ocnt, o, 64);
+    }
+    if (lane_id() == 0) {
+        s_max[threadIdx.x >> 6] = v;
+        s_sum[threadIdx.x >> 6] = osum;
+        s_cnt[threadIdx.x >> 6] = ocnt;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
         if (m > 0.0f && __float_as_uint(m) > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(w, __float_as_uint(m));
+        const unsigned long long cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        if (sums && cnt) {
+            unsigned long long *slot = sums + (blockIdx.x % kStencilSumSlots) * kStencilSumStride;
+            atomicAdd(slot, s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3]);
+            atomicAdd(slot + 1, cnt);
+        }
     }
 }
 
@@ -1811,6 +1834,8 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
                                                               StencilArgs a) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     float worst = 0.0f;
+    uint32_t pairs = 0;
+    unsigned long long osum = 0;
     if (i < p.n) {
         const float2 q = a.pin[i];
         uint32_t k;
@@ -1821,10 +1846,11 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
             k = a.cell[i];
         }
         long long ax = 0, ay = 0;
-        uint32_t pairs = 0;
-        if (k < p.ncell) worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs);
+        if (k < p.ncell) worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
     }
-    block_raise_max(&a.words[a.in_slot], worst);
+    unsigned long long *sums =
+        a.stat ? a.sums + (a.stat - 1) * kStencilSumSlots * kStencilSumStride : nullptr;
+    block_report(&a.words[a.in_slot], worst, sums, osum, pairs);
 }
 
 // Pass a.pass: a.pin -> a.pout, run only while the overlap at its start
@@ -1842,8 +1868,9 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
     if (i == 0) a.words[kStencilPassesWord] = a.pass + 1;
     long long ax = 0, ay = 0;
     uint32_t pairs = 0;
+    unsigned long long osum = 0;
     const uint32_t k = a.cell[i];
-    if (k < p.ncell) (void)stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs);
+    if (k < p.ncell) (void)stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
     // The mean of the pair corrections (constraint averaging): a particle
     // moves at most 0.5 per pass.
     float dx = __ll2float_rn(ax) * kStencilUnit;
@@ -1855,8 +1882,12 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
     a.pout[i] = make_float2(q.x + dx, q.y + dy);
 }
 
-__global__ __launch_bounds__(64) void stencil_clear_kernel(uint32_t *w) {
-    for (uint32_t i = threadIdx.x; i < kStencilWords; i += 64) w[i] = 0;
+__global__ __launch_bounds__(256) void stencil_clear_kernel(uint32_t *w, unsigned long long *sums) {
+    for (uint32_t i = threadIdx.x; i < kStencilWords; i += 256) w[i] = 0;
+    for (uint32_t i = threadIdx.x; i < 2 * kStencilSumSlots; i += 256) {
+        sums[i * kStencilSumStride] = 0;
+        sums[i * kStencilSumStride + 1] = 0;
+    }
 }
 
 __global__ __launch_bounds__(256) void copy_pos_kernel(float2 *dst, const float2 *src, uint32_t n) {
@@ -1994,8 +2025,8 @@ hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, cons
     return hipGetLastError();
 }
 
-hipError_t launch_stencil_clear(uint32_t *words, hipStream_t stream) {
-    hipLaunchKernelGGL(stencil_clear_kernel, dim3(1), dim3(64), 0, stream, words);
+hipError_t launch_stencil_clear(uint32_t *words, unsigned long long *sums, hipStream_t stream) {
+    hipLaunchKernelGGL(stencil_clear_kernel, dim3(1), dim3(256), 0, stream, words, sums);
     return hipGetLastError();
 }
 
