@@ -226,6 +226,20 @@ int wrach_set_tuning(wrach_ctx *ctx, const wrach_tuning *tuning);
  * world evolves).  *out (may be NULL) receives it.  Synchronises. */
 int wrach_autotune(wrach_ctx *ctx, uint32_t frames, wrach_tuning *out);
 
+/* Moves the view (the simulated window) to anchor (x, y), its size kept:
+ * the active window over the full store (runners/bevy/src/
+ * particle_store.rs:19-26, SpatialBin::get_active_cells, spatial_bin.rs:
+ * 68-95; the reference keeps the anchor at (0, 0), builder.rs:61).  The
+ * simulated particles are written back to the store (wrach_store_write_back),
+ * which keeps the particles outside the view in their store order, and the
+ * store's particles inside the new view are packed as wrach_add_particles
+ * packs them; walls and cell keys then follow the new view.  The anchor
+ * must be a multiple of the cell size, non-negative, with x + width and
+ * y + height at most 2^24 (WRACH_E_INVALID otherwise); not for a batch or a
+ * host-packed store.  WRACH_E_CAPACITY when the new view holds more
+ * particles than the capacity: the view then stays where it was. */
+int wrach_set_view_anchor(wrach_ctx *ctx, float x, float y);
+
 /* Solvers.  WRACH_SOLVER_R (the default) is the reference's step:
  * collisions within one cell (shaders/physics/src/cell.rs:52-75).
  * WRACH_SOLVER_S is an extension with no counterpart in the reference: each

@@ -50,6 +50,7 @@ EXPORTS = (
     "wrach_set_tuning",
     "wrach_autotune",
     "wrach_set_graph",
+    "wrach_set_view_anchor",
     "wrach_set_solver",
     "wrach_solver_stats",
     "wrach_step_profiled",
@@ -163,6 +164,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(TuningC)]
     L.wrach_set_graph.argtypes = [vp, ctypes.c_int]
     L.wrach_set_solver.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]
+    L.wrach_set_view_anchor.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     L.wrach_solver_stats.argtypes = [vp, ctypes.POINTER(SolverReportC)]
     L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.wrach_last_error.argtypes = []

@@ -198,6 +198,10 @@ class World:
     def sync(self) -> None:
         check(self._L.wrach_sync(self._h))
 
+    def set_view_anchor(self, x: float, y: float) -> None:
+        """wrach_set_view_anchor: move the simulated window over the store."""
+        check(self._L.wrach_set_view_anchor(self._h, float(x), float(y)))
+
     def set_solver(self, mode: str = "R", max_passes: int = 0, overlap_tol: float = 0.0) -> None:
         """wrach_set_solver: "R", the reference's step, or "S", up to
         max_passes separation passes over the 3x3 cells around each particle

@@ -97,6 +97,7 @@ struct wrach_ctx {
     uint32_t *dstore_world = nullptr;  // world of each store particle (a batch)
     uint32_t worlds = 1;               // worlds of a batch (one grid each, keys one after another)
     uint64_t world_cells = 0;          // cells of one world
+    int32_t blx = 0, bly = 0;  // the grid's bottom-left cell: view_anchor / cell_size
     uint32_t cells_grid0 = 1;  // the cells kernel's grid chosen at creation
     uint64_t chunks = 0;       // chunks of kChunkCells cells
     // Solver: WRACH_SOLVER_R, or WRACH_SOLVER_S with up to `passes`
@@ -381,6 +382,8 @@ wrach::UploadParams store_params(const wrach_ctx *c, size_t n) {
     u.cap = c->capacity;
     u.world = c->worlds > 1 ? c->dstore_world : nullptr;
     u.world_cells = (uint32_t)c->world_cells;
+    u.blx = c->blx;
+    u.bly = c->bly;
     return u;
 }
 
@@ -886,11 +889,31 @@ int wrach_store_write_back(wrach_ctx *c) {
             e = hipMemcpyAsync(v, view.velocities, (size_t)n * sizeof(float2), hipMemcpyDeviceToDevice,
                                c->stream);
     }
-    if (e == hipSuccess) e = hipMemsetAsync(c->dvalid, 0, sizeof(uint32_t), c->stream);
-    if (e == hipSuccess) e = wrach::launch_store_outside(store_params(c, c->dstore_n), p, v, n, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(&outside, c->dvalid, sizeof outside, hipMemcpyDeviceToHost, c->stream);
+    // The old store's particles outside the grid, in store order: per-block
+    // counts, their exclusive offsets on the host, then the copy.
+    const size_t blocks = (c->dstore_n + 255) / 256;
+    uint32_t *dcounts = nullptr;
+    std::vector<uint32_t> counts(blocks);
+    if (blocks) {
+        if (e == hipSuccess) e = hipMalloc(&dcounts, blocks * sizeof(uint32_t));
+        if (e == hipSuccess) e = wrach::launch_store_outside_count(store_params(c, c->dstore_n), dcounts, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(counts.data(), dcounts, blocks * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) {
+            for (size_t b = 0; b < blocks; ++b) {
+                const uint32_t m = counts[b];
+                counts[b] = outside;
+                outside += m;
+            }
+            e = hipMemcpyAsync(dcounts, counts.data(), blocks * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               c->stream);
+        }
+        if (e == hipSuccess) e = wrach::launch_store_outside(store_params(c, c->dstore_n), p, v, n, dcounts, c->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dcounts);
     if (e != hipSuccess) {
         (void)hipFree(p);
         (void)hipFree(v);
@@ -899,6 +922,56 @@ int wrach_store_write_back(wrach_ctx *c) {
     }
     swap_store(c, p, v, k, cap);
     c->dstore_n = (size_t)n + outside;
+    g_err.clear();
+    return WRACH_OK;
+}
+
+namespace {
+
+void place_view(wrach_ctx *c, float x, float y) {
+    const float cs = (float)c->cfg.cell_size;
+    c->settings.view_anchor[0] = x;
+    c->settings.view_anchor[1] = y;
+    c->fp.ax = x;
+    c->fp.ay = y;
+    c->fp.x1 = x + c->settings.view_dimensions[0];
+    c->fp.y1 = y + c->settings.view_dimensions[1];
+    c->blx = (int32_t)(x / cs);
+    c->bly = (int32_t)(y / cs);
+}
+
+}  // namespace
+
+int wrach_set_view_anchor(wrach_ctx *c, float x, float y) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (c->worlds > 1) return fail(WRACH_E_INVALID, "the worlds of a batch keep their views at the origin");
+    if (c->host_pack) return fail(WRACH_E_INVALID, "moving the view needs the device store (no WRACH_FLAG_HOST_PACK)");
+    const float cs = (float)c->cfg.cell_size;
+    const float w = c->settings.view_dimensions[0], h = c->settings.view_dimensions[1];
+    // Integers below 2^24 keep x + W and every x - anchor exact in f32.
+    if (!(x >= 0.0f) || !(y >= 0.0f) || !(x + w <= 16777216.0f) || !(y + h <= 16777216.0f))
+        return fail(WRACH_E_INVALID, "the view must lie inside [0, 2^24] on both axes");
+    if (std::fmod(x, cs) != 0.0f || std::fmod(y, cs) != 0.0f)
+        return fail(WRACH_E_INVALID, "the view anchor must be a multiple of the cell size");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    const float ox = c->settings.view_anchor[0], oy = c->settings.view_anchor[1];
+    if (x == ox && y == oy) return WRACH_OK;
+    // The store takes the simulated particles and keeps those outside the
+    // view in their order; the new view's particles are packed from it.
+    int rc = wrach_store_write_back(c);
+    if (rc) return rc;
+    place_view(c, x, y);
+    free_graphs(c);  // the captured frames hold the old walls
+    c->canonical_valid = false;
+    rc = add_device_packed(c, nullptr, nullptr, 0, 0);
+    if (rc) {
+        // Too many particles for the new view: the packed state is still the
+        // old view's, which the written-back store matches.
+        const std::string err = wrach_last_error();
+        place_view(c, ox, oy);
+        return fail(rc, err);
+    }
     g_err.clear();
     return WRACH_OK;
 }

@@ -144,14 +144,18 @@ struct UploadParams {
     uint32_t cap;       // packed-array capacity
     const uint32_t *world;  // a batch: world of each store particle (else null)
     uint32_t world_cells;   // cells of one world (a world's keys start at world * world_cells)
+    int32_t blx, bly;       // the grid's bottom-left cell: the view anchor / cell size
 };
 // Cell and slot of every store particle, counts into b.cnt, inside-grid
 // total into u.valid (zeroed by the caller).
 hipError_t launch_upload_bin(const UploadParams &u, const FrameBuffers &b, hipStream_t stream);
-// The store's particles outside the grid, copied to dpos/dvel from index
-// base on, their number counted into u.valid (zeroed by the caller).
+// The store's particles outside the grid in store order, copied to
+// dpos/dvel from index base on: counts[b] receives block b's number (blocks
+// of 256 store particles), and after the caller has turned them into
+// exclusive offsets, launch_store_outside writes them.
+hipError_t launch_store_outside_count(const UploadParams &u, uint32_t *counts, hipStream_t stream);
 hipError_t launch_store_outside(const UploadParams &u, float2 *dpos, float2 *dvel, uint32_t base,
-                                hipStream_t stream);
+                                const uint32_t *offsets, hipStream_t stream);
 // After launch_scan: copy each particle to idx[key + 1] + slot, marked as a
 // run of its own (src = store index | run flag).
 hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, hipStream_t stream);

@@ -1626,9 +1626,10 @@ __device__ __forceinline__ float div_euclid_pos(float a, float b) {
 // The packer's cell of a store particle (spatial_bin.rs:48-64); false
 // outside the grid.
 __device__ __forceinline__ bool store_cell(const UploadParams &u, float2 q, uint32_t &key) {
-    const int32_t cx = saturating_i32(div_euclid_pos(q.x, u.cs));
-    const int32_t cy = saturating_i32(div_euclid_pos(q.y, u.cs));
-    const bool in = cx >= 0 && cy >= 0 && (uint32_t)cx < u.gx && (uint32_t)cy < u.gy;
+    // The cell relative to the grid's bottom-left cell (the view anchor's).
+    const int64_t cx = (int64_t)saturating_i32(div_euclid_pos(q.x, u.cs)) - u.blx;
+    const int64_t cy = (int64_t)saturating_i32(div_euclid_pos(q.y, u.cs)) - u.bly;
+    const bool in = cx >= 0 && cy >= 0 && cx < (int64_t)u.gx && cy < (int64_t)u.gy;
     key = in ? (uint32_t)cy * u.gx + (uint32_t)cx : 0xffffffffu;
     return in;
 }
@@ -1650,10 +1651,29 @@ __global__ __launch_bounds__(256) void upload_bin_kernel(UploadParams u, FrameBu
 }
 
 // The store's particles outside the grid, appended at dpos/dvel[base + ...]
-// (wrach_store_write_back), one atomic on u.valid per wavefront.  Their
-// order among themselves is free: they never enter a cell.
+// (wrach_store_write_back) in their store order: a moved view can bring
+// them into cells later, where that order is their in-cell order.  Two
+// launches: each block counts its own (counts[block]); the host turns the
+// counts into exclusive offsets; each block then writes its particles from
+// its offset on, ranked within the block by ballots.
+__global__ __launch_bounds__(256) void store_outside_count_kernel(UploadParams u, uint32_t *counts) {
+    __shared__ uint32_t s_n[4];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool out = false;
+    if (i < u.n) {
+        uint32_t key;
+        out = !store_cell(u, u.pos[i], key);
+    }
+    const unsigned long long m = __ballot(out);
+    if (lane_id() == 0) s_n[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+}
+
 __global__ __launch_bounds__(256) void store_outside_kernel(UploadParams u, float2 *dpos,
-                                                            float2 *dvel, uint32_t base) {
+                                                            float2 *dvel, uint32_t base,
+                                                            const uint32_t *offsets) {
+    __shared__ uint32_t s_n[4];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool out = false;
     float2 q = make_float2(0.0f, 0.0f);
@@ -1663,11 +1683,13 @@ __global__ __launch_bounds__(256) void store_outside_kernel(UploadParams u, floa
         out = !store_cell(u, q, key);
     }
     const unsigned long long m = __ballot(out);
-    uint32_t at = 0;
-    if (lane_id() == 0 && m != 0) at = atomicAdd(u.valid, (uint32_t)__popcll(m));
-    at = (uint32_t)__shfl((int)at, 0, 64);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) s_n[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += s_n[w];
     if (out) {
-        const uint32_t j = base + at + (uint32_t)__popcll(m & lanemask_lt());
+        const uint32_t j = base + offsets[blockIdx.x] + before + (uint32_t)__popcll(m & lanemask_lt());
         dpos[j] = q;
         dvel[j] = u.vel[i];
     }
@@ -1997,11 +2019,18 @@ hipError_t launch_upload_bin(const UploadParams &u, const FrameBuffers &b, hipSt
     return hipGetLastError();
 }
 
+hipError_t launch_store_outside_count(const UploadParams &u, uint32_t *counts, hipStream_t stream) {
+    if (u.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(store_outside_count_kernel, dim3((u.n + 255) / 256), dim3(256), 0, stream, u,
+                       counts);
+    return hipGetLastError();
+}
+
 hipError_t launch_store_outside(const UploadParams &u, float2 *dpos, float2 *dvel, uint32_t base,
-                                hipStream_t stream) {
+                                const uint32_t *offsets, hipStream_t stream) {
     if (u.n == 0) return hipSuccess;
     hipLaunchKernelGGL(store_outside_kernel, dim3((u.n + 255) / 256), dim3(256), 0, stream, u, dpos,
-                       dvel, base);
+                       dvel, base, offsets);
     return hipGetLastError();
 }
 
