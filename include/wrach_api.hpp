@@ -149,6 +149,12 @@ class WrachAPI {
                                 reinterpret_cast<const float *>(d.velocities.data()), n));
         check(wrach_settings(ctx_, &state_.shader_settings));
     }
+    // The active window over the store (particle_store.rs:19-26): move the
+    // view; the reference keeps it at the origin (builder.rs:61).
+    void set_view_anchor(float x, float y) {
+        check(wrach_set_view_anchor(ctx_, x, y));
+        check(wrach_settings(ctx_, &state_.shader_settings));
+    }
     // Not in the reference: the solver (WRACH_SOLVER_R, or WRACH_SOLVER_S with
     // separation passes over the 3x3 cells before the reference step) and
     // the last mode S frame's overlap before and after its passes.
