@@ -156,8 +156,9 @@ int wrach_canonical_device(wrach_ctx *ctx, wrach_device_view *out);
 /* ParticleStore::update_from_gpu, a TODO in the reference
  * (particle_store.rs:74-85, with cells_to_read_from_gpu, :22-26): the store
  * becomes the simulated state -- the frame's particles in canonical packed
- * order, plus the store's particles outside the grid, which are never
- * simulated -- so that a later wrach_add_particles continues from the
+ * order, then the store's particles outside the grid in their store order
+ * (they are not simulated until a moved view covers them) -- so that a
+ * later wrach_add_particles continues from the
  * simulated state instead of restarting from the uploads.  The device state
  * is unchanged.  Synchronises. */
 int wrach_store_write_back(wrach_ctx *ctx);
