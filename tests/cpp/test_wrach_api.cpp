@@ -132,6 +132,32 @@ static void stencil_solver() {
     CHECK(a.positions[1].first - a.positions[0].first == 1.0f);
 }
 
+// The movable view through the mirror: two particles in the first view, three
+// beyond it; moving the view packs the three (and keeps the two in the
+// store), moving back packs the two again.
+static void movable_view() {
+    wrach::WrachConfig cfg;
+    cfg.dimensions = {16, 16};
+    cfg.cell_size = 1;
+    wrach::WrachAPI a(cfg);
+    a.add_particles({{{3.5f, 3.5f}, {0.0f, 0.0f}}, {{9.5f, 2.5f}, {0.0f, 0.0f}},
+                     {{20.5f, 4.5f}, {0.0f, 0.0f}}, {{25.5f, 9.5f}, {0.0f, 0.0f}},
+                     {{30.5f, 1.5f}, {0.0f, 0.0f}}});
+    a.tick();
+    CHECK(a.get_simulation_state().shader_settings.particles_in_frame_count == 2);
+    a.set_view_anchor(16.0f, 0.0f);
+    CHECK(a.get_simulation_state().shader_settings.particles_in_frame_count == 3);
+    CHECK(a.get_simulation_state().shader_settings.view_anchor[0] == 16.0f);
+    a.tick();
+    // row-major cells of the new view: rows 1, 4 and 9
+    CHECK(a.positions[0] == std::make_pair(30.5f, 1.5f) && a.positions[1] == std::make_pair(20.5f, 4.5f) &&
+          a.positions[2] == std::make_pair(25.5f, 9.5f));
+    a.set_view_anchor(0.0f, 0.0f);
+    a.tick();
+    CHECK(a.get_simulation_state().shader_settings.particles_in_frame_count == 2);
+    CHECK(a.positions[0] == std::make_pair(9.5f, 2.5f) && a.positions[1] == std::make_pair(3.5f, 3.5f));
+}
+
 int main() {
     api_returns_data();
     prefix_sum_for_small_arrays();
@@ -140,6 +166,7 @@ int main() {
     checkpoint_and_write_back(false);
     checkpoint_and_write_back(true);
     stencil_solver();
+    movable_view();
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);
     return failures ? 1 : 0;
 }
