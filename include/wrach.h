@@ -260,9 +260,10 @@ int wrach_set_view_anchor(wrach_ctx *ctx, float x, float y);
 #define WRACH_STENCIL_BUDGET 4096u
 
 /* Selects the solver for the following frames.  WRACH_E_INVALID for an
- * unknown mode, max_passes > WRACH_MAX_PASSES, a negative, infinite or NaN
- * tolerance, or WRACH_SOLVER_S on a batch of worlds.  max_passes = 0 makes
- * WRACH_SOLVER_S the reference's step. */
+ * unknown mode, max_passes > WRACH_MAX_PASSES, or a negative, infinite or
+ * NaN tolerance.  max_passes = 0 makes WRACH_SOLVER_S the reference's step.
+ * In a batch every world's particles meet only their own world's cells; the
+ * tolerance and the statistics cover the whole batch. */
 int wrach_set_solver(wrach_ctx *ctx, uint32_t mode, uint32_t max_passes, float overlap_tol);
 
 /* The last WRACH_SOLVER_S frame's overlaps (1 - d over the pairs its passes

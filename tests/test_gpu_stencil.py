@@ -12,7 +12,7 @@ import pytest
 
 from wrach_amd import World, WrachConfig, WrachError, workloads
 from wrach_amd import _lib
-from tests.test_gpu_parity import assert_same_state, f32, oracle_world, say
+from tests.test_gpu_parity import _batch_worlds, assert_same_state, f32, oracle_world, say
 
 pytestmark = pytest.mark.gpu
 
@@ -140,11 +140,6 @@ def test_stencil_argument_errors(oracle):
         w.set_solver("S", 1)
         w.step(1)
         assert w.solver_stats()[2] == 1
-    with World(cfg, worlds=2) as b:
-        with pytest.raises(WrachError) as e:
-            b.set_solver("S", 1)
-        assert e.value.code == _lib.WRACH_E_INVALID
-        b.set_solver("R")
 
 
 def test_stencil_lattice_1m_full_size(oracle):
@@ -176,3 +171,32 @@ def test_stencil_with_autotune_and_profiled_frames(oracle):
         w.step(5)
         *ref, _ = oracle.step_s(P, *ref, max_passes=2, nsteps=5)
         assert_same_state(w.read(), tuple(ref), "graph frames after")
+
+
+@pytest.mark.parametrize("cs", [1, 3])
+def test_stencil_batch_matches_each_world_alone(oracle, cs):
+    """Mode S on a batch: every world's particles meet only their own
+    world's cells, so each world matches the single-world oracle bit for bit
+    (with no tolerance, a world without overlaps is unchanged by the passes
+    that the others run); the batch's statistics are the worlds' combined."""
+    W, H = 48, 32
+    say(f"[stencil] batch of 5 worlds {W}x{H} cs={cs}, 2 passes")
+    cfg = WrachConfig(dimensions=(W, H), cell_size=cs, gravity=(0.0, -0.1))
+    worlds = _batch_worlds(cs, W, H, 70 + cs)
+    with World(cfg, worlds=len(worlds)) as b:
+        for w, (p, v) in enumerate(worlds):
+            b.add_particles(p, v, world=w)
+        b.set_solver("S", 2)
+        refs = [oracle_world(oracle, cfg, p, v) for p, v in worlds]
+        for step in range(3):
+            b.step(3)
+            stats = []
+            for w, (P, ref) in enumerate(refs):
+                *ref, st = oracle.step_s(P, *ref, max_passes=2, nsteps=3)
+                refs[w] = (P, tuple(ref))
+                stats.append(st)
+                assert_same_state(b.read_world(w), tuple(ref), f"world {w}, frame {3 * (step + 1)}")
+            before, after, ran, sums = b.solver_stats()
+            assert before == max(st[0] for st in stats) and after == max(st[1] for st in stats)
+            assert sums == tuple(sum(st[3][j] for st in stats) for j in range(4))
+            assert ran <= 2 and (ran > 0) == (before > 0)

@@ -728,7 +728,6 @@ int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float ove
             return fail(WRACH_E_INVALID, "more than " + std::to_string(wrach::kMaxStencilPasses) + " passes");
         if (!(overlap_tol >= 0.0f) || std::isinf(overlap_tol))
             return fail(WRACH_E_INVALID, "overlap tolerance must be finite and >= 0");
-        if (c->worlds > 1) return fail(WRACH_E_INVALID, "the stencil solver steps one world, not a batch");
     }
     WRACH_HIP(hipSetDevice(c->cfg.device));
     WRACH_HIP(hipStreamSynchronize(c->stream));

@@ -1736,19 +1736,35 @@ __device__ __forceinline__ uint32_t cell_start(const FrameParams &p, const Frame
 // layout, and [start(x0), start(x1 + 1)) covers columns x0..x1 (past the
 // row's last column, start(x1 + 1) is the next row's first cell, which
 // begins where the row ends).
-__device__ __forceinline__ void stencil_rows(const FrameParams &p, const FrameBuffers &b, int cx,
-                                             int cy, uint32_t rs[3], uint32_t re[3]) {
+__device__ __forceinline__ void stencil_rows(const FrameParams &p, const FrameBuffers &b,
+                                             uint32_t base, int cx, int cy, uint32_t rs[3],
+                                             uint32_t re[3]) {
     const int gx = (int)p.gx, gy = (int)p.gy;
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gx ? cx + 1 : gx - 1;
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         const int y = cy + t - 1;
         rs[t] = re[t] = 0;
-        if (y < 0 || y >= gy) continue;
-        rs[t] = cell_start(p, b, (uint32_t)(y * gx + x0));
-        re[t] = cell_start(p, b, (uint32_t)(y * gx + x1 + 1));
+        if (y < 0 || y >= gy) continue;  // the world's own rows only
+        rs[t] = cell_start(p, b, base + (uint32_t)(y * gx + x0));
+        re[t] = cell_start(p, b, base + (uint32_t)(y * gx + x1 + 1));
         if (re[t] < rs[t]) re[t] = rs[t];
     }
+}
+
+// First cell of the world that holds packed particle i: the last world whose
+// first cell starts at or before i (an empty world starts where the next
+// one does, so it is never the last such).  One world: 0.
+__device__ __forceinline__ uint32_t particle_world_base(const FrameParams &p, const FrameBuffers &b,
+                                                        uint32_t i) {
+    if (p.world_cells == p.ncell) return 0u;
+    uint32_t lo = 0, hi = p.ncell / p.world_cells;  // world lo starts at or before i
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (cell_start(p, b, mid * p.world_cells) <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo * p.world_cells;
 }
 
 // collide_pair's rule for particle i alone: whether j is within reach, the
@@ -1787,7 +1803,8 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
                                                   long long &ay, uint32_t &pairs,
                                                   unsigned long long &osum) {
     uint32_t rs[3], re[3];
-    stencil_rows(p, b, (int)(k % p.gx), (int)(k / p.gx), rs, re);
+    const uint32_t base = world_base(p, k), local = k - base;
+    stencil_rows(p, b, base, (int)(local % p.gx), (int)(local / p.gx), rs, re);
     const uint64_t total = (uint64_t)(re[0] - rs[0]) + (re[1] - rs[1]) + (re[2] - rs[2]);
     float worst = 0.0f;
     if (total > kStencilBudget) return worst;
@@ -1862,7 +1879,7 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
         const float2 q = a.pin[i];
         uint32_t k;
         if (a.first) {
-            k = cell_key(p, q.x, q.y, 0u);
+            k = cell_key(p, q.x, q.y, particle_world_base(p, b, i));
             a.cell[i] = k;
         } else {
             k = a.cell[i];
