@@ -1820,7 +1820,9 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
                 ax -= __float2ll_rn(ex * kStencilScale);
                 ay -= __float2ll_rn(ey * kStencilScale);
             } else {
-                osum += (unsigned long long)__float2ll_rn(ov * kOverlapScale);
+                // ov * 2^24 < 2^24: rintf (round half to even, as llrintf)
+                // is integral and converts exactly through 32 bits.
+                osum += (uint32_t)rintf(ov * kOverlapScale);
             }
         }
     }
