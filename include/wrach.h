@@ -249,7 +249,7 @@ int wrach_set_view_anchor(wrach_ctx *ctx, float x, float y);
  * particle i moves by the mean, over the other particles j in the 3x3 cells
  * around its frame-start cell with d <= 1, of the reference's pair
  * displacement -(p_j - p_i) * (0.5 * (1 - d)) / d (particles.rs:65-92;
- * d == 0 counts as 0.0001).  The terms are rounded to multiples of 2^-32 and
+ * d == 0 counts as 0.0001).  The terms are rounded to multiples of 2^-30 and
  * summed exactly, so the result does not depend on the order of particles
  * in a cell.  A pass runs only while the largest overlap 1 - d at its start
  * exceeds overlap_tol.  A particle whose 3x3 cells hold more than

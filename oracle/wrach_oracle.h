@@ -156,7 +156,7 @@ int wo_step_mt(const wo_params *p, uint32_t *indices, float *pos, float *vel,
  * (itself included) takes no displacement and measures no overlap: a bound
  * on the work per particle.  With cell sides >= 1 every pair closer than 1
  * lies in the 3x3 cells, so one pass equals the all-pairs Jacobi pass. */
-#define WO_STENCIL_SCALE 4294967296.0f
+#define WO_STENCIL_SCALE 1073741824.0f  /* 2^30: a term (|term| <= 0.5) fits 32 bits */
 #define WO_STENCIL_BUDGET 4096
 /* Overlaps are summed as multiples of 1/WO_OVERLAP_SCALE (llrintf). */
 #define WO_OVERLAP_SCALE 16777216.0f

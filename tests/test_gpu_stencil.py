@@ -3,7 +3,7 @@ the reference step) on the HIP path vs the oracle's wo_step_s, through the
 C ABI (needs a GPU).
 
 Mode S is an extension with no counterpart in the reference.  Its passes sum
-each particle's pair terms in 2^-32 fixed point, so the device's in-cell
+each particle's pair terms in 2^-30 fixed point, so the device's in-cell
 order does not matter and the bar is the same as mode R's: indices exact,
 positions and velocities bit for bit, and the overlap statistics equal.
 """

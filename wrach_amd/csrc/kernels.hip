@@ -1713,12 +1713,15 @@ __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, Fra
 // the reference step (an extension: the reference collides within one cell,
 // shaders/physics/src/cell.rs:60-69).  A pass is Jacobi -- every particle
 // reads the positions the pass started from -- and a particle's pair terms
-// are rounded to multiples of 2^-32 and summed in int64, exactly, so its
+// are rounded to multiples of 2^-30 and summed in int64, exactly, so its
 // displacement does not depend on the order of the particles inside a cell
 // (which the device layout keeps canonical only through `src`).
 
-constexpr float kStencilScale = 4294967296.0f;            // 2^32
-constexpr float kStencilUnit = 2.3283064365386963e-10f;   // 2^-32
+// A term is at most 0.5 (plus rounding) in magnitude, so term * 2^30 is
+// below 2^31: rintf (round half to even, as llrintf) and a 32-bit
+// conversion give it exactly, in place of the long float-to-int64 sequence.
+constexpr float kStencilScale = 1073741824.0f;            // 2^30
+constexpr float kStencilUnit = 9.313225746154785e-10f;    // 2^-30
 
 // First particle of cell k (k == ncell: the end).  Cells of a chunk the last
 // scan found empty all start at the chunk's prefix (their idx entries are
@@ -1793,7 +1796,7 @@ __device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float
 
 // Particle i (in cell (cx, cy) at the frame's start) against the 3x3 cells
 // around it: the largest overlap, the number of overlapping pairs, and with
-// MOVE the sum of its pair terms in 2^-32 units, without it the sum of the
+// MOVE the sum of its pair terms in 2^-30 units, without it the sum of the
 // overlaps in 2^-24 units.  Over kStencilBudget particles in those cells:
 // none of these (a bound on the work of one lane).
 template <bool MOVE>
@@ -1817,8 +1820,8 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
             worst = fmaxf(worst, ov);
             pairs += 1;
             if (MOVE) {
-                ax -= __float2ll_rn(ex * kStencilScale);
-                ay -= __float2ll_rn(ey * kStencilScale);
+                ax -= (int32_t)rintf(ex * kStencilScale);
+                ay -= (int32_t)rintf(ey * kStencilScale);
             } else {
                 // ov * 2^24 < 2^24: rintf (round half to even, as llrintf)
                 // is integral and converts exactly through 32 bits.
