@@ -21,6 +21,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -85,6 +86,39 @@ def copy_probe(nbytes: int) -> int:
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
     return 0
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> int:
+    """Bandwidth of a plain device-to-device copy (bytes read + written per
+    second, HIP events around `reps` torch copies of nbytes): the practical
+    HBM ceiling beside the published peak.  Run as a child process of the
+    bench (torch does not initialise HIP beside the library in one process)."""
+    import torch
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        b.copy_(a)
+    stop.record()
+    stop.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    print(json.dumps({"copy_GBs": 2 * nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes, "reps": reps}), flush=True)
+    return 0
+
+
+def measured_copy_gbs() -> float | None:
+    """copy_bandwidth() in a child process; None if it fails."""
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-bandwidth"],
+                           capture_output=True, text=True, timeout=180)
+        return float(json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"])
+    except Exception as e:  # noqa: BLE001 -- a missing figure must not end the bench
+        print(f"[bench] device copy bandwidth not measured: {e}", file=sys.stderr, flush=True)
+        return None
 
 
 def upload_times(wl) -> dict:
@@ -209,11 +243,15 @@ def main() -> int:
     ap.add_argument("--passes", type=int, default=0, metavar="K",
                     help="mode S: K separation passes over the 3x3 cells around each particle "
                          "before the reference step (an extension; default 0, the reference's step)")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="only measure a device-to-device copy's bandwidth (the bench's child process)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.copy_bandwidth:
+        return copy_bandwidth()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the world would shard by row strips with a migrant exchange; not built).",
@@ -274,6 +312,9 @@ def main() -> int:
           file=sys.stderr, flush=True)
     upload = upload_times(wl)
     print(f"[bench] upload ms: {json.dumps(upload)}", file=sys.stderr, flush=True)
+    copy_gbs = measured_copy_gbs()
+    if copy_gbs:
+        print(f"[bench] device copy {copy_gbs:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline and args.passes == 0:
@@ -320,6 +361,8 @@ def main() -> int:
             "step_algorithmic_bytes": step_bytes,
             "step_achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
             "step_frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "measured_copy_GBs": copy_gbs,
+            "frac_of_measured_copy": achieved / copy_gbs if copy_gbs else None,
         },
         "cpu_baseline": cpu,
         "upload": upload,
