@@ -131,26 +131,39 @@ int enqueue_stencil(wrach_ctx *c) {
     a.sums = c->ssums;
     a.tol = c->overlap_tol;
     WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->ssums, c->stream));
-    a.pin = c->buf.pos;
-    a.first = 1;
-    a.in_slot = 0;
-    a.stat = 1;  // the overlap sums before the passes
-    WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
-    a.first = 0;
-    for (uint32_t k = 0; k < c->passes; ++k) {
+    const uint32_t passes = c->passes;
+    // With a tolerance, the start of every pass is measured before it (the
+    // pass runs only above the tolerance).  With tolerance 0 a pass over no
+    // overlap moves nothing, so each pass measures its own start and only
+    // the end has a measure of its own.
+    const bool check = c->overlap_tol > 0.0f;
+    a.first = 1;  // the frame's first kernel derives the cells
+    if (check) {
+        a.pin = c->buf.pos;
+        a.in_slot = 0;
+        a.stat = 1;  // the overlap sums before the passes
+        WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
+        a.first = 0;
+    }
+    for (uint32_t k = 0; k < passes; ++k) {
         float2 *in = k % 2 == 0 ? c->buf.pos : c->buf.pos_out;
         float2 *out = k % 2 == 0 ? c->buf.pos_out : c->buf.pos;
         a.pin = in;
         a.pout = out;
         a.in_slot = k;
-        a.pass = k;
+        a.check = check ? 1 : 0;
+        a.stat = !check && k == 0 ? 1 : 0;
         WRACH_HIP(wrach::launch_stencil_move(c->fp, c->buf, a, c->stream));
-        a.pin = out;
-        a.in_slot = k + 1;
-        a.stat = k + 1 == c->passes ? 2 : 0;  // the last measure: the sums after them
-        WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
+        a.first = 0;
+        if (check || k + 1 == passes) {
+            a.pin = out;
+            a.in_slot = k + 1;
+            a.check = 0;
+            a.stat = k + 1 == passes ? 2 : 0;  // the last measure: the sums after the passes
+            WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
+        }
     }
-    if (c->passes % 2) WRACH_HIP(wrach::launch_copy_pos(c->buf.pos, c->buf.pos_out, c->fp.n, c->stream));
+    if (passes % 2) WRACH_HIP(wrach::launch_copy_pos(c->buf.pos, c->buf.pos_out, c->fp.n, c->stream));
     return WRACH_OK;
 }
 
@@ -772,7 +785,13 @@ int wrach_solver_stats(wrach_ctx *c, wrach_solver_report *out) {
     wrach_solver_report r{};
     std::memcpy(&r.max_overlap_before, &w[0], sizeof r.max_overlap_before);
     std::memcpy(&r.max_overlap_after, &w[c->passes], sizeof r.max_overlap_after);
-    r.passes = w[wrach::kStencilPassesWord];
+    // The passes that ran: those whose start exceeded the tolerance (a pass
+    // over no more overlap than that moves nothing, or is skipped).
+    for (uint32_t k = 0; k < c->passes; ++k) {
+        float start;
+        std::memcpy(&start, &w[k], sizeof start);
+        if (start > c->overlap_tol) r.passes += 1;
+    }
     uint64_t acc[4] = {0, 0, 0, 0};
     for (uint32_t which = 0; which < 2; ++which)
         for (uint32_t slot = 0; slot < wrach::kStencilSumSlots; ++slot) {

@@ -162,11 +162,10 @@ hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, h
 
 // Mode S: separation passes over the 3x3 cells around each particle before
 // the reference step (the kernels' comment has the rule).  `words` holds the
-// largest overlap (float bits) after 0..max_passes passes, then the number
-// of passes that ran, for the last frame.
+// largest overlap (float bits) after 0..max_passes passes for the last
+// frame; the passes that ran are those whose start exceeded the tolerance.
 constexpr uint32_t kMaxStencilPasses = 64;
-constexpr uint32_t kStencilPassesWord = kMaxStencilPasses + 1;
-constexpr uint32_t kStencilWords = kMaxStencilPasses + 2;
+constexpr uint32_t kStencilWords = kMaxStencilPasses + 1;
 constexpr uint32_t kStencilBudget = 4096;  // most particles one lane visits in its 3x3 cells
 // Overlap sums (1 - d in 2^-24 units) and pair counts of the frame's first
 // and last measure, striped over kStencilSumSlots counters 128 bytes apart
@@ -185,8 +184,8 @@ struct StencilArgs {
     unsigned long long *sums;  // kStencilSumEntries entries
     uint32_t stat;      // a measure: 0, or 1 / 2 to add its overlaps to the first / last sums
     uint32_t in_slot;   // a measure: the word it raises; a move: the word of its start
-    uint32_t first;     // a measure of the frame-start positions (writes `cell`)
-    uint32_t pass;      // a move's number, from 0
+    uint32_t first;     // the frame's first kernel: derives and writes `cell`
+    uint32_t check;     // a move: skip (copy) when the measured start is within tol
     float tol;          // a move runs while the overlap at its start exceeds this
 };
 hipError_t launch_stencil_measure(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,

@@ -1795,16 +1795,16 @@ __device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float
 }
 
 // Particle i (in cell (cx, cy) at the frame's start) against the 3x3 cells
-// around it: the largest overlap, the number of overlapping pairs, and with
-// MOVE the sum of its pair terms in 2^-30 units, without it the sum of the
-// overlaps in 2^-24 units.  Over kStencilBudget particles in those cells:
-// none of these (a bound on the work of one lane).
+// around it: the largest overlap, the number of overlapping pairs, with
+// MOVE the sum of its pair terms in 2^-30 units, and with `sums` the sum of
+// the overlaps in 2^-24 units.  Over kStencilBudget particles in those
+// cells: none of these (a bound on the work of one lane).
 template <bool MOVE>
 __device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
                                                   const float2 *pin, uint32_t i, uint32_t k,
                                                   float xi, float yi, long long &ax,
                                                   long long &ay, uint32_t &pairs,
-                                                  unsigned long long &osum) {
+                                                  unsigned long long &osum, bool sums) {
     uint32_t rs[3], re[3];
     const uint32_t base = world_base(p, k), local = k - base;
     stencil_rows(p, b, base, (int)(local % p.gx), (int)(local / p.gx), rs, re);
@@ -1819,13 +1819,12 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
             if (!stencil_pair(xi, yi, q.x, q.y, ex, ey, ov)) continue;
             worst = fmaxf(worst, ov);
             pairs += 1;
+            // ov * 2^24 < 2^24: rintf (round half to even, as llrintf) is
+            // integral and converts exactly through 32 bits.
+            if (sums) osum += (uint32_t)rintf(ov * kOverlapScale);
             if (MOVE) {
                 ax -= (int32_t)rintf(ex * kStencilScale);
                 ay -= (int32_t)rintf(ey * kStencilScale);
-            } else {
-                // ov * 2^24 < 2^24: rintf (round half to even, as llrintf)
-                // is integral and converts exactly through 32 bits.
-                osum += (uint32_t)rintf(ov * kOverlapScale);
             }
         }
     }
@@ -1874,6 +1873,20 @@ __device__ __forceinline__ void block_report(uint32_t *w, float v, unsigned long
 // The largest overlap at positions a.pin, raised into a.words[a.in_slot]
 // (non-negative floats order as their bits).  a.first: the frame-start
 // positions, whose cells every pass of the frame uses.
+// Particle i's cell at the frame's start: derived from its position by the
+// frame's first stencil kernel (a.first), which records it; read after.
+__device__ __forceinline__ uint32_t stencil_cell(const FrameParams &p, const FrameBuffers &b,
+                                                 const StencilArgs &a, uint32_t i, float2 q) {
+    if (!a.first) return a.cell[i];
+    const uint32_t k = cell_key(p, q.x, q.y, particle_world_base(p, b, i));
+    a.cell[i] = k;
+    return k;
+}
+
+__device__ __forceinline__ unsigned long long *stencil_sums(const StencilArgs &a) {
+    return a.stat ? a.sums + (a.stat - 1) * kStencilSumSlots * kStencilSumStride : nullptr;
+}
+
 __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, FrameBuffers b,
                                                               StencilArgs a) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -1882,39 +1895,40 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
     unsigned long long osum = 0;
     if (i < p.n) {
         const float2 q = a.pin[i];
-        uint32_t k;
-        if (a.first) {
-            k = cell_key(p, q.x, q.y, particle_world_base(p, b, i));
-            a.cell[i] = k;
-        } else {
-            k = a.cell[i];
-        }
+        const uint32_t k = stencil_cell(p, b, a, i, q);
         long long ax = 0, ay = 0;
-        if (k < p.ncell) worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
+        if (k < p.ncell)
+            worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, a.stat != 0);
     }
-    unsigned long long *sums =
-        a.stat ? a.sums + (a.stat - 1) * kStencilSumSlots * kStencilSumStride : nullptr;
-    block_report(&a.words[a.in_slot], worst, sums, osum, pairs);
+    block_report(&a.words[a.in_slot], worst, stencil_sums(a), osum, pairs);
 }
 
-// Pass a.pass: a.pin -> a.pout, run only while the overlap at its start
-// (a.words[a.in_slot], the same for the whole grid) exceeds the tolerance;
-// otherwise a copy.
+// One pass: a.pin -> a.pout.  With a.check it runs only while the
+// overlap at its start (a.words[a.in_slot], measured before, the same for
+// the whole grid) exceeds the tolerance, and is a copy otherwise.  Either
+// way it raises a.words[a.in_slot] to the largest overlap of its input, so
+// that a frame without a check (tolerance 0, where a pass over no overlap
+// moves nothing) needs no measure between passes.
 __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameBuffers b,
                                                            StencilArgs a) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= p.n) return;
-    const float2 q = a.pin[i];
-    if (!(__uint_as_float(a.words[a.in_slot]) > a.tol)) {
-        a.pout[i] = q;
-        return;
+    if (a.check && !(__uint_as_float(a.words[a.in_slot]) > a.tol)) {
+        if (i < p.n) a.pout[i] = a.pin[i];
+        return;  // the whole grid: no barrier follows
     }
-    if (i == 0) a.words[kStencilPassesWord] = a.pass + 1;
+    float worst = 0.0f;
     long long ax = 0, ay = 0;
     uint32_t pairs = 0;
     unsigned long long osum = 0;
-    const uint32_t k = a.cell[i];
-    if (k < p.ncell) (void)stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
+    float2 q = make_float2(0.0f, 0.0f);
+    if (i < p.n) {
+        q = a.pin[i];
+        const uint32_t k = stencil_cell(p, b, a, i, q);
+        if (k < p.ncell)
+            worst = stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, a.stat != 0);
+    }
+    block_report(&a.words[a.in_slot], worst, stencil_sums(a), osum, pairs);
+    if (i >= p.n) return;
     // The mean of the pair corrections (constraint averaging): a particle
     // moves at most 0.5 per pass.
     float dx = __ll2float_rn(ax) * kStencilUnit;
