@@ -634,3 +634,31 @@ def test_blobs_4m_full_size(oracle):
     than the reference's 4,194,304-cell scan limit): exact for 3 frames."""
     wl = workloads.gaussian_blobs()
     run_parity(oracle, wl.config, wl.pos, wl.vel, 3, every=1, label=wl.name, threads=THREADS)
+
+
+def test_blobs_4m_rollout_500(oracle):
+    """The 4,000,000-particle blobs workload for 500 frames: binning, offsets,
+    in-cell order and state exact against the oracle at frames 250 and 500,
+    and the invariants at the end (every particle in its cell, offsets
+    sorted, inside the walls, |v| <= 1)."""
+    wl = workloads.gaussian_blobs()
+    say(f"[rollout] {wl.name} 500 frames")
+    P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        for upto in (250, 500):
+            w.step(250)
+            ref = oracle.step(P, *ref, nsteps=250, nthreads=THREADS)
+            say(f"[rollout] {wl.name} frame {upto}")
+            assert_same_state(w.read(), ref, f"blobs frame {upto}")
+        idx, pos, vel = w.read()
+    W, H = wl.config.dimensions
+    gx = W // wl.config.cell_size + 1
+    assert idx[0] == 0 and idx[1] == 0 and idx[-1] == len(pos)
+    assert np.all(np.diff(idx.astype(np.int64)) >= 0)
+    cs = wl.config.cell_size
+    keys = (np.floor(pos[:, 1] / cs).astype(np.int64) * gx + np.floor(pos[:, 0] / cs).astype(np.int64))
+    cell_of = np.repeat(np.arange(len(idx) - 2), np.diff(idx[1:].astype(np.int64)))
+    assert np.array_equal(keys, cell_of)
+    assert pos.min() >= 0 and pos[:, 0].max() <= W and pos[:, 1].max() <= H
+    assert np.abs(vel).max() <= 1.0
