@@ -1812,6 +1812,7 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
     float worst = 0.0f;
     if (total > kStencilBudget) return worst;
     for (int t = 0; t < 3; ++t) {
+#pragma unroll 4
         for (uint32_t j = rs[t]; j < re[t]; ++j) {
             if (j == i) continue;
             const float2 q = pin[j];
