@@ -243,6 +243,8 @@ def main() -> int:
     ap.add_argument("--passes", type=int, default=0, metavar="K",
                     help="mode S: K separation passes over the 3x3 cells around each particle "
                          "before the reference step (an extension; default 0, the reference's step)")
+    ap.add_argument("--quiet", action="store_true",
+                    help="with --passes: no overlap measure after the passes (WRACH_SOLVER_QUIET)")
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="only measure a device-to-device copy's bandwidth (the bench's child process)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -270,7 +272,7 @@ def main() -> int:
     world.add_particles(wl.pos, wl.vel)
     world.set_graph(not args.no_graph)
     if args.passes > 0:
-        world.set_solver("S", args.passes)
+        world.set_solver("S", args.passes, quiet=args.quiet)
     world.step(args.warmup)
     world.sync()
     tuning = world.autotune(args.autotune) if args.autotune > 0 else world.tuning()
@@ -285,7 +287,7 @@ def main() -> int:
     solver = {"mode": "R"}
     if args.passes > 0:
         before, after, ran, (sb, nb, sa, na) = world.solver_stats()
-        solver = {"mode": "S", "passes": args.passes, "overlap_tol": 0.0,
+        solver = {"mode": "S", "passes": args.passes, "overlap_tol": 0.0, "quiet": args.quiet,
                   "last_frame_max_overlap": [before, after], "last_frame_passes": ran,
                   "last_frame_mean_overlap": [sb / 2**24 / nb if nb else 0.0, sa / 2**24 / na if na else 0.0],
                   "last_frame_overlapping_pairs": [nb // 2, na // 2]}

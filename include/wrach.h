@@ -256,6 +256,9 @@ int wrach_set_view_anchor(wrach_ctx *ctx, float x, float y);
  * WRACH_STENCIL_BUDGET particles is not displaced by the passes. */
 #define WRACH_SOLVER_R 0u
 #define WRACH_SOLVER_S 1u
+/* OR-ed into WRACH_SOLVER_S: no measure after the passes (they are the
+ * same), so the report's "after" figures are not taken (after_measured 0). */
+#define WRACH_SOLVER_QUIET 0x100u
 #define WRACH_MAX_PASSES 64u
 #define WRACH_STENCIL_BUDGET 4096u
 
@@ -275,7 +278,7 @@ typedef struct wrach_solver_report {
     float max_overlap_before;     /* largest 1 - d */
     float max_overlap_after;
     uint32_t passes;              /* passes that ran */
-    uint32_t reserved;
+    uint32_t after_measured;      /* 1: the "after" figures were taken (not WRACH_SOLVER_QUIET) */
     uint64_t overlap_sum_before;  /* sum of 1 - d, in units of 2^-24 */
     uint64_t pairs_before;        /* overlapping pairs, each counted twice */
     uint64_t overlap_sum_after;

@@ -33,6 +33,7 @@ def test_header_constants_match_python():
     assert flag == _lib.WRACH_FLAG_HOST_PACK
     kernels = int(re.search(r"#define WRACH_NUM_FRAME_KERNELS (\d+)", text).group(1))
     assert kernels == _lib.NUM_FRAME_KERNELS == len(_lib.FRAME_KERNEL_NAMES)
+    assert int(re.search(r"#define WRACH_SOLVER_QUIET (0x[0-9a-fA-F]+)u", text).group(1), 16) == _lib.WRACH_SOLVER_QUIET
     for name in ("WRACH_SOLVER_R", "WRACH_SOLVER_S", "WRACH_MAX_PASSES", "WRACH_STENCIL_BUDGET"):
         value = int(re.search(rf"#define {name} (\d+)u", text).group(1))
         assert value == getattr(_lib, name), name

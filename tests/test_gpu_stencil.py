@@ -200,3 +200,32 @@ def test_stencil_batch_matches_each_world_alone(oracle, cs):
             assert before == max(st[0] for st in stats) and after == max(st[1] for st in stats)
             assert sums == tuple(sum(st[3][j] for st in stats) for j in range(4))
             assert ran <= 2 and (ran > 0) == (before > 0)
+
+
+@pytest.mark.parametrize("tol", [0.0, 0.3])
+def test_stencil_quiet_skips_only_the_closing_measure(oracle, tol):
+    """WRACH_SOLVER_QUIET: the same frames as mode S (bit for bit against the
+    oracle), the same overlap before the passes and passes run; the figures
+    after the passes are not taken."""
+    pos, vel = uniform(6000, 160, 128, 21)
+    cfg = WrachConfig(dimensions=(160, 128), cell_size=1, gravity=(0.0, -0.1))
+    say(f"[stencil] quiet, tol={tol}")
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        w.set_solver("S", 3, tol, quiet=True)
+        for _ in range(3):
+            w.step(2)
+            *ref, st = oracle.step_s(P, *ref, max_passes=3, overlap_tol=tol, nsteps=2)
+            assert_same_state(w.read(), tuple(ref), "quiet")
+            before, after, ran, sums = w.solver_stats()
+            assert (before, ran, sums[:2]) == (st[0], st[2], st[3][:2])
+            assert after == 0.0 and sums[2:] == (0, 0) and not w.after_measured()
+        w.set_solver("S", 3, tol)
+        w.step(1)
+        *ref, st = oracle.step_s(P, *ref, max_passes=3, overlap_tol=tol, nsteps=1)
+        assert w.solver_stats() == st and w.after_measured()
+    with World(cfg) as w:
+        with pytest.raises(WrachError) as e:
+            _lib.check(w._L.wrach_set_solver(w._h, _lib.WRACH_SOLVER_R | _lib.WRACH_SOLVER_QUIET, 0, 0.0))
+        assert e.value.code == _lib.WRACH_E_INVALID

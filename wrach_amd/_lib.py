@@ -22,6 +22,7 @@ WRACH_E_STATE = -7
 WRACH_FLAG_HOST_PACK = 0x1
 WRACH_SOLVER_R = 0
 WRACH_SOLVER_S = 1
+WRACH_SOLVER_QUIET = 0x100
 WRACH_MAX_PASSES = 64
 WRACH_STENCIL_BUDGET = 4096
 NUM_FRAME_KERNELS = 4
@@ -87,7 +88,7 @@ class SolverReportC(ctypes.Structure):
         ("max_overlap_before", ctypes.c_float),
         ("max_overlap_after", ctypes.c_float),
         ("passes", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("after_measured", ctypes.c_uint32),
         ("overlap_sum_before", ctypes.c_uint64),
         ("pairs_before", ctypes.c_uint64),
         ("overlap_sum_after", ctypes.c_uint64),

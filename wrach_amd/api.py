@@ -202,14 +202,23 @@ class World:
         """wrach_set_view_anchor: move the simulated window over the store."""
         check(self._L.wrach_set_view_anchor(self._h, float(x), float(y)))
 
-    def set_solver(self, mode: str = "R", max_passes: int = 0, overlap_tol: float = 0.0) -> None:
+    def set_solver(self, mode: str = "R", max_passes: int = 0, overlap_tol: float = 0.0,
+                   quiet: bool = False) -> None:
         """wrach_set_solver: "R", the reference's step, or "S", up to
         max_passes separation passes over the 3x3 cells around each particle
-        (each while the largest overlap exceeds overlap_tol), then that step."""
+        (each while the largest overlap exceeds overlap_tol), then that step.
+        quiet (mode S): no measure after the passes (WRACH_SOLVER_QUIET)."""
         modes = {"R": _lib.WRACH_SOLVER_R, "S": _lib.WRACH_SOLVER_S}
         if mode not in modes:
             raise ValueError(f"unknown solver {mode!r}")
-        check(self._L.wrach_set_solver(self._h, modes[mode], int(max_passes), float(overlap_tol)))
+        m = modes[mode] | (_lib.WRACH_SOLVER_QUIET if quiet else 0)
+        check(self._L.wrach_set_solver(self._h, m, int(max_passes), float(overlap_tol)))
+
+    def after_measured(self) -> bool:
+        """Whether the last report holds the figures after the passes."""
+        r = _lib.SolverReportC()
+        check(self._L.wrach_solver_stats(self._h, ctypes.byref(r)))
+        return bool(r.after_measured)
 
     def solver_stats(self):
         """The last mode S frame (wrach_solver_stats): (largest overlap

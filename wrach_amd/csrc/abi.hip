@@ -103,6 +103,7 @@ struct wrach_ctx {
     // Solver: WRACH_SOLVER_R, or WRACH_SOLVER_S with up to `passes`
     // separation passes before the reference step (stencil_* kernels).
     uint32_t solver = WRACH_SOLVER_R;
+    bool quiet = false;  // WRACH_SOLVER_QUIET: no measure after the passes
     uint32_t passes = 0;
     float overlap_tol = 0.0f;
     uint32_t *scell = nullptr;   // frame-start cell of each particle (capacity entries)
@@ -155,7 +156,7 @@ int enqueue_stencil(wrach_ctx *c) {
         a.stat = !check && k == 0 ? 1 : 0;
         WRACH_HIP(wrach::launch_stencil_move(c->fp, c->buf, a, c->stream));
         a.first = 0;
-        if (check || k + 1 == passes) {
+        if (check ? (k + 1 < passes || !c->quiet) : (k + 1 == passes && !c->quiet)) {
             a.pin = out;
             a.in_slot = k + 1;
             a.check = 0;
@@ -735,6 +736,8 @@ static_assert(WRACH_STENCIL_BUDGET == wrach::kStencilBudget, "one budget");
 
 int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float overlap_tol) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
+    const bool quiet = (mode & WRACH_SOLVER_QUIET) != 0 && (mode & ~WRACH_SOLVER_QUIET) == WRACH_SOLVER_S;
+    if (quiet) mode = WRACH_SOLVER_S;
     if (mode != WRACH_SOLVER_R && mode != WRACH_SOLVER_S) return fail(WRACH_E_INVALID, "no such solver");
     if (mode == WRACH_SOLVER_S) {
         if (max_passes > wrach::kMaxStencilPasses)
@@ -765,6 +768,7 @@ int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float ove
     if (c->ssums) WRACH_HIP(hipMemset(c->ssums, 0, wrach::kStencilSumEntries * sizeof(unsigned long long)));
     free_graphs(c);  // captured with the previous solver
     c->solver = mode;
+    c->quiet = quiet;
     c->passes = mode == WRACH_SOLVER_S ? max_passes : 0;
     c->overlap_tol = mode == WRACH_SOLVER_S ? overlap_tol : 0.0f;
     g_err.clear();
@@ -803,6 +807,7 @@ int wrach_solver_stats(wrach_ctx *c, wrach_solver_report *out) {
     r.pairs_before = acc[1];
     r.overlap_sum_after = acc[2];
     r.pairs_after = acc[3];
+    r.after_measured = c->solver == WRACH_SOLVER_S && c->passes > 0 && !c->quiet ? 1u : 0u;
     *out = r;
     return WRACH_OK;
 }
