@@ -137,3 +137,29 @@ def test_view_errors_and_capacity(oracle):
         with pytest.raises(WrachError) as e:
             h.set_view_anchor(64.0, 0.0)
         assert e.value.code == _lib.WRACH_E_INVALID
+
+
+def test_view_with_mode_s(oracle):
+    """Mode S on a moved view: the stencil's cells are the view's (keys
+    relative to the anchor); frames bit for bit against the store model
+    stepped by the oracle's mode S."""
+    cfg = WrachConfig(dimensions=(64, 48), cell_size=1, gravity=(0.0, -0.1))
+    pos, vel = spread(3000, 200, 50, 60)
+    say("[view] mode S on a moved view")
+    model = StoreModel(oracle, cfg)
+    model.add(pos, vel)
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        w.set_solver("S", 2)
+        P, ref = model.pack()
+        w.step(3)
+        *ref, _ = oracle.step_s(P, *ref, max_passes=2, nsteps=3)
+        assert_same_state(w.read(), tuple(ref), "origin, mode S")
+        for anchor in [(70, 0), (120, 2)]:
+            model.move(ref, anchor)
+            P, ref = model.pack()
+            w.set_view_anchor(*anchor)
+            w.step(4)
+            *ref, st = oracle.step_s(P, *ref, max_passes=2, nsteps=4)
+            assert_same_state(w.read(), tuple(ref), f"view at {anchor}, mode S")
+            assert w.solver_stats() == st
