@@ -55,9 +55,25 @@ def particles_by_kernel(idx: np.ndarray) -> dict:
     return {"single": single, "wide": wide, "group": int(cnt.sum()) - single - wide}
 
 
+def host_cores() -> dict:
+    """The host's cores as the process sees them: nproc (os.cpu_count), the
+    CPUs this process may run on (sched_getaffinity) and the CPU share the
+    environment grants (OMP_NUM_THREADS; 16 per GPU on the MI355X boxes)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(share) if share.isdigit() and int(share) > 0 else None
+    threads = min(affinity, share) if share else affinity
+    return {"nproc": nproc, "affinity": affinity, "share": share, "threads": threads}
+
+
 def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
     from oracle import oracle as o
-    threads = min(16, os.cpu_count() or 1)
+    hc = host_cores()
+    threads = hc["threads"]
     w, h = wl.config.dimensions
     idx, p, v = o.pack(wl.pos, wl.vel, (0, 0, w, h), wl.config.cell_size)
     P = o.params(w, h, wl.config.cell_size, len(p), gravity=wl.config.gravity, dt=wl.config.dt)
@@ -69,9 +85,11 @@ def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
     o.step(P, idx, p, v, nsteps=frames, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": len(p) * frames / dt, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": hc,
             "sample": f"{frames} frames of {wl.name} ({len(p)} particles) after 1 warm-up frame, "
-                      f"oracle wo_step_mt on {threads} host threads, {dt:.2f} s"}
+                      f"oracle wo_step_mt on {threads} host threads (nproc {hc['nproc']}, "
+                      f"{hc['affinity']} CPUs in the affinity mask, CPU share "
+                      f"{hc['share'] if hc['share'] else 'not set'}), {dt:.2f} s"}
 
 
 def copy_probe(nbytes: int) -> int:
@@ -175,6 +193,7 @@ def bench_many_worlds(args) -> int:
           f"upload {upload_s:.2f} s", file=sys.stderr, flush=True)
     for _, _, b in batches:
         b.step(args.warmup)
+        b.prepare(1)  # the one-frame graph the timed loop replays
     for _, _, b in batches:
         b.sync()
     times = []
@@ -277,6 +296,10 @@ def main() -> int:
     world.sync()
     tuning = world.autotune(args.autotune) if args.autotune > 0 else world.tuning()
     print(f"[bench] launch shapes: {json.dumps(tuning)}", file=sys.stderr, flush=True)
+    # Capture the frame graphs the timed call launches (after any autotune,
+    # which drops them) without running a frame: the timed region replays.
+    world.prepare(args.steps)
+    world.sync()
 
     t0 = time.perf_counter()
     world.step(args.steps)
@@ -293,7 +316,9 @@ def main() -> int:
                   "last_frame_overlapping_pairs": [nb // 2, na // 2]}
         print(f"[bench] mode S: {json.dumps(solver)}", file=sys.stderr, flush=True)
 
-    # Per-kernel durations: kernel-stamped HIP events, plain launches.
+    # Per-kernel durations: kernel-stamped HIP events, plain launches, over
+    # the frames right after the timed ones.
+    kernel_frames = [args.warmup + args.steps, args.warmup + args.steps + args.profile_steps]
     split_before = particles_by_kernel(world.read()[0])
     ms = world.step_profiled(args.profile_steps)
     split_after = particles_by_kernel(world.read()[0])
@@ -303,11 +328,21 @@ def main() -> int:
     dom = max(avg_us, key=avg_us.get)
     achieved = bytes_[dom] / (avg_us[dom] * 1e-6) / 1e9
     step_bytes = sum(bytes_.values())
-    traffic = None
+    # Counter-measured HBM bytes of the dominant kernel (rocprofv3 FETCH_SIZE
+    # and WRITE_SIZE runs, summarised by --traffic-summary), reported as
+    # `traffic` only when they were taken over the same frames as the
+    # kernel times above; otherwise beside them, with their own frames.
+    traffic, traffic_other = None, None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{wl.name}.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath)).get(dom)
-        traffic = t.get("hbm_bytes_per_launch") if isinstance(t, dict) else None
+    if os.path.exists(tpath) and args.passes == 0:
+        tj = json.load(open(tpath))
+        t = tj.get("kernels", {}).get(dom)
+        if isinstance(t, dict):
+            if tj.get("frames") == kernel_frames:
+                traffic = t.get("hbm_bytes_per_launch")
+            else:
+                traffic_other = {"hbm_bytes_per_launch": t.get("hbm_bytes_per_launch"),
+                                 "frames": tj.get("frames")}
 
     world.close()
     print(f"[bench] kernel avg us: {json.dumps({k: round(v, 3) for k, v in avg_us.items()})}",
@@ -347,6 +382,8 @@ def main() -> int:
             "graph": not args.no_graph,
             "launch_shapes": {k: v for k, v in tuning.items() if k != "frame_us"},
             "autotuned": args.autotune > 0,
+            "graphs_captured_before_timing": not args.no_graph,
+            "timed_frames": [args.warmup, args.warmup + args.steps],
             "solver": solver,
         },
         "roofline": {
@@ -357,8 +394,10 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_other_frames": traffic_other,
             "algorithmic_bytes_per_launch": bytes_[dom],
             "kernel_avg_us": avg_us,
+            "kernel_frames": kernel_frames,
             "particles_by_kernel": [split_before, split_after],
             "step_algorithmic_bytes": step_bytes,
             "step_achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,

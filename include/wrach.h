@@ -38,7 +38,7 @@ typedef enum wrach_status {
     WRACH_E_TOO_MANY_CELLS = -2, /* grid too large (builder.rs:39-42 asserts) */
     WRACH_E_CAPACITY = -3,       /* more particles than the buffers hold */
     WRACH_E_NONFINITE = -4,      /* NaN position or velocity in an upload */
-    WRACH_E_DEVICE = -5,         /* HIP runtime error */
+    WRACH_E_DEVICE = -5,         /* HIP runtime error, or a device wait that gave up */
     WRACH_E_BUFFER = -6,         /* caller's output array too small */
     WRACH_E_STATE = -7           /* device state failed a consistency check */
 } wrach_status;
@@ -296,6 +296,22 @@ int wrach_set_graph(wrach_ctx *ctx, int enable);
  * kernel, synchronises, and adds each kernel's summed elapsed time in
  * milliseconds to kernel_ms[0..WRACH_NUM_FRAME_KERNELS). */
 int wrach_step_profiled(wrach_ctx *ctx, uint32_t nframes, double *kernel_ms);
+
+/* Captures the frame graphs that wrach_step(ctx, nframes) launches, without
+ * running a frame, so that the first such call starts at once (no
+ * counterpart in the reference).  A no-op with graphs off or no particles.
+ * The graphs are captured again after wrach_set_solver, wrach_set_tuning,
+ * wrach_autotune, a moved view or a new upload size. */
+int wrach_prepare(wrach_ctx *ctx, uint32_t nframes);
+
+/* Testing hook for the scan's bounded look-back (no counterpart in the
+ * reference).  A scan tile gives up its look-back after wait_limit rounds
+ * (default 2^20, about a second); the next synchronising call then returns
+ * WRACH_E_DEVICE and the context must be destroyed.  epoch_skew != 0 makes
+ * every look-back wait for an epoch no tile writes, so that a test can force
+ * that path from the host; all offsets stay clamped to the particles, so the
+ * frames that follow stay in bounds.  Production code never calls this. */
+int wrach_test_scan_wait(wrach_ctx *ctx, uint32_t wait_limit, uint32_t epoch_skew);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *wrach_last_error(void);

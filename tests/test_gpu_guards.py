@@ -1,0 +1,148 @@
+"""Device-side guards, graph preparation and the device views' contents
+(needs a GPU).
+
+- The scan's look-back is a bounded wait: forced to give up from the host
+  (an epoch skew no status word carries, and a small round limit), every
+  kernel still exits and the next synchronising call reports
+  WRACH_E_DEVICE.
+- wrach_prepare captures the frame graphs without running a frame.
+- The device views a renderer would read (wrach_device_state,
+  wrach_canonical_device; the reference binds its buffers to the draw pass,
+  runners/bevy/src/plugin/bind_groups.rs:27-83) hold the state that
+  wrach_read returns, including the offsets of chunks the frames left
+  empty.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from wrach_amd import World, WrachConfig, WrachError, workloads
+from wrach_amd import _lib
+
+from .test_gpu_parity import assert_same_state, oracle_world
+
+pytestmark = pytest.mark.gpu
+
+_HIP_D2H = 2  # hipMemcpyDeviceToHost
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _hip():
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    return hip
+
+
+def copy_out(hip, ptr, count, dtype):
+    out = np.empty(count, dtype)
+    if count:
+        rc = hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, _HIP_D2H)
+        assert rc == 0, f"hipMemcpy returned {rc}"
+    return out
+
+
+def test_lookback_bound_reports_a_device_error(oracle):
+    """Every look-back waits for an epoch that no tile writes and gives up
+    after 8 rounds: the frames finish, sync reports WRACH_E_DEVICE, and so
+    does a read; a fresh context is unaffected."""
+    wl = workloads.small_uniform(seed=11)
+    say("[guard] look-back bound, 512x512 world, skewed epoch, limit 8")
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        w.step(1)
+        w.sync()
+        w.test_scan_wait(8, epoch_skew=1)
+        w.step(3)
+        with pytest.raises(WrachError) as e:
+            w.sync()
+        assert e.value.code == _lib.WRACH_E_DEVICE
+        assert "look-back" in str(e.value)
+        with pytest.raises(WrachError) as e:
+            w.read()
+        assert e.value.code == _lib.WRACH_E_DEVICE
+    P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        w.step(4)
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=4), "fresh context")
+
+
+def test_lookback_limit_argument(oracle):
+    with World(WrachConfig(dimensions=(32, 32), cell_size=1)) as w:
+        with pytest.raises(WrachError) as e:
+            w.test_scan_wait(0)
+        assert e.value.code == _lib.WRACH_E_INVALID
+
+
+@pytest.mark.parametrize("frames", [20, 16, 5])
+def test_prepare_captures_without_running(oracle, frames):
+    """prepare(K) runs no frame; step(K) after it equals the oracle's K
+    frames, and so do further frames on the same graphs."""
+    wl = workloads.small_uniform(seed=12)
+    say(f"[guard] prepare({frames})")
+    P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
+    with World(wl.config) as w:
+        w.add_particles(wl.pos, wl.vel)
+        w.prepare(frames)
+        w.sync()
+        assert_same_state(w.read(), ref, "after prepare, no frame")
+        w.step(frames)
+        ref = oracle.step(P, *ref, nsteps=frames)
+        assert_same_state(w.read(), ref, f"after {frames} frames")
+        w.step(frames)
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=frames), f"after {2 * frames} frames")
+
+
+@pytest.mark.parametrize("cs", [1, 3])
+def test_device_views_hold_the_state(oracle, cs):
+    """After frames that leave the upper half's chunks empty: the canonical
+    view equals wrach_read and the oracle bit for bit; the live view has the
+    same offsets and, inside every cell, the same particles in some order."""
+    r = np.random.Generator(np.random.PCG64(13))
+    n, W, H = 12_000, 512, 512
+    pos = np.stack([r.uniform(0, W, n), r.uniform(0, H / 4, n)], 1).astype(np.float32)
+    vel = r.uniform(-1, 1, (n, 2)).astype(np.float32)
+    cfg = WrachConfig(dimensions=(W, H), cell_size=cs, gravity=(0.0, -0.1))
+    say(f"[views] {n} particles, {W}x{H}, cs {cs}")
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    hip = _hip()
+    with World(cfg) as w:
+        w.add_particles(pos, vel)
+        for frames in (3, 17):
+            w.step(frames)
+            ref = oracle.step(P, *ref, nsteps=frames)
+            live = w.device_state()
+            w.sync()
+            li = copy_out(hip, live.indices, live.n_indices, np.uint32)
+            lp = copy_out(hip, live.positions, 2 * live.n_particles, np.float32).reshape(-1, 2)
+            lv = copy_out(hip, live.velocities, 2 * live.n_particles, np.float32).reshape(-1, 2)
+            canon = w.device_state(canonical=True)
+            w.sync()
+            ci = copy_out(hip, canon.indices, canon.n_indices, np.uint32)
+            cp = copy_out(hip, canon.positions, 2 * canon.n_particles, np.float32).reshape(-1, 2)
+            cv = copy_out(hip, canon.velocities, 2 * canon.n_particles, np.float32).reshape(-1, 2)
+            assert_same_state((ci, cp, cv), ref, f"canonical view, cs {cs}")
+            assert_same_state(w.read(), ref, f"read, cs {cs}")
+            np.testing.assert_array_equal(li, ref[0])
+            # The upper three quarters of the world hold no particle: their
+            # offsets (chunks the scan skipped) must still read N.
+            gx = W // cs + 1
+            top = (H // cs // 2) * gx
+            assert (li[1 + top:] == li[-1]).all()
+            # Inside each cell the live view is a permutation of the
+            # canonical order: compare after sorting each cell's bit patterns.
+            cell = np.repeat(np.arange(len(li) - 2), np.diff(li[1:].astype(np.int64)))
+            key = np.stack([lp.view(np.uint32)[:, 0], lp.view(np.uint32)[:, 1],
+                            lv.view(np.uint32)[:, 0], lv.view(np.uint32)[:, 1]], 1).astype(np.uint64)
+            rkey = np.stack([ref[1].view(np.uint32)[:, 0], ref[1].view(np.uint32)[:, 1],
+                             ref[2].view(np.uint32)[:, 0], ref[2].view(np.uint32)[:, 1]], 1).astype(np.uint64)
+            o1 = np.lexsort(np.concatenate([key.T[::-1], cell[None]], 0))
+            o2 = np.lexsort(np.concatenate([rkey.T[::-1], cell[None]], 0))
+            np.testing.assert_array_equal(key[o1], rkey[o2])

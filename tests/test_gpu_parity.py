@@ -35,7 +35,11 @@ def assert_same_state(gpu, ref, what=""):
     ri, rp, rv = ref
     assert np.array_equal(gi, ri), f"{what}: indices differ at {np.flatnonzero(gi != ri)[:10]}"
     assert gp.shape == rp.shape and gv.shape == rv.shape, what
-    err = max(float(np.max(np.abs(gp - rp), initial=0.0)), float(np.max(np.abs(gv - rv), initial=0.0)))
+    # Non-finite values fail here on their own (np.max propagates NaN, and
+    # NaN <= TOL is false); the bit comparison below is stricter still.
+    assert np.isfinite(gp).all() and np.isfinite(gv).all(), f"{what}: non-finite state"
+    diff = np.concatenate([np.abs(gp - rp).ravel(), np.abs(gv - rv).ravel(), np.zeros(1, np.float32)])
+    err = float(np.max(diff))
     assert err <= TOL, f"{what}: max abs error {err}"
     bad = np.flatnonzero(np.any(gp.view(np.uint32) != rp.view(np.uint32), axis=1) |
                          np.any(gv.view(np.uint32) != rv.view(np.uint32), axis=1))

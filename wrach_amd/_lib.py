@@ -55,6 +55,8 @@ EXPORTS = (
     "wrach_set_solver",
     "wrach_solver_stats",
     "wrach_step_profiled",
+    "wrach_prepare",
+    "wrach_test_scan_wait",
     "wrach_last_error",
 )
 
@@ -168,6 +170,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_set_view_anchor.argtypes = [vp, ctypes.c_float, ctypes.c_float]
     L.wrach_solver_stats.argtypes = [vp, ctypes.POINTER(SolverReportC)]
     L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    L.wrach_prepare.argtypes = [vp, ctypes.c_uint32]
+    L.wrach_test_scan_wait.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.wrach_last_error.argtypes = []
     L.wrach_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:

@@ -155,6 +155,16 @@ class World:
     def step(self, nframes: int = 1) -> None:
         check(self._L.wrach_step(self._h, int(nframes)))
 
+    def prepare(self, nframes: int) -> None:
+        """Captures the graphs step(nframes) launches, running no frame
+        (wrach_prepare)."""
+        check(self._L.wrach_prepare(self._h, int(nframes)))
+
+    def test_scan_wait(self, wait_limit: int, epoch_skew: int = 0) -> None:
+        """Testing hook (wrach_test_scan_wait): the scan's look-back bound and
+        an epoch skew that makes every look-back give up."""
+        check(self._L.wrach_test_scan_wait(self._h, int(wait_limit), int(epoch_skew)))
+
     _TUNING = ("scan_threads", "scan_items", "scatter_per_lane", "cells_grid", "frame_us")
 
     def tuning(self) -> dict:

@@ -109,6 +109,7 @@ struct wrach_ctx {
     uint32_t *scell = nullptr;   // frame-start cell of each particle (capacity entries)
     uint32_t *swords = nullptr;  // wrach::kStencilWords words
     unsigned long long *ssums = nullptr;  // wrach::kStencilSumEntries entries
+    uint32_t *err_host = nullptr;  // pinned copy of buf.error, read at each synchronisation
 };
 
 namespace {
@@ -120,6 +121,26 @@ void free_graphs(wrach_ctx *c) {
     if (c->graph_many) (void)hipGraphExecDestroy(c->graph_many);
     c->graph_one = c->graph_many = nullptr;
 }
+
+// Synchronises the stream and reports a wait a kernel gave up
+// (FrameBuffers::error, raised by the scan's bounded look-back).  The word
+// is never cleared: the offsets of that frame, and so every later frame of
+// the context, are not valid.
+int settle(wrach_ctx *c) {
+    WRACH_HIP(hipMemcpyAsync(c->err_host, c->buf.error, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    WRACH_HIP(hipStreamSynchronize(c->stream));
+    if (*c->err_host & wrach::kErrLookBack)
+        return fail(WRACH_E_DEVICE, "the exclusive scan's look-back waited past its bound: the offsets "
+                                    "of a frame are not valid and the context must be destroyed");
+    return WRACH_OK;
+}
+
+#define WRACH_SETTLE(c)                 \
+    do {                                \
+        const int rc_ = settle(c);      \
+        if (rc_ != WRACH_OK) return rc_; \
+    } while (0)
 
 // Mode S's passes: P0 = pos; pass k reads pos or pos_out (k even or odd)
 // and writes the other (pos_out is free until the collide kernel writes
@@ -213,6 +234,8 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.chunk_live);
     (void)hipFree(c->buf.chunk_prefix);
     (void)hipFree(c->buf.epoch);
+    (void)hipFree(c->buf.error);
+    if (c->err_host) (void)hipHostFree(c->err_host);
     (void)hipFree(c->dstore_pos);
     (void)hipFree(c->dstore_vel);
     (void)hipFree(c->dks);
@@ -297,7 +320,7 @@ int add_host_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint
     WRACH_HIP(wrach::launch_iota(c->buf.src, packed, c->stream));
     // every offset came from the host: no chunk's offsets are stale
     WRACH_HIP(hipMemsetAsync(c->buf.chunk_live, 1, c->chunks, c->stream));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     if (c->fp.n != packed) free_graphs(c);
     c->fp.n = packed;
     c->settings.particles_in_frame_count = packed;
@@ -428,18 +451,18 @@ int add_device_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, ui
     WRACH_HIP(wrach::launch_upload_bin(u, c->buf, c->stream));
     uint32_t packed = 0;
     WRACH_HIP(hipMemcpyAsync(&packed, c->dvalid, sizeof packed, hipMemcpyDeviceToHost, c->stream));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     if (packed > c->capacity) {
         // the counts are the only state touched: clear them for the next frame
         WRACH_HIP(hipMemsetAsync(c->buf.cnt, 0, (c->ncell + 16) * sizeof(uint32_t), c->stream));
-        WRACH_HIP(hipStreamSynchronize(c->stream));
+        WRACH_SETTLE(c);
         return fail(WRACH_E_CAPACITY, "particles in frame (" + std::to_string(packed) +
                                           ") exceed the buffer capacity (" +
                                           std::to_string(c->capacity) + ")");
     }
     WRACH_HIP(wrach::launch_scan(c->fp, c->buf, c->stream));
     WRACH_HIP(wrach::launch_upload_scatter(u, c->buf, c->stream));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     c->dstore_n = total;
     if (c->fp.n != packed) free_graphs(c);
     c->fp.n = packed;
@@ -529,6 +552,8 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     fp.ncell = (uint32_t)ncell;
     fp.n = 0;
     fp.world_cells = (uint32_t)world_cells;
+    fp.wait_limit = wrach::kDefaultWaitLimit;
+    fp.epoch_skew = 0;
 
     wrach::FrameBuffers &b = c->buf;
     b.scan_shape = 0;
@@ -585,6 +610,9 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.chunk_live, c->chunks));
     WRACH_HIP_C(dmalloc(&b.chunk_prefix, c->chunks));
     WRACH_HIP_C(dmalloc(&b.epoch, 1));
+    WRACH_HIP_C(dmalloc(&b.error, 1));
+    WRACH_HIP_C(hipHostMalloc((void **)&c->err_host, sizeof(uint32_t), hipHostMallocDefault));
+    *c->err_host = 0;
     WRACH_HIP_C(dmalloc(&b.cells, ncells_total));
     WRACH_HIP_C(dmalloc(&b.class_ctr, wrach::kCtrWords));
     WRACH_HIP_C(dmalloc(&c->dvalid, 1));
@@ -604,6 +632,7 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
                                    c->stream));
     }
     WRACH_HIP_C(hipMemsetAsync(b.class_ctr, 0, wrach::kCtrWords * sizeof(uint32_t), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.error, 0, sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipStreamSynchronize(c->stream));
 #undef WRACH_HIP_C
     *out = c;
@@ -629,7 +658,7 @@ int add_to_world(wrach_ctx *c, uint32_t world, const float *pos_xy, const float 
         if (std::isnan(pos_xy[i]) || std::isnan(vel_xy[i]))
             return fail(WRACH_E_NONFINITE, "NaN in particle " + std::to_string(i / 2));
     WRACH_HIP(hipSetDevice(c->cfg.device));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     const int rc = c->host_pack ? add_host_packed(c, pos_xy, vel_xy, n)
                                 : add_device_packed(c, pos_xy, vel_xy, n, world);
     if (rc) return rc;
@@ -655,6 +684,42 @@ int wrach_add_particles_to_world(wrach_ctx *c, uint32_t world, const float *pos_
     return add_to_world(c, world, pos_xy, vel_xy, n);
 }
 
+namespace {
+
+// Captures the graphs wrach_step(nframes) launches (kFramesPerGraph frames,
+// and one frame for the remainder) that are not captured yet.
+int ensure_graphs(wrach_ctx *c, uint32_t nframes) {
+    if (nframes >= kFramesPerGraph && !c->graph_many) {
+        const int rc = capture(c, kFramesPerGraph, &c->graph_many);
+        if (rc) return rc;
+    }
+    if (nframes % kFramesPerGraph && !c->graph_one) {
+        const int rc = capture(c, 1, &c->graph_one);
+        if (rc) return rc;
+    }
+    return WRACH_OK;
+}
+
+}  // namespace
+
+int wrach_prepare(wrach_ctx *c, uint32_t nframes) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (nframes == 0 || c->fp.n == 0 || !c->use_graph) return WRACH_OK;
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    return ensure_graphs(c, nframes);
+}
+
+int wrach_test_scan_wait(wrach_ctx *c, uint32_t wait_limit, uint32_t epoch_skew) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (wait_limit == 0) return fail(WRACH_E_INVALID, "the wait limit must be at least one round");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_SETTLE(c);
+    free_graphs(c);  // the graphs hold the frame parameters by value
+    c->fp.wait_limit = wait_limit;
+    c->fp.epoch_skew = epoch_skew;
+    return WRACH_OK;
+}
+
 int wrach_step(wrach_ctx *c, uint32_t nframes) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
     if (nframes == 0 || c->fp.n == 0) return WRACH_OK;  // an empty world does not change
@@ -667,12 +732,8 @@ int wrach_step(wrach_ctx *c, uint32_t nframes) {
         }
         return WRACH_OK;
     }
-    if (nframes >= kFramesPerGraph && !c->graph_many) {
-        const int rc = capture(c, kFramesPerGraph, &c->graph_many);
-        if (rc) return rc;
-    }
-    if (nframes % kFramesPerGraph && !c->graph_one) {
-        const int rc = capture(c, 1, &c->graph_one);
+    {
+        const int rc = ensure_graphs(c, nframes);
         if (rc) return rc;
     }
     for (uint32_t f = 0; f < nframes / kFramesPerGraph; ++f)
@@ -707,6 +768,7 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
     hipError_t se = hipStreamSynchronize(c->stream);
     if (rc == WRACH_OK && se != hipSuccess)
         rc = fail(WRACH_E_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+    if (rc == WRACH_OK) rc = settle(c);
     if (rc == WRACH_OK) {
         for (uint32_t f = 0; f < nframes; ++f) {
             for (int k = 0; k < K; ++k) {
@@ -727,7 +789,7 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
 
 int wrach_sync(wrach_ctx *c) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     return WRACH_OK;
 }
 
@@ -746,7 +808,7 @@ int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float ove
             return fail(WRACH_E_INVALID, "overlap tolerance must be finite and >= 0");
     }
     WRACH_HIP(hipSetDevice(c->cfg.device));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     // Each buffer on its own: a failed allocation leaves the solver as it was,
     // and the next call allocates what is still missing.
     if (mode == WRACH_SOLVER_S && !c->scell) {
@@ -781,7 +843,7 @@ int wrach_solver_stats(wrach_ctx *c, wrach_solver_report *out) {
     std::vector<unsigned long long> sums(wrach::kStencilSumEntries, 0ull);
     if (c->swords && c->ssums) {
         WRACH_HIP(hipSetDevice(c->cfg.device));
-        WRACH_HIP(hipStreamSynchronize(c->stream));
+        WRACH_SETTLE(c);
         WRACH_HIP(hipMemcpy(w, c->swords, sizeof w, hipMemcpyDeviceToHost));
         WRACH_HIP(hipMemcpy(sums.data(), c->ssums, sums.size() * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost));
@@ -880,7 +942,7 @@ int wrach_store_write_back(wrach_ctx *c) {
             WRACH_HIP(hipMemcpyAsync(sv.data(), view.velocities, sv.size() * sizeof(float),
                                      hipMemcpyDeviceToHost, c->stream));
         }
-        WRACH_HIP(hipStreamSynchronize(c->stream));
+        WRACH_SETTLE(c);
         const uint32_t gx = c->settings.grid_dimensions[0], gy = c->settings.grid_dimensions[1];
         const float cs = (float)c->cfg.cell_size;
         for (size_t i = 0; i < c->store_pos.size() / 2; ++i) {
@@ -977,7 +1039,7 @@ int wrach_set_view_anchor(wrach_ctx *c, float x, float y) {
     if (std::fmod(x, cs) != 0.0f || std::fmod(y, cs) != 0.0f)
         return fail(WRACH_E_INVALID, "the view anchor must be a multiple of the cell size");
     WRACH_HIP(hipSetDevice(c->cfg.device));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     const float ox = c->settings.view_anchor[0], oy = c->settings.view_anchor[1];
     if (x == ox && y == oy) return WRACH_OK;
     // The store takes the simulated particles and keeps those outside the
@@ -1020,7 +1082,7 @@ int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, c
         if (std::isnan(pos_xy[i]) || std::isnan(vel_xy[i]))
             return fail(WRACH_E_NONFINITE, "NaN in particle " + std::to_string(i / 2));
     WRACH_HIP(hipSetDevice(c->cfg.device));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     float2 *p = nullptr, *v = nullptr;
     uint2 *k = nullptr;
     size_t cap = c->dstore_cap;
@@ -1052,7 +1114,7 @@ int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, c
         }
         c->dstore_n = n;
     }
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     if (c->fp.n != n) free_graphs(c);
     c->fp.n = n;
     c->settings.particles_in_frame_count = n;
@@ -1112,7 +1174,7 @@ int wrach_set_tuning(wrach_ctx *c, const wrach_tuning *t) {
     if (t->cells_grid == 0 || t->cells_grid > 4 * c->cells_grid0)
         return fail(WRACH_E_INVALID, "cells grid outside 1.." + std::to_string(4 * c->cells_grid0));
     WRACH_HIP(hipSetDevice(c->cfg.device));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     apply_shapes(c, (uint32_t)scan, (uint32_t)scatter, t->cells_grid);
     return WRACH_OK;
 }
@@ -1121,7 +1183,7 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
     if (frames == 0) frames = 1;
     WRACH_HIP(hipSetDevice(c->cfg.device));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     const uint32_t n = c->fp.n;
     if (n == 0) {  // nothing to time: an empty world does not change
         if (out) *out = current_tuning(c);
@@ -1261,7 +1323,7 @@ int wrach_read(wrach_ctx *c, float *pos_xy, float *vel_xy, size_t particle_cap,
     if (vel_xy && n)
         WRACH_HIP(hipMemcpyAsync(vel_xy, v.velocities, (size_t)n * 2 * sizeof(float),
                                  hipMemcpyDeviceToHost, c->stream));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     if (n_particles) *n_particles = n;
     return WRACH_OK;
 }
@@ -1284,7 +1346,7 @@ int wrach_read_world(wrach_ctx *c, uint32_t world, float *pos_xy, float *vel_xy,
     std::vector<uint32_t> g(wc + 1);
     WRACH_HIP(hipMemcpyAsync(g.data(), v.indices + 1 + (size_t)world * wc, (wc + 1) * sizeof(uint32_t),
                              hipMemcpyDeviceToHost, c->stream));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     if (g[0] > g[wc] || g[wc] > c->fp.n)
         return fail(WRACH_E_STATE, "world " + std::to_string(world) + ": offsets outside the particles");
     const uint32_t off = g[0], n = g[wc] - g[0];
@@ -1301,7 +1363,7 @@ int wrach_read_world(wrach_ctx *c, uint32_t world, float *pos_xy, float *vel_xy,
     if (vel_xy && n)
         WRACH_HIP(hipMemcpyAsync(vel_xy, v.velocities + 2 * (size_t)off, (size_t)n * 2 * sizeof(float),
                                  hipMemcpyDeviceToHost, c->stream));
-    WRACH_HIP(hipStreamSynchronize(c->stream));
+    WRACH_SETTLE(c);
     if (n_particles) *n_particles = n;
     return WRACH_OK;
 }

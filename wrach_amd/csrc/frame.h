@@ -26,7 +26,22 @@ struct FrameParams {
     // A batch of equal worlds: world w owns cell keys [w * world_cells,
     // (w + 1) * world_cells); one world: world_cells == ncell.
     uint32_t world_cells;
+    // The scan's look-back gives up after wait_limit rounds without finding
+    // its prefix and raises kErrLookBack in FrameBuffers::error (the host
+    // turns that into WRACH_E_DEVICE at its next synchronisation).
+    // epoch_skew is 0 except in a test that forces that path from the host:
+    // the look-back then expects an epoch no status word carries.
+    uint32_t wait_limit;
+    uint32_t epoch_skew;
 };
+
+// Look-back rounds before the scan gives up.  A round is a few L2 round
+// trips (about a microsecond); a legitimate wait is bounded by the time the
+// earlier tiles, dispatched first, take to finish (under a millisecond for
+// the largest grids here), so this only ends a wait that would never end.
+constexpr uint32_t kDefaultWaitLimit = 1u << 20;
+// Bits of FrameBuffers::error.
+constexpr uint32_t kErrLookBack = 1u;
 
 // Device buffers of one world.  `idx` has ncell + 2 entries and is placed 3
 // words past a 256-byte boundary so that &idx[1] is 16-byte aligned (the
@@ -48,6 +63,7 @@ struct FrameBuffers {
     uint8_t *chunk_next, *chunk_live;
     uint32_t *chunk_prefix;
     uint32_t *epoch;      // frame counter tagging the scan's status words; the scatter advances it
+    uint32_t *error;      // kErr* bits raised by a kernel that gave up a wait; zero otherwise
     uint4 *cells;         // queued cells (s, n, cell, 0), one region per size class
     uint32_t *class_ctr;  // fill of every slice and overflow list (ctr_index); reset by the scan
     uint32_t class_off[8];  // first entry of each class's region in `cells`

@@ -1382,12 +1382,22 @@ __device__ __forceinline__ unsigned long long status_word(unsigned long long fla
 }
 
 // Exclusive prefix of `tile` from its predecessors' status words, by one
-// wavefront.  Lane l of window w reads tile j - 64w - l.
-__device__ uint32_t look_back(const FrameBuffers &b, uint32_t tile, uint32_t epoch) {
+// wavefront.  Lane l of window w reads tile j - 64w - l.  A tile waits only
+// on tiles dispatched before it, so the wait ends; should it not within
+// p.wait_limit rounds, the tile raises kErrLookBack and takes 0 (the frame
+// kernels clamp every offset range to the particles, so what follows stays
+// in bounds, and the host reports the error).
+__device__ uint32_t look_back(const FrameParams &p, const FrameBuffers &b, uint32_t tile,
+                              uint32_t epoch) {
     const uint32_t lane = lane_id();
     uint32_t excl = 0;
     int64_t j = (int64_t)tile - 1;
-    while (true) {
+    epoch = (epoch + p.epoch_skew) & kEpochMask;
+    for (uint32_t round = 0;; ++round) {
+        if (round >= p.wait_limit) {
+            if (lane == 0) atomicOr(b.error, kErrLookBack);
+            return 0u;
+        }
         unsigned long long w[kLookWindows];
 #pragma unroll
         for (int q = 0; q < kLookWindows; ++q) {
@@ -1499,7 +1509,7 @@ __global__ __launch_bounds__(THREADS) void scan_kernel(FrameParams p, FrameBuffe
             if (lane == 0)
                 __hip_atomic_store(&b.tile_state[tile], status_word(kFlagAggregate, epoch, agg),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            excl = look_back(b, tile, epoch);
+            excl = look_back(p, b, tile, epoch);
             if (lane == 0)
                 __hip_atomic_store(&b.tile_state[tile],
                                    status_word(kFlagInclusive, epoch, excl + agg),
