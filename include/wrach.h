@@ -256,6 +256,9 @@ int wrach_set_view_anchor(wrach_ctx *ctx, float x, float y);
  * WRACH_STENCIL_BUDGET particles is not displaced by the passes. */
 #define WRACH_SOLVER_R 0u
 #define WRACH_SOLVER_S 1u
+/* WRACH_SOLVER_L: the learned warm start before the passes of WRACH_SOLVER_S
+ * (see "The learned warm start" below). */
+#define WRACH_SOLVER_L 2u
 /* OR-ed into WRACH_SOLVER_S: no measure after the passes (they are the
  * same), so the report's "after" figures are not taken (after_measured 0). */
 #define WRACH_SOLVER_QUIET 0x100u
@@ -312,6 +315,95 @@ int wrach_prepare(wrach_ctx *ctx, uint32_t nframes);
  * that path from the host; all offsets stay clamped to the particles, so the
  * frames that follow stay in bounds.  Production code never calls this. */
 int wrach_test_scan_wait(wrach_ctx *ctx, uint32_t wait_limit, uint32_t epoch_skew);
+
+/* The learned warm start (an extension with no counterpart in the
+ * reference, whose separation is one Gauss-Seidel sweep per cell,
+ * shaders/physics/src/particles.rs:62-93).
+ *
+ * Features.  For each particle, the 16 nearest other particles of the 5x5
+ * cells around its cell, ordered by squared distance and then by the bits
+ * of their (x, y, vx, vy), so that the choice does not depend on the order
+ * of particles inside a cell.  Row entries 4r..4r+3 hold the r-th nearest's
+ * (x_j - x_i, y_j - y_i, vx_j - vx_i, vy_j - vy_i) rounded to bf16 (nearest
+ * even); missing neighbours, and particles whose 5x5 cells hold more than
+ * WRACH_LEARN_BUDGET particles, give zeros.
+ *
+ * Network.  64 -> 128 -> 128 -> 2, SiLU after the two hidden layers:
+ * z1 = x W1 + b1, a1 = SiLU(z1), z2 = bf16(a1) W2 + b2, a2 = SiLU(z2),
+ * y = a2 W3 + b3, with W1, W2, W3 rounded to bf16 and f32 accumulation (the
+ * hidden layers on bf16 MFMA).  Parameters: one f32 vector of
+ * WRACH_LEARN_PARAMS entries, W1[64][128] (input-major), b1[128],
+ * W2[128][128], b2[128], W3[128][2], b3[2].  The output predicts the
+ * displacement the plain separation passes would give a particle.
+ *
+ * WRACH_SOLVER_L with k passes (wrach_set_solver, tolerance 0): features
+ * of the frame start, then position + y (each component clamped to +-1),
+ * then k passes of WRACH_SOLVER_S from there, then the reference step.  The
+ * plain passes are the fallback: when the warm start leaves more mean
+ * overlap than the frame start had, the second pass restarts from the
+ * frame start (with k >= 2).  wrach_solver_stats reports the passes (its
+ * "before" maximum is the warm start's); wrach_learned_stats the means. */
+#define WRACH_LEARN_FEATURES 64u
+#define WRACH_LEARN_PARAMS 25090u
+#define WRACH_LEARN_BUDGET 4096u
+
+/* Sets the network's parameters (WRACH_LEARN_PARAMS floats, the layout
+ * above) and resets Adam's moments and step count.  WRACH_E_NONFINITE for
+ * a NaN or infinity. */
+int wrach_learn_set_params(wrach_ctx *ctx, const float *params, size_t n);
+
+/* Copies the current parameters out (n >= WRACH_LEARN_PARAMS). */
+int wrach_learn_get_params(wrach_ctx *ctx, float *params, size_t n);
+
+/* Feature rows (64 bf16 bit patterns each) of the current state in
+ * canonical packed order, as wrach_read orders the particles; rows may be
+ * NULL.  overlap (may be NULL) receives the overlap sum (1 - d in 2^-24
+ * units) and the number of overlapping pairs at these positions, as
+ * wrach_solver_stats counts them.  Synchronises. */
+int wrach_learn_features(wrach_ctx *ctx, uint16_t *rows, size_t rows_cap, uint64_t overlap[2]);
+
+/* The network on n given feature rows: y[2i], y[2i+1] for row i. */
+int wrach_learn_forward(wrach_ctx *ctx, const uint16_t *rows, uint32_t n, float *y);
+
+/* The gradient of the loss sum_i |y_i - t_i|^2 / (2n) over n rows (n a
+ * positive multiple of 64) with respect to the parameters, and the loss.
+ * The parameters do not change. */
+int wrach_learn_gradient(wrach_ctx *ctx, const uint16_t *rows, const float *targets, uint32_t n,
+                         float *grad, float *loss);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with a given gradient:
+ * m = b1 m + (1 - b1) g; v = b2 v + (1 - b2) g^2; w -= lr (m c1) /
+ * (sqrt(v c2) + eps), c1 = 1 / (1 - b1^t), c2 = 1 / (1 - b2^t), each a
+ * rounded f32 operation. */
+int wrach_learn_adam(wrach_ctx *ctx, const float *grad, float lr);
+
+/* Appends the current state's particles to the training set: their
+ * feature rows and, as targets, the displacement that target_passes
+ * passes of WRACH_SOLVER_S give them (the state does not change; the
+ * solver's report does).  *rows_total (may be NULL): the set's size. */
+int wrach_learn_sample(wrach_ctx *ctx, uint32_t target_passes, uint64_t *rows_total);
+
+/* Empties the training set. */
+int wrach_learn_clear_samples(wrach_ctx *ctx);
+
+/* floor(n_order / batch) Adam steps on the device, step s on training rows
+ * order[s*batch .. (s+1)*batch) (batch a positive multiple of 64); losses
+ * (may be NULL) receives each step's loss before its update. */
+int wrach_learn_train(wrach_ctx *ctx, const uint32_t *order, size_t n_order, uint32_t batch,
+                      float lr, float *losses);
+
+/* The last learned frame: overlap sums (2^-24 units) and pairs at the frame
+ * start and after the warm start, and whether it fell back. */
+typedef struct wrach_learned_report {
+    uint64_t start_overlap_sum;
+    uint64_t start_pairs;
+    uint64_t warm_overlap_sum;
+    uint64_t warm_pairs;
+    uint32_t fell_back;
+    uint32_t reserved;
+} wrach_learned_report;
+
+int wrach_learned_stats(wrach_ctx *ctx, wrach_learned_report *out);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *wrach_last_error(void);

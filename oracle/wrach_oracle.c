@@ -503,3 +503,109 @@ int wo_step_s(const wo_params *p, uint32_t *idx, float *pos, float *vel, float *
     free(b);
     return wo_step(p, idx, pos, vel, scratch);
 }
+
+/* ---------------------------------------------------------------------- */
+/* The learned warm start's neighbour features (an extension with no       */
+/* counterpart in the reference).                                          */
+
+static uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, sizeof u);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+static uint32_t f32_bits(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, sizeof u);
+    return u;
+}
+
+typedef struct feat_key {
+    uint32_t k[5];  /* squared distance, x, y, vx, vy (bits, compared unsigned) */
+    uint64_t j;
+} feat_key;
+
+static int feat_less(const feat_key *a, const feat_key *b)
+{
+    for (int t = 0; t < 5; ++t)
+        if (a->k[t] != b->k[t]) return a->k[t] < b->k[t];
+    return 0;
+}
+
+void wo_learn_features(const wo_params *p, const uint32_t *idx, const float *pos, const float *vel,
+                       uint16_t *feat, uint64_t sums[2])
+{
+    uint64_t n = p->particles_in_frame_count;
+    int64_t gx = p->grid_dimensions[0], gy = p->grid_dimensions[1];
+    if (sums) sums[0] = sums[1] = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        float xi = pos[2 * i], yi = pos[2 * i + 1];
+        int64_t k = wo_cell_key(p, xi, yi);
+        uint16_t *row = feat + i * WO_LEARN_FEATURES;
+        memset(row, 0, WO_LEARN_FEATURES * sizeof(uint16_t));
+        if (k < 0) continue;
+        int64_t cx = k % gx, cy = k / gx;
+        uint64_t tot5 = 0, tot3 = 0;
+        for (int64_t y = cy - 2; y <= cy + 2; ++y) {
+            if (y < 0 || y >= gy) continue;
+            for (int64_t x = cx - 2; x <= cx + 2; ++x) {
+                if (x < 0 || x >= gx) continue;
+                int64_t m = y * gx + x;
+                uint64_t c = idx[m + 2] - idx[m + 1];
+                tot5 += c;
+                if (y >= cy - 1 && y <= cy + 1 && x >= cx - 1 && x <= cx + 1) tot3 += c;
+            }
+        }
+        feat_key best[WO_LEARN_NEIGHBOURS];
+        int nb = 0;
+        for (int64_t y = cy - 2; y <= cy + 2; ++y) {
+            if (y < 0 || y >= gy) continue;
+            for (int64_t x = cx - 2; x <= cx + 2; ++x) {
+                if (x < 0 || x >= gx) continue;
+                int inner = y >= cy - 1 && y <= cy + 1 && x >= cx - 1 && x <= cx + 1;
+                int64_t m = y * gx + x;
+                for (uint64_t j = idx[m + 1]; j < idx[m + 2]; ++j) {
+                    if (j == i) continue;
+                    float xj = pos[2 * j], yj = pos[2 * j + 1];
+                    if (sums && inner && tot3 <= WO_STENCIL_BUDGET) {
+                        float ex, ey, ov;
+                        if (stencil_pair(xi, yi, xj, yj, &ex, &ey, &ov)) {
+                            sums[0] += (uint64_t)llrintf(ov * WO_OVERLAP_SCALE);
+                            sums[1] += 1;
+                        }
+                    }
+                    if (tot5 > WO_LEARN_BUDGET) continue;
+                    float dx = xj - xi;
+                    float dy = yj - yi;
+                    float sq = dx * dx;
+                    sq = sq + dy * dy;
+                    feat_key c;
+                    c.k[0] = f32_bits(sq);
+                    c.k[1] = f32_bits(xj);
+                    c.k[2] = f32_bits(yj);
+                    c.k[3] = f32_bits(vel[2 * j]);
+                    c.k[4] = f32_bits(vel[2 * j + 1]);
+                    c.j = j;
+                    /* insertion into the ascending list of the nb nearest */
+                    if (nb == WO_LEARN_NEIGHBOURS && !feat_less(&c, &best[nb - 1])) continue;
+                    int at = nb < WO_LEARN_NEIGHBOURS ? nb++ : WO_LEARN_NEIGHBOURS - 1;
+                    while (at > 0 && feat_less(&c, &best[at - 1])) {
+                        best[at] = best[at - 1];
+                        --at;
+                    }
+                    best[at] = c;
+                }
+            }
+        }
+        for (int r = 0; r < nb; ++r) {
+            uint64_t j = best[r].j;
+            row[4 * r + 0] = bf16_rne(pos[2 * j] - xi);
+            row[4 * r + 1] = bf16_rne(pos[2 * j + 1] - yi);
+            row[4 * r + 2] = bf16_rne(vel[2 * j] - vel[2 * i]);
+            row[4 * r + 3] = bf16_rne(vel[2 * j + 1] - vel[2 * i + 1]);
+        }
+    }
+}

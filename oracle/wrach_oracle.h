@@ -39,6 +39,10 @@
  *                        overlap equal to the all-pairs maximum
  *   wo_step_s            zero passes equal wo_step; a tolerance above the
  *                        first overlap runs no pass
+ *   wo_learn_features    brute force (numpy: every particle's distances to all
+ *                        others, Chebyshev cell distance <= 2, a lexsort); the
+ *                        sums equal wo_stencil_pass's; order inside a cell
+ *                        does not matter (permuted cells give permuted rows)
  */
 #ifndef WRACH_ORACLE_H
 #define WRACH_ORACLE_H
@@ -176,6 +180,21 @@ float wo_stencil_pass(const wo_params *p, const uint32_t *indices, const float *
 int wo_step_s(const wo_params *p, uint32_t *indices, float *pos, float *vel, float *scratch,
               uint32_t max_passes, float overlap_tol, float overlap[2], uint32_t *passes,
               uint64_t sums[4]);
+
+/* The learned warm start's neighbour features (an extension): for each
+ * particle i of the packed state, the other particles of the 5x5 cells
+ * around its cell (wo_cell_key of its position), the 16 nearest by squared
+ * distance, then by the bits of (x, y, vx, vy) as unsigned integers; row i
+ * of feat holds, for the r-th, (x_j - x_i, y_j - y_i, vx_j - vx_i,
+ * vy_j - vy_i) rounded to bf16 (nearest even) at 4r..4r+3, zeros past the
+ * last.  More than WO_LEARN_BUDGET particles in the 5x5 cells: a zero row.
+ * sums (if not NULL): the overlap sum and pair count of wo_stencil_pass at
+ * these positions (its 3x3 cells and budget). */
+#define WO_LEARN_NEIGHBOURS 16
+#define WO_LEARN_FEATURES 64
+#define WO_LEARN_BUDGET 4096
+void wo_learn_features(const wo_params *p, const uint32_t *indices, const float *pos,
+                       const float *vel, uint16_t *feat, uint64_t sums[2]);
 
 #ifdef __cplusplus
 }

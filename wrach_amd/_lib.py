@@ -22,6 +22,10 @@ WRACH_E_STATE = -7
 WRACH_FLAG_HOST_PACK = 0x1
 WRACH_SOLVER_R = 0
 WRACH_SOLVER_S = 1
+WRACH_SOLVER_L = 2
+WRACH_LEARN_FEATURES = 64
+WRACH_LEARN_PARAMS = 25090
+WRACH_LEARN_BUDGET = 4096
 WRACH_SOLVER_QUIET = 0x100
 WRACH_MAX_PASSES = 64
 WRACH_STENCIL_BUDGET = 4096
@@ -56,6 +60,16 @@ EXPORTS = (
     "wrach_solver_stats",
     "wrach_step_profiled",
     "wrach_prepare",
+    "wrach_learn_set_params",
+    "wrach_learn_get_params",
+    "wrach_learn_features",
+    "wrach_learn_forward",
+    "wrach_learn_gradient",
+    "wrach_learn_adam",
+    "wrach_learn_sample",
+    "wrach_learn_clear_samples",
+    "wrach_learn_train",
+    "wrach_learned_stats",
     "wrach_test_scan_wait",
     "wrach_last_error",
 )
@@ -82,6 +96,17 @@ class WorldSettingsC(ctypes.Structure):
         ("grid_dimensions", ctypes.c_uint32 * 2),
         ("cell_size", ctypes.c_uint32),
         ("particles_in_frame_count", ctypes.c_uint32),
+    ]
+
+
+class LearnedReportC(ctypes.Structure):
+    _fields_ = [
+        ("start_overlap_sum", ctypes.c_uint64),
+        ("start_pairs", ctypes.c_uint64),
+        ("warm_overlap_sum", ctypes.c_uint64),
+        ("warm_pairs", ctypes.c_uint64),
+        ("fell_back", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -171,6 +196,18 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_solver_stats.argtypes = [vp, ctypes.POINTER(SolverReportC)]
     L.wrach_step_profiled.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
     L.wrach_prepare.argtypes = [vp, ctypes.c_uint32]
+    u16p = ctypes.POINTER(ctypes.c_uint16)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.wrach_learn_set_params.argtypes = [vp, f32p, ctypes.c_size_t]
+    L.wrach_learn_get_params.argtypes = [vp, f32p, ctypes.c_size_t]
+    L.wrach_learn_features.argtypes = [vp, u16p, ctypes.c_size_t, u64p]
+    L.wrach_learn_forward.argtypes = [vp, u16p, ctypes.c_uint32, f32p]
+    L.wrach_learn_gradient.argtypes = [vp, u16p, f32p, ctypes.c_uint32, f32p, f32p]
+    L.wrach_learn_adam.argtypes = [vp, f32p, ctypes.c_float]
+    L.wrach_learn_sample.argtypes = [vp, ctypes.c_uint32, u64p]
+    L.wrach_learn_clear_samples.argtypes = [vp]
+    L.wrach_learn_train.argtypes = [vp, u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_float, f32p]
+    L.wrach_learned_stats.argtypes = [vp, ctypes.POINTER(LearnedReportC)]
     L.wrach_test_scan_wait.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.wrach_last_error.argtypes = []
     L.wrach_last_error.restype = ctypes.c_char_p

@@ -214,11 +214,13 @@ class World:
 
     def set_solver(self, mode: str = "R", max_passes: int = 0, overlap_tol: float = 0.0,
                    quiet: bool = False) -> None:
-        """wrach_set_solver: "R", the reference's step, or "S", up to
+        """wrach_set_solver: "R", the reference's step; "S", up to
         max_passes separation passes over the 3x3 cells around each particle
-        (each while the largest overlap exceeds overlap_tol), then that step.
-        quiet (mode S): no measure after the passes (WRACH_SOLVER_QUIET)."""
-        modes = {"R": _lib.WRACH_SOLVER_R, "S": _lib.WRACH_SOLVER_S}
+        (each while the largest overlap exceeds overlap_tol), then that step;
+        "L", the learned warm start, then max_passes such passes (tolerance
+        0), then that step.  quiet (S, L): no measure after the passes
+        (WRACH_SOLVER_QUIET)."""
+        modes = {"R": _lib.WRACH_SOLVER_R, "S": _lib.WRACH_SOLVER_S, "L": _lib.WRACH_SOLVER_L}
         if mode not in modes:
             raise ValueError(f"unknown solver {mode!r}")
         m = modes[mode] | (_lib.WRACH_SOLVER_QUIET if quiet else 0)
@@ -239,6 +241,77 @@ class World:
         check(self._L.wrach_solver_stats(self._h, ctypes.byref(r)))
         return (r.max_overlap_before, r.max_overlap_after, r.passes,
                 (r.overlap_sum_before, r.pairs_before, r.overlap_sum_after, r.pairs_after))
+
+    # -------------------------------------------------- learned warm start
+
+    def learn_set_params(self, params) -> None:
+        """wrach_learn_set_params: the network's f32 parameter vector
+        (resets Adam)."""
+        params = np.ascontiguousarray(params, np.float32).ravel()
+        check(self._L.wrach_learn_set_params(self._h, params.ctypes.data_as(_f32p), len(params)))
+
+    def learn_params(self) -> np.ndarray:
+        out = np.zeros(_lib.WRACH_LEARN_PARAMS, np.float32)
+        check(self._L.wrach_learn_get_params(self._h, out.ctypes.data_as(_f32p), len(out)))
+        return out
+
+    def learn_features(self):
+        """(feature rows as uint16 bf16 bits [N, 64] in canonical order,
+        (overlap sum in 2^-24 units, pairs)) of the current state."""
+        n = self.n_particles
+        rows = np.zeros((max(n, 1), _lib.WRACH_LEARN_FEATURES), np.uint16)
+        ov = np.zeros(2, np.uint64)
+        check(self._L.wrach_learn_features(self._h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                           rows.shape[0], ov.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return rows[:n], (int(ov[0]), int(ov[1]))
+
+    def learn_forward(self, rows) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, np.uint16).reshape(-1, _lib.WRACH_LEARN_FEATURES)
+        y = np.zeros((len(rows), 2), np.float32)
+        check(self._L.wrach_learn_forward(self._h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                          len(rows), y.ctypes.data_as(_f32p)))
+        return y
+
+    def learn_gradient(self, rows, targets):
+        rows = np.ascontiguousarray(rows, np.uint16).reshape(-1, _lib.WRACH_LEARN_FEATURES)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 2)
+        grad = np.zeros(_lib.WRACH_LEARN_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        check(self._L.wrach_learn_gradient(self._h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                           targets.ctypes.data_as(_f32p), len(rows),
+                                           grad.ctypes.data_as(_f32p), ctypes.byref(loss)))
+        return grad, loss.value
+
+    def learn_adam(self, grad, lr: float) -> None:
+        grad = np.ascontiguousarray(grad, np.float32).ravel()
+        check(self._L.wrach_learn_adam(self._h, grad.ctypes.data_as(_f32p), float(lr)))
+
+    def learn_sample(self, target_passes: int) -> int:
+        """Appends the state's features and target_passes-pass displacements
+        to the training set; returns its size."""
+        total = ctypes.c_uint64()
+        check(self._L.wrach_learn_sample(self._h, int(target_passes), ctypes.byref(total)))
+        return total.value
+
+    def learn_clear_samples(self) -> None:
+        check(self._L.wrach_learn_clear_samples(self._h))
+
+    def learn_train(self, order, batch: int, lr: float) -> np.ndarray:
+        """Adam steps over consecutive batches of `order` (training rows);
+        returns each step's loss."""
+        order = np.ascontiguousarray(order, np.uint32)
+        steps = len(order) // int(batch)
+        losses = np.zeros(max(steps, 1), np.float32)
+        check(self._L.wrach_learn_train(self._h, order.ctypes.data_as(_u32p), len(order), int(batch),
+                                        float(lr), losses.ctypes.data_as(_f32p)))
+        return losses[:steps]
+
+    def learned_stats(self) -> dict:
+        r = _lib.LearnedReportC()
+        check(self._L.wrach_learned_stats(self._h, ctypes.byref(r)))
+        return {"start_overlap_sum": r.start_overlap_sum, "start_pairs": r.start_pairs,
+                "warm_overlap_sum": r.warm_overlap_sum, "warm_pairs": r.warm_pairs,
+                "fell_back": bool(r.fell_back)}
 
     def set_graph(self, enable: bool) -> None:
         check(self._L.wrach_set_graph(self._h, 1 if enable else 0))

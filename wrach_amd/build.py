@@ -21,7 +21,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libwrach_hip.so")
-SOURCES = ["kernels.hip", "abi.hip"]
+SOURCES = ["kernels.hip", "learn.hip", "abi.hip"]
 ARCH = os.environ.get("WRACH_OFFLOAD_ARCH", "gfx950")
 
 

@@ -5,6 +5,7 @@
 // compute worker (runners/bevy/src/compute/builder.rs:70-84), one HIP stream
 // and the captured frame graphs.
 #include "frame.h"
+#include "learn.h"
 #include "wrach.h"
 
 #include <algorithm>
@@ -110,6 +111,27 @@ struct wrach_ctx {
     uint32_t *swords = nullptr;  // wrach::kStencilWords words
     unsigned long long *ssums = nullptr;  // wrach::kStencilSumEntries entries
     uint32_t *err_host = nullptr;  // pinned copy of buf.error, read at each synchronisation
+    // The learned warm start (wrach_learn_*, WRACH_SOLVER_L): the network's
+    // master parameters, gradient and Adam moments (f32, wrach::kParams
+    // each), its packed weights, the frame's feature rows and a third
+    // position buffer for the passes after the warm start.
+    float *lparams = nullptr, *lgrad = nullptr, *lm = nullptr, *lv = nullptr, *lw3 = nullptr;
+    uint16_t *lfrag = nullptr;
+    uint32_t adam_t = 0;
+    uint16_t *lfeat = nullptr;
+    float2 *lbuf = nullptr;
+    // Training set (rows of features and target displacements) and the
+    // workspace of one batch.
+    uint16_t *tfeat = nullptr;
+    float2 *ttarget = nullptr;
+    size_t trows = 0, tcap = 0;
+    uint32_t *tsel = nullptr;
+    size_t tsel_cap = 0;
+    uint32_t tbsz = 0;
+    uint16_t *txt = nullptr, *ta1t = nullptr, *tdz1t = nullptr, *tdz2t = nullptr;
+    float *ta2t = nullptr, *tdyt = nullptr, *tsq = nullptr, *tpart1 = nullptr, *tpart2 = nullptr;
+    float *tloss = nullptr;  // one loss per step of a training call
+    size_t tloss_cap = 0;
 };
 
 namespace {
@@ -189,7 +211,93 @@ int enqueue_stencil(wrach_ctx *c) {
     return WRACH_OK;
 }
 
+wrach::MlpWeights mlp_weights(const wrach_ctx *c) {
+    wrach::MlpWeights w{};
+    w.frag = c->lfrag;
+    w.w3 = c->lw3;
+    w.params = c->lparams;
+    return w;
+}
+
+// The learned frame (WRACH_SOLVER_L), before the reference step.  With X =
+// pos (the frame start), Y = pos_out and Z = lbuf:
+//   features of X (and each particle's cell, and X's overlap sums, set 0);
+//   the network's warm start X -> Y;
+//   pass 1 Y -> Z (its input's sums: set 2);
+//   pass 2 Z -> Y, reading X instead of Z when the warm start left more mean
+//     overlap than X had (the fallback to the plain passes);
+//   the other passes end in X (a copy when they cannot).
+// One pass: Y -> X, no fallback.  No pass: the warm start, copied to X.
+int enqueue_learned(wrach_ctx *c) {
+    if (c->solver != WRACH_SOLVER_L || c->fp.n == 0) return WRACH_OK;
+    const uint32_t k = c->passes;
+    float2 *X = c->buf.pos, *Y = c->buf.pos_out, *Z = c->lbuf;
+    WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->ssums, c->stream));
+    wrach::FeatureArgs f{};
+    f.pos = X;
+    f.vel = c->buf.vel;
+    f.feat = c->lfeat;
+    f.row0 = 0;
+    f.cell = c->scell;
+    f.sums = c->ssums;  // set 0: the frame start
+    WRACH_HIP(wrach::launch_learn_features(c->fp, c->buf, f, c->stream));
+    wrach::InferArgs in{};
+    in.feat = c->lfeat;
+    in.n = c->fp.n;
+    in.pin = X;
+    in.pout = Y;
+    WRACH_HIP(wrach::launch_learn_infer(mlp_weights(c), in, c->stream));
+    std::vector<std::pair<float2 *, float2 *>> plan;
+    float2 *cur = Y;
+    if (k == 1) {
+        plan.push_back({Y, X});
+        cur = X;
+    } else if (k >= 2) {
+        plan.push_back({Y, Z});
+        plan.push_back({Z, Y});
+        uint32_t r = k - 2;
+        if (r > 0 && r % 2 == 0) {
+            plan.push_back({Y, Z});
+            plan.push_back({Z, X});
+            cur = X;
+            r -= 2;
+        }
+        for (; r > 0; --r) {
+            float2 *nx = cur == X ? Y : X;
+            plan.push_back({cur, nx});
+            cur = nx;
+        }
+    }
+    wrach::StencilArgs a{};
+    a.cell = c->scell;
+    a.words = c->swords;
+    a.sums = c->ssums;
+    a.tol = 0.0f;
+    a.first = 0;  // the cells came from the features
+    for (uint32_t j = 0; j < plan.size(); ++j) {
+        a.pin = plan[j].first;
+        a.pout = plan[j].second;
+        a.in_slot = j;
+        a.check = 0;
+        a.stat = j == 0 ? 3 : 0;  // set 2: the warm start
+        a.pin_alt = j == 1 ? X : nullptr;
+        WRACH_HIP(wrach::launch_stencil_move(c->fp, c->buf, a, c->stream));
+    }
+    if (cur != X) WRACH_HIP(wrach::launch_copy_pos(X, cur, c->fp.n, c->stream));
+    if (!c->quiet) {
+        a.pin = X;
+        a.pout = nullptr;
+        a.in_slot = k;
+        a.stat = 2;  // set 1: after the passes
+        a.pin_alt = nullptr;
+        WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
+    }
+    return WRACH_OK;
+}
+
 int enqueue_frame(wrach_ctx *c) {
+    const int learned = enqueue_learned(c);
+    if (learned) return learned;
     const int rc = enqueue_stencil(c);
     if (rc) return rc;
     WRACH_HIP(wrach::launch_collide(c->fp, c->buf, c->stream));
@@ -246,6 +354,12 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->scell);
     (void)hipFree(c->swords);
     (void)hipFree(c->ssums);
+    for (void *q : {(void *)c->lparams, (void *)c->lgrad, (void *)c->lm, (void *)c->lv, (void *)c->lw3,
+                    (void *)c->lfrag, (void *)c->lfeat, (void *)c->lbuf, (void *)c->tfeat,
+                    (void *)c->ttarget, (void *)c->tsel, (void *)c->txt, (void *)c->ta1t,
+                    (void *)c->tdz1t, (void *)c->tdz2t, (void *)c->ta2t, (void *)c->tdyt, (void *)c->tsq,
+                    (void *)c->tpart1, (void *)c->tpart2, (void *)c->tloss})
+        (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -757,6 +871,8 @@ int wrach_step_profiled(wrach_ctx *c, uint32_t nframes, double *kernel_ms) {
         hipError_t (*const launch[K])(const wrach::FrameParams &, const wrach::FrameBuffers &,
                                       hipStream_t, hipEvent_t, hipEvent_t) = {
             wrach::launch_collide, wrach::launch_cells, wrach::launch_scan, wrach::launch_scatter};
+        rc = enqueue_learned(c);  // the learned warm start and its passes, untimed
+        if (rc) break;
         rc = enqueue_stencil(c);  // mode S's passes, untimed
         if (rc) break;
         hipError_t e = hipSuccess;
@@ -796,42 +912,84 @@ int wrach_sync(wrach_ctx *c) {
 static_assert(WRACH_MAX_PASSES == wrach::kMaxStencilPasses, "one pass limit");
 static_assert(WRACH_STENCIL_BUDGET == wrach::kStencilBudget, "one budget");
 
+namespace {
+
+// The separation passes' buffers (mode S and L, and the learned samples):
+// each on its own, so that a failed allocation leaves the context as it
+// was and the next call allocates what is still missing.
+int ensure_stencil(wrach_ctx *c) {
+    if (!c->scell) {
+        uint32_t *cell = nullptr;
+        WRACH_HIP(dmalloc(&cell, c->capacity ? c->capacity : 1));
+        c->scell = cell;
+    }
+    if (!c->swords) {
+        uint32_t *words = nullptr;
+        WRACH_HIP(dmalloc(&words, wrach::kStencilWords));
+        WRACH_HIP(hipMemset(words, 0, wrach::kStencilWords * sizeof(uint32_t)));
+        c->swords = words;
+    }
+    if (!c->ssums) {
+        unsigned long long *sums = nullptr;
+        WRACH_HIP(dmalloc(&sums, wrach::kStencilSumEntries));
+        WRACH_HIP(hipMemset(sums, 0, wrach::kStencilSumEntries * sizeof(unsigned long long)));
+        c->ssums = sums;
+    }
+    return WRACH_OK;
+}
+
+// The learned frame's buffers: feature rows and a third position buffer.
+int ensure_learned_frame(wrach_ctx *c) {
+    const size_t rows = c->capacity ? c->capacity : 1;
+    if (!c->lfeat) {
+        uint16_t *f = nullptr;
+        WRACH_HIP(dmalloc(&f, rows * wrach::kFeat));
+        c->lfeat = f;
+    }
+    if (!c->lbuf) {
+        float2 *b = nullptr;
+        WRACH_HIP(dmalloc(&b, rows));
+        c->lbuf = b;
+    }
+    return WRACH_OK;
+}
+
+}  // namespace
+
 int wrach_set_solver(wrach_ctx *c, uint32_t mode, uint32_t max_passes, float overlap_tol) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
-    const bool quiet = (mode & WRACH_SOLVER_QUIET) != 0 && (mode & ~WRACH_SOLVER_QUIET) == WRACH_SOLVER_S;
-    if (quiet) mode = WRACH_SOLVER_S;
-    if (mode != WRACH_SOLVER_R && mode != WRACH_SOLVER_S) return fail(WRACH_E_INVALID, "no such solver");
-    if (mode == WRACH_SOLVER_S) {
+    const uint32_t base = mode & ~WRACH_SOLVER_QUIET;
+    const bool quiet = (mode & WRACH_SOLVER_QUIET) != 0 && (base == WRACH_SOLVER_S || base == WRACH_SOLVER_L);
+    if (quiet) mode = base;
+    if (mode != WRACH_SOLVER_R && mode != WRACH_SOLVER_S && mode != WRACH_SOLVER_L)
+        return fail(WRACH_E_INVALID, "no such solver");
+    if (mode != WRACH_SOLVER_R) {
         if (max_passes > wrach::kMaxStencilPasses)
             return fail(WRACH_E_INVALID, "more than " + std::to_string(wrach::kMaxStencilPasses) + " passes");
         if (!(overlap_tol >= 0.0f) || std::isinf(overlap_tol))
             return fail(WRACH_E_INVALID, "overlap tolerance must be finite and >= 0");
     }
+    if (mode == WRACH_SOLVER_L) {
+        if (overlap_tol != 0.0f)
+            return fail(WRACH_E_INVALID, "the learned solver runs a fixed number of passes (tolerance 0)");
+        if (!c->lparams) return fail(WRACH_E_STATE, "the learned solver needs parameters (wrach_learn_set_params)");
+    }
     WRACH_HIP(hipSetDevice(c->cfg.device));
     WRACH_SETTLE(c);
-    // Each buffer on its own: a failed allocation leaves the solver as it was,
-    // and the next call allocates what is still missing.
-    if (mode == WRACH_SOLVER_S && !c->scell) {
-        uint32_t *cell = nullptr;
-        WRACH_HIP(dmalloc(&cell, c->capacity ? c->capacity : 1));
-        c->scell = cell;
+    if (mode != WRACH_SOLVER_R) {
+        const int rc = ensure_stencil(c);
+        if (rc) return rc;
     }
-    if (mode == WRACH_SOLVER_S && !c->swords) {
-        uint32_t *words = nullptr;
-        WRACH_HIP(dmalloc(&words, wrach::kStencilWords));
-        c->swords = words;
-    }
-    if (mode == WRACH_SOLVER_S && !c->ssums) {
-        unsigned long long *sums = nullptr;
-        WRACH_HIP(dmalloc(&sums, wrach::kStencilSumEntries));
-        c->ssums = sums;
+    if (mode == WRACH_SOLVER_L) {
+        const int rc = ensure_learned_frame(c);
+        if (rc) return rc;
     }
     if (c->swords) WRACH_HIP(hipMemset(c->swords, 0, wrach::kStencilWords * sizeof(uint32_t)));
     if (c->ssums) WRACH_HIP(hipMemset(c->ssums, 0, wrach::kStencilSumEntries * sizeof(unsigned long long)));
     free_graphs(c);  // captured with the previous solver
     c->solver = mode;
     c->quiet = quiet;
-    c->passes = mode == WRACH_SOLVER_S ? max_passes : 0;
+    c->passes = mode != WRACH_SOLVER_R ? max_passes : 0;
     c->overlap_tol = mode == WRACH_SOLVER_S ? overlap_tol : 0.0f;
     g_err.clear();
     return WRACH_OK;
@@ -869,7 +1027,8 @@ int wrach_solver_stats(wrach_ctx *c, wrach_solver_report *out) {
     r.pairs_before = acc[1];
     r.overlap_sum_after = acc[2];
     r.pairs_after = acc[3];
-    r.after_measured = c->solver == WRACH_SOLVER_S && c->passes > 0 && !c->quiet ? 1u : 0u;
+    r.after_measured = c->solver != WRACH_SOLVER_R && (c->passes > 0 || c->solver == WRACH_SOLVER_L) &&
+                               !c->quiet ? 1u : 0u;
     *out = r;
     return WRACH_OK;
 }
@@ -1365,6 +1524,416 @@ int wrach_read_world(wrach_ctx *c, uint32_t world, float *pos_xy, float *vel_xy,
                                  hipMemcpyDeviceToHost, c->stream));
     WRACH_SETTLE(c);
     if (n_particles) *n_particles = n;
+    return WRACH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The learned warm start (no counterpart in the reference).
+
+extern "C++" {
+namespace {
+
+constexpr float kAdamBeta1 = 0.9f, kAdamBeta2 = 0.999f, kAdamEps = 1e-8f;
+
+int ensure_learn(wrach_ctx *c) {
+    if (!c->lparams) {
+        float *q = nullptr;
+        WRACH_HIP(dmalloc(&q, wrach::kParams));
+        WRACH_HIP(hipMemset(q, 0, wrach::kParams * sizeof(float)));
+        c->lparams = q;
+    }
+    for (float **q : {&c->lgrad, &c->lm, &c->lv}) {
+        if (!*q) {
+            float *t = nullptr;
+            WRACH_HIP(dmalloc(&t, wrach::kParams));
+            WRACH_HIP(hipMemset(t, 0, wrach::kParams * sizeof(float)));
+            *q = t;
+        }
+    }
+    if (!c->lw3) {
+        float *t = nullptr;
+        WRACH_HIP(dmalloc(&t, (size_t)wrach::kHid * wrach::kOutDim));
+        c->lw3 = t;
+    }
+    if (!c->lfrag) {
+        uint16_t *t = nullptr;
+        WRACH_HIP(dmalloc(&t, (size_t)wrach::kFrags * wrach::kFragElems));
+        c->lfrag = t;
+    }
+    return WRACH_OK;
+}
+
+// The workspace of a training batch of bsz rows (a multiple of 64).
+int ensure_batch(wrach_ctx *c, uint32_t bsz) {
+    if (c->tbsz >= bsz) return WRACH_OK;
+    for (void *q : {(void *)c->txt, (void *)c->ta1t, (void *)c->tdz1t, (void *)c->tdz2t, (void *)c->ta2t,
+                    (void *)c->tdyt, (void *)c->tsq, (void *)c->tpart1, (void *)c->tpart2})
+        (void)hipFree(q);
+    c->txt = c->ta1t = c->tdz1t = c->tdz2t = nullptr;
+    c->ta2t = c->tdyt = c->tsq = c->tpart1 = c->tpart2 = nullptr;
+    c->tbsz = 0;
+    const size_t b = bsz;
+    const size_t chunks = b / 32;  // the most k chunks a GEMM uses
+    WRACH_HIP(dmalloc(&c->txt, (size_t)wrach::kFeat * b));
+    WRACH_HIP(dmalloc(&c->ta1t, (size_t)wrach::kHid * b));
+    WRACH_HIP(dmalloc(&c->tdz1t, (size_t)wrach::kHid * b));
+    WRACH_HIP(dmalloc(&c->tdz2t, (size_t)wrach::kHid * b));
+    WRACH_HIP(dmalloc(&c->ta2t, (size_t)wrach::kHid * b));
+    WRACH_HIP(dmalloc(&c->tdyt, 2 * b));
+    WRACH_HIP(dmalloc(&c->tsq, b / 16));
+    WRACH_HIP(dmalloc(&c->tpart1, (size_t)wrach::kFeat * wrach::kHid * std::min<size_t>(chunks, 64)));
+    WRACH_HIP(dmalloc(&c->tpart2, (size_t)wrach::kHid * wrach::kHid * std::min<size_t>(chunks, 64)));
+    c->tbsz = bsz;
+    return WRACH_OK;
+}
+
+// k chunk of the weight GEMMs: at most 64 chunks, each a multiple of 32.
+uint32_t gemm_chunk(uint32_t bsz) {
+    uint32_t kc = 32;
+    while (bsz / kc > 64 || bsz % kc) kc *= 2;
+    return kc;
+}
+
+// One training step on the batch rows sel[0..bsz) of (feat, target):
+// gradient into lgrad, the loss into *loss_dev (if not null).
+int enqueue_gradient(wrach_ctx *c, const uint16_t *feat, const float2 *target, const uint32_t *sel,
+                     uint32_t bsz, float *loss_dev) {
+    wrach::TrainArgs t{};
+    t.feat = feat;
+    t.target = target;
+    t.sel = sel;
+    t.bsz = bsz;
+    t.inv_bsz = 1.0f / (float)bsz;
+    t.xt = c->txt;
+    t.a1t = c->ta1t;
+    t.dz1t = c->tdz1t;
+    t.dz2t = c->tdz2t;
+    t.a2t = c->ta2t;
+    t.dyt = c->tdyt;
+    t.sqerr = c->tsq;
+    WRACH_HIP(wrach::launch_learn_rows(mlp_weights(c), t, c->stream));
+    const uint32_t kc = gemm_chunk(bsz), chunks = bsz / kc;
+    WRACH_HIP(wrach::launch_learn_gemm_nt(c->txt, c->tdz1t, wrach::kFeat, wrach::kHid, bsz, kc, chunks,
+                                          c->tpart1, c->stream));
+    WRACH_HIP(wrach::launch_learn_gemm_nt(c->ta1t, c->tdz2t, wrach::kHid, wrach::kHid, bsz, kc, chunks,
+                                          c->tpart2, c->stream));
+    WRACH_HIP(wrach::launch_learn_reduce(t, c->tpart1, c->tpart2, chunks, c->lgrad, loss_dev, c->stream));
+    return WRACH_OK;
+}
+
+int enqueue_adam(wrach_ctx *c, float lr) {
+    c->adam_t += 1;
+    wrach::AdamArgs a{};
+    a.params = c->lparams;
+    a.m = c->lm;
+    a.v = c->lv;
+    a.grad = c->lgrad;
+    a.lr = lr;
+    a.beta1 = kAdamBeta1;
+    a.beta2 = kAdamBeta2;
+    a.eps = kAdamEps;
+    a.c1 = (float)(1.0 / (1.0 - std::pow((double)kAdamBeta1, (double)c->adam_t)));
+    a.c2 = (float)(1.0 / (1.0 - std::pow((double)kAdamBeta2, (double)c->adam_t)));
+    WRACH_HIP(wrach::launch_learn_adam(a, c->stream));
+    WRACH_HIP(wrach::launch_learn_pack(c->lparams, c->lfrag, c->lw3, c->stream));
+    return WRACH_OK;
+}
+
+template <typename T>
+struct DevTemp {  // a device buffer for the length of one call
+    T *p = nullptr;
+    ~DevTemp() { (void)hipFree(p); }
+};
+
+bool valid_batch(uint32_t n) { return n >= 64 && n % 64 == 0 && n <= (1u << 20); }
+
+}  // namespace
+}  // extern "C++"
+
+int wrach_learn_set_params(wrach_ctx *c, const float *params, size_t n) {
+    if (!c || !params) return fail(WRACH_E_INVALID, "null argument");
+    if (n != wrach::kParams)
+        return fail(WRACH_E_INVALID, "the network has " + std::to_string(wrach::kParams) + " parameters");
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return fail(WRACH_E_NONFINITE, "non-finite parameter");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_SETTLE(c);
+    {
+        const int rc = ensure_learn(c);
+        if (rc) return rc;
+    }
+    WRACH_HIP(hipMemcpy(c->lparams, params, n * sizeof(float), hipMemcpyHostToDevice));
+    WRACH_HIP(hipMemset(c->lm, 0, n * sizeof(float)));
+    WRACH_HIP(hipMemset(c->lv, 0, n * sizeof(float)));
+    c->adam_t = 0;
+    WRACH_HIP(wrach::launch_learn_pack(c->lparams, c->lfrag, c->lw3, c->stream));
+    WRACH_SETTLE(c);
+    return WRACH_OK;
+}
+
+int wrach_learn_get_params(wrach_ctx *c, float *params, size_t n) {
+    if (!c || !params) return fail(WRACH_E_INVALID, "null argument");
+    if (n < wrach::kParams) return fail(WRACH_E_BUFFER, "parameter array too small");
+    if (!c->lparams) return fail(WRACH_E_STATE, "no parameters set");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_SETTLE(c);
+    WRACH_HIP(hipMemcpy(params, c->lparams, wrach::kParams * sizeof(float), hipMemcpyDeviceToHost));
+    return WRACH_OK;
+}
+
+int wrach_learn_features(wrach_ctx *c, uint16_t *rows, size_t rows_cap, uint64_t overlap[2]) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    const uint32_t n = c->fp.n;
+    if (rows && rows_cap < n)
+        return fail(WRACH_E_BUFFER, "feature array holds " + std::to_string(rows_cap) + " rows, the frame has " +
+                                        std::to_string(n));
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    {
+        const int rc = ensure_learned_frame(c);
+        if (rc) return rc;
+    }
+    DevTemp<unsigned long long> sums;
+    WRACH_HIP(dmalloc(&sums.p, (size_t)wrach::kStencilSumSlots * wrach::kStencilSumStride));
+    WRACH_HIP(hipMemsetAsync(sums.p, 0,
+                             (size_t)wrach::kStencilSumSlots * wrach::kStencilSumStride * sizeof(unsigned long long),
+                             c->stream));
+    // On the canonical copy, so that row i is canonical particle i (the
+    // features themselves do not depend on the order inside a cell).
+    wrach_device_view v{};
+    {
+        const int rc = wrach_canonical_device(c, &v);
+        if (rc) return rc;
+    }
+    wrach::FeatureArgs f{};
+    f.pos = reinterpret_cast<const float2 *>(v.positions);
+    f.vel = reinterpret_cast<const float2 *>(v.velocities);
+    f.feat = c->lfeat;
+    f.sums = sums.p;
+    WRACH_HIP(wrach::launch_learn_features(c->fp, c->buf, f, c->stream));
+    if (rows && n)
+        WRACH_HIP(hipMemcpyAsync(rows, c->lfeat, (size_t)n * wrach::kFeat * sizeof(uint16_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+    std::vector<unsigned long long> hs((size_t)wrach::kStencilSumSlots * wrach::kStencilSumStride);
+    WRACH_HIP(hipMemcpyAsync(hs.data(), sums.p, hs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    WRACH_SETTLE(c);
+    if (overlap) {
+        overlap[0] = overlap[1] = 0;
+        for (uint32_t slot = 0; slot < wrach::kStencilSumSlots; ++slot) {
+            overlap[0] += hs[(size_t)slot * wrach::kStencilSumStride];
+            overlap[1] += hs[(size_t)slot * wrach::kStencilSumStride + 1];
+        }
+    }
+    return WRACH_OK;
+}
+
+int wrach_learn_forward(wrach_ctx *c, const uint16_t *rows, uint32_t n, float *y) {
+    if (!c || (n && (!rows || !y))) return fail(WRACH_E_INVALID, "null argument");
+    if (!c->lparams) return fail(WRACH_E_STATE, "no parameters set");
+    if (n == 0) return WRACH_OK;
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    DevTemp<uint16_t> f;
+    DevTemp<float2> out;
+    WRACH_HIP(dmalloc(&f.p, (size_t)n * wrach::kFeat));
+    WRACH_HIP(dmalloc(&out.p, (size_t)n));
+    WRACH_HIP(hipMemcpyAsync(f.p, rows, (size_t)n * wrach::kFeat * sizeof(uint16_t), hipMemcpyHostToDevice,
+                             c->stream));
+    wrach::InferArgs in{};
+    in.feat = f.p;
+    in.n = n;
+    in.y_out = out.p;
+    WRACH_HIP(wrach::launch_learn_infer(mlp_weights(c), in, c->stream));
+    WRACH_HIP(hipMemcpyAsync(y, out.p, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    WRACH_SETTLE(c);
+    return WRACH_OK;
+}
+
+int wrach_learn_gradient(wrach_ctx *c, const uint16_t *rows, const float *targets, uint32_t n, float *grad,
+                         float *loss) {
+    if (!c || !rows || !targets || !grad) return fail(WRACH_E_INVALID, "null argument");
+    if (!c->lparams) return fail(WRACH_E_STATE, "no parameters set");
+    if (!valid_batch(n)) return fail(WRACH_E_INVALID, "a batch is a positive multiple of 64 rows, at most 2^20");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    {
+        const int rc = ensure_batch(c, n);
+        if (rc) return rc;
+    }
+    DevTemp<uint16_t> f;
+    DevTemp<float2> t;
+    DevTemp<uint32_t> sel;
+    DevTemp<float> l;
+    WRACH_HIP(dmalloc(&f.p, (size_t)n * wrach::kFeat));
+    WRACH_HIP(dmalloc(&t.p, (size_t)n));
+    WRACH_HIP(dmalloc(&sel.p, (size_t)n));
+    WRACH_HIP(dmalloc(&l.p, 1));
+    std::vector<uint32_t> iota(n);
+    for (uint32_t i = 0; i < n; ++i) iota[i] = i;
+    WRACH_HIP(hipMemcpyAsync(f.p, rows, (size_t)n * wrach::kFeat * sizeof(uint16_t), hipMemcpyHostToDevice,
+                             c->stream));
+    WRACH_HIP(hipMemcpyAsync(t.p, targets, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    WRACH_HIP(hipMemcpyAsync(sel.p, iota.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    {
+        const int rc = enqueue_gradient(c, f.p, t.p, sel.p, n, l.p);
+        if (rc) return rc;
+    }
+    WRACH_HIP(hipMemcpyAsync(grad, c->lgrad, wrach::kParams * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    float hl = 0.0f;
+    WRACH_HIP(hipMemcpyAsync(&hl, l.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    WRACH_SETTLE(c);
+    if (loss) *loss = hl;
+    return WRACH_OK;
+}
+
+int wrach_learn_adam(wrach_ctx *c, const float *grad, float lr) {
+    if (!c || !grad) return fail(WRACH_E_INVALID, "null argument");
+    if (!c->lparams) return fail(WRACH_E_STATE, "no parameters set");
+    if (!(lr > 0.0f) || std::isinf(lr)) return fail(WRACH_E_INVALID, "learning rate must be finite and > 0");
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(hipMemcpyAsync(c->lgrad, grad, wrach::kParams * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    {
+        const int rc = enqueue_adam(c, lr);
+        if (rc) return rc;
+    }
+    WRACH_SETTLE(c);
+    return WRACH_OK;
+}
+
+int wrach_learn_sample(wrach_ctx *c, uint32_t target_passes, uint64_t *rows_total) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    if (target_passes == 0 || target_passes > wrach::kMaxStencilPasses)
+        return fail(WRACH_E_INVALID, "target passes must be 1.." + std::to_string(wrach::kMaxStencilPasses));
+    const uint32_t n = c->fp.n;
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_SETTLE(c);
+    {
+        int rc = ensure_stencil(c);
+        if (!rc) rc = ensure_learned_frame(c);
+        if (rc) return rc;
+    }
+    if (c->trows + n > c->tcap) {  // grow the training set, keeping its rows
+        size_t cap = c->tcap ? c->tcap : 1u << 20;
+        while (cap < c->trows + n) cap *= 2;
+        uint16_t *f = nullptr;
+        float2 *t = nullptr;
+        hipError_t e = hipMalloc((void **)&f, cap * wrach::kFeat * sizeof(uint16_t) + 256);
+        if (e == hipSuccess) e = hipMalloc((void **)&t, cap * sizeof(float2) + 256);
+        if (e == hipSuccess && c->trows)
+            e = hipMemcpy(f, c->tfeat, c->trows * wrach::kFeat * sizeof(uint16_t), hipMemcpyDeviceToDevice);
+        if (e == hipSuccess && c->trows)
+            e = hipMemcpy(t, c->ttarget, c->trows * sizeof(float2), hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(f);
+            (void)hipFree(t);
+            return fail(WRACH_E_DEVICE, std::string("training set: ") + hipGetErrorString(e));
+        }
+        (void)hipFree(c->tfeat);
+        (void)hipFree(c->ttarget);
+        c->tfeat = f;
+        c->ttarget = t;
+        c->tcap = cap;
+    }
+    if (n) {
+        c->canonical_valid = false;  // pos_out is used below
+        wrach::FeatureArgs f{};
+        f.pos = c->buf.pos;
+        f.vel = c->buf.vel;
+        f.feat = c->tfeat;
+        f.row0 = (uint32_t)c->trows;
+        f.cell = c->scell;
+        WRACH_HIP(wrach::launch_learn_features(c->fp, c->buf, f, c->stream));
+        // The plain passes from the state, which stays as it is: pos ->
+        // pos_out -> lbuf -> pos_out ...
+        wrach::StencilArgs a{};
+        a.cell = c->scell;
+        a.words = c->swords;
+        a.sums = c->ssums;
+        WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->ssums, c->stream));
+        const float2 *in = c->buf.pos;
+        for (uint32_t k = 0; k < target_passes; ++k) {
+            float2 *out = k % 2 == 0 ? c->buf.pos_out : c->lbuf;
+            a.pin = in;
+            a.pout = out;
+            a.in_slot = k;
+            WRACH_HIP(wrach::launch_stencil_move(c->fp, c->buf, a, c->stream));
+            in = out;
+        }
+        WRACH_HIP(wrach::launch_learn_target(in, c->buf.pos, c->ttarget, (uint32_t)c->trows, n, c->stream));
+    }
+    WRACH_SETTLE(c);
+    c->trows += n;
+    if (rows_total) *rows_total = c->trows;
+    return WRACH_OK;
+}
+
+int wrach_learn_clear_samples(wrach_ctx *c) {
+    if (!c) return fail(WRACH_E_INVALID, "null context");
+    c->trows = 0;
+    return WRACH_OK;
+}
+
+int wrach_learn_train(wrach_ctx *c, const uint32_t *order, size_t n_order, uint32_t batch, float lr,
+                      float *losses) {
+    if (!c || !order) return fail(WRACH_E_INVALID, "null argument");
+    if (!c->lparams) return fail(WRACH_E_STATE, "no parameters set");
+    if (!valid_batch(batch)) return fail(WRACH_E_INVALID, "a batch is a positive multiple of 64 rows, at most 2^20");
+    if (!(lr > 0.0f) || std::isinf(lr)) return fail(WRACH_E_INVALID, "learning rate must be finite and > 0");
+    for (size_t i = 0; i < n_order; ++i)
+        if (order[i] >= c->trows) return fail(WRACH_E_INVALID, "a batch row names no training sample");
+    const size_t steps = n_order / batch;
+    if (steps == 0) return WRACH_OK;
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_SETTLE(c);
+    {
+        const int rc = ensure_batch(c, batch);
+        if (rc) return rc;
+    }
+    if (c->tsel_cap < steps * batch) {
+        (void)hipFree(c->tsel);
+        c->tsel = nullptr;
+        c->tsel_cap = 0;
+        WRACH_HIP(dmalloc(&c->tsel, steps * batch));
+        c->tsel_cap = steps * batch;
+    }
+    if (c->tloss_cap < steps) {
+        (void)hipFree(c->tloss);
+        c->tloss = nullptr;
+        c->tloss_cap = 0;
+        WRACH_HIP(dmalloc(&c->tloss, steps));
+        c->tloss_cap = steps;
+    }
+    WRACH_HIP(hipMemcpyAsync(c->tsel, order, steps * batch * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    for (size_t s = 0; s < steps; ++s) {
+        int rc = enqueue_gradient(c, c->tfeat, c->ttarget, c->tsel + s * batch, batch, c->tloss + s);
+        if (!rc) rc = enqueue_adam(c, lr);
+        if (rc) return rc;
+    }
+    if (losses)
+        WRACH_HIP(hipMemcpyAsync(losses, c->tloss, steps * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    WRACH_SETTLE(c);
+    return WRACH_OK;
+}
+
+int wrach_learned_stats(wrach_ctx *c, wrach_learned_report *out) {
+    if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
+    wrach_learned_report r{};
+    if (c->swords && c->ssums) {
+        WRACH_HIP(hipSetDevice(c->cfg.device));
+        WRACH_SETTLE(c);
+        std::vector<unsigned long long> sums(wrach::kStencilSumEntries, 0ull);
+        uint32_t fb = 0;
+        WRACH_HIP(hipMemcpy(sums.data(), c->ssums, sums.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        WRACH_HIP(hipMemcpy(&fb, c->swords + wrach::kFallbackWord, sizeof fb, hipMemcpyDeviceToHost));
+        for (uint32_t slot = 0; slot < wrach::kStencilSumSlots; ++slot) {
+            const size_t e0 = (size_t)slot * wrach::kStencilSumStride;
+            const size_t e2 = ((size_t)2 * wrach::kStencilSumSlots + slot) * wrach::kStencilSumStride;
+            r.start_overlap_sum += sums[e0];
+            r.start_pairs += sums[e0 + 1];
+            r.warm_overlap_sum += sums[e2];
+            r.warm_pairs += sums[e2 + 1];
+        }
+        r.fell_back = fb;
+    }
+    *out = r;
     return WRACH_OK;
 }
 

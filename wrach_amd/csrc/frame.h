@@ -181,16 +181,22 @@ hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, h
 // largest overlap (float bits) after 0..max_passes passes for the last
 // frame; the passes that ran are those whose start exceeded the tolerance.
 constexpr uint32_t kMaxStencilPasses = 64;
-constexpr uint32_t kStencilWords = kMaxStencilPasses + 1;
+constexpr uint32_t kStencilWords = kMaxStencilPasses + 2;
 constexpr uint32_t kStencilBudget = 4096;  // most particles one lane visits in its 3x3 cells
 // Overlap sums (1 - d in 2^-24 units) and pair counts of the frame's first
-// and last measure, striped over kStencilSumSlots counters 128 bytes apart
-// (block b adds to slot b % kStencilSumSlots): entry
+// and last measure (sets 0 and 1), and of the learned warm start's output
+// (set 2), striped over kStencilSumSlots counters 128 bytes apart (block b
+// adds to slot b % kStencilSumSlots): entry
 // ((which * kStencilSumSlots + slot) * kStencilSumStride) holds the sum,
 // the next one the count.
 constexpr uint32_t kStencilSumSlots = 64;
 constexpr uint32_t kStencilSumStride = 16;  // u64 entries between slots
-constexpr uint32_t kStencilSumEntries = 2 * kStencilSumSlots * kStencilSumStride;
+constexpr uint32_t kStencilSumSets = 3;
+constexpr uint32_t kStencilSumEntries = kStencilSumSets * kStencilSumSlots * kStencilSumStride;
+// words[kFallbackWord]: 1 when the last learned frame fell back to the
+// frame-start positions (the warm start left more mean overlap than it
+// found), else 0.
+constexpr uint32_t kFallbackWord = kMaxStencilPasses + 1;
 constexpr float kOverlapScale = 16777216.0f;  // 2^24
 struct StencilArgs {
     const float2 *pin;  // positions entering the kernel
@@ -203,6 +209,11 @@ struct StencilArgs {
     uint32_t first;     // the frame's first kernel: derives and writes `cell`
     uint32_t check;     // a move: skip (copy) when the measured start is within tol
     float tol;          // a move runs while the overlap at its start exceeds this
+    // A move after a learned warm start (else null): it reads pin_alt (the
+    // frame-start positions) instead of pin when the mean overlap of the
+    // warm start (sums set 2) exceeds that of the frame start (set 0), and
+    // block 0 records the choice in words[kFallbackWord].
+    const float2 *pin_alt;
 };
 hipError_t launch_stencil_measure(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
                                   hipStream_t stream);

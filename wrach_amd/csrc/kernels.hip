@@ -44,6 +44,7 @@
 #include <hip/hip_ext.h>
 
 #include "frame.h"
+#include "learn.h"
 
 namespace wrach {
 namespace {
@@ -1898,6 +1899,30 @@ __device__ __forceinline__ unsigned long long *stencil_sums(const StencilArgs &a
     return a.stat ? a.sums + (a.stat - 1) * kStencilSumSlots * kStencilSumStride : nullptr;
 }
 
+// The learned frame's fallback: whether the warm start's mean overlap
+// (sums set 2) exceeds the frame start's (set 0), compared exactly as
+// sum_w * n_0 > sum_0 * n_w in 128 bits.  Every block of the move computes
+// it from the same words; wavefront 0 reads the 64 slots.
+__device__ __forceinline__ bool warm_rejected(const StencilArgs &a) {
+    __shared__ uint32_t s_rej;
+    if (threadIdx.x < 64) {
+        const unsigned long long *s0 = a.sums + (size_t)threadIdx.x * kStencilSumStride;
+        const unsigned long long *s2 = a.sums + (size_t)(2 * kStencilSumSlots + threadIdx.x) * kStencilSumStride;
+        unsigned long long v[4] = {s0[0], s0[1], s2[0], s2[1]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        if (threadIdx.x == 0) {
+            const unsigned __int128 lhs = (unsigned __int128)v[2] * v[1];
+            const unsigned __int128 rhs = (unsigned __int128)v[0] * v[3];
+            s_rej = lhs > rhs ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    return s_rej != 0;
+}
+
 __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, FrameBuffers b,
                                                               StencilArgs a) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -1932,6 +1957,11 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
     uint32_t pairs = 0;
     unsigned long long osum = 0;
     float2 q = make_float2(0.0f, 0.0f);
+    if (a.pin_alt) {
+        const bool rej = warm_rejected(a);
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.words[kFallbackWord] = rej ? 1u : 0u;
+        if (rej) a.pin = a.pin_alt;  // a copy of the arguments: the whole grid decides alike
+    }
     if (i < p.n) {
         q = a.pin[i];
         const uint32_t k = stencil_cell(p, b, a, i, q);
@@ -1953,7 +1983,7 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
 
 __global__ __launch_bounds__(256) void stencil_clear_kernel(uint32_t *w, unsigned long long *sums) {
     for (uint32_t i = threadIdx.x; i < kStencilWords; i += 256) w[i] = 0;
-    for (uint32_t i = threadIdx.x; i < 2 * kStencilSumSlots; i += 256) {
+    for (uint32_t i = threadIdx.x; i < kStencilSumSets * kStencilSumSlots; i += 256) {
         sums[i * kStencilSumStride] = 0;
         sums[i * kStencilSumStride + 1] = 0;
     }
@@ -1962,6 +1992,147 @@ __global__ __launch_bounds__(256) void stencil_clear_kernel(uint32_t *w, unsigne
 __global__ __launch_bounds__(256) void copy_pos_kernel(float2 *dst, const float2 *src, uint32_t n) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour features of the learned warm start (an extension with no
+// counterpart in the reference).
+//
+// Particle i's candidates are the other particles of the 5x5 cells around
+// its cell (two cells each way, the world's own cells only), five row ranges
+// of the packed layout.  It keeps the 16 nearest by (squared distance, packed
+// index) -- the packed index is canonical, so the choice is unique -- and its
+// feature row holds, for the r-th nearest, (x_j - x_i, y_j - y_i,
+// vx_j - vx_i, vy_j - vy_i) rounded to bf16 (nearest even) at entries
+// 4r..4r+3; fewer than 16 candidates leave zeros.  More than kFeatBudget
+// candidates: an all-zero row (a bound on one lane's work).
+//
+// With a.sums it also adds the overlaps of i's pairs over the 3x3 cells
+// (the pairs a separation pass visits, under the pass's budget) to the
+// stencil sums, as stencil_particle does: every pair closer than 1 lies in
+// the 3x3 cells, and the 3x3 ranges are tracked exactly inside the 5x5 ones.
+
+// f32 -> bf16 bits, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Inserts key c into the ascending list k[0..15] (dropping the largest).
+__device__ __forceinline__ void insert16(unsigned long long (&k)[kNbrs], unsigned long long c) {
+    if (!(c < k[kNbrs - 1])) return;
+    k[kNbrs - 1] = c;
+#pragma unroll
+    for (int t = kNbrs - 1; t > 0; --t) {
+        const unsigned long long lo = k[t - 1], hi = k[t];
+        const bool sw = hi < lo;
+        k[t - 1] = sw ? hi : lo;
+        k[t] = sw ? lo : hi;
+    }
+}
+
+__global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, FrameBuffers b,
+                                                             FeatureArgs a) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long osum = 0, ocnt = 0;
+    if (i < p.n) {
+        const float2 q = a.pos[i];
+        const float2 w = a.vel[i];
+        const uint32_t base = particle_world_base(p, b, i);
+        const uint32_t k = cell_key(p, q.x, q.y, base);
+        if (a.cell) a.cell[i] = k;
+        const uint32_t local = k - base;
+        const int cx = (int)(local % p.gx), cy = (int)(local / p.gx);
+        const int gx = (int)p.gx, gy = (int)p.gy;
+        const int x0 = cx > 1 ? cx - 2 : 0, x1 = cx + 2 < gx ? cx + 2 : gx - 1;
+        const int i0 = cx > 0 ? cx - 1 : 0, i1 = cx + 1 < gx ? cx + 1 : gx - 1;
+        uint32_t rs[5], re[5], is[5], ie[5];
+        uint64_t tot5 = 0, tot3 = 0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            const int y = cy + t - 2;
+            rs[t] = re[t] = is[t] = ie[t] = 0;
+            if (y < 0 || y >= gy) continue;
+            const uint32_t row = base + (uint32_t)(y * gx);
+            rs[t] = cell_start(p, b, row + (uint32_t)x0);
+            re[t] = cell_start(p, b, row + (uint32_t)x1 + 1);
+            if (re[t] < rs[t]) re[t] = rs[t];
+            tot5 += re[t] - rs[t];
+            if (t >= 1 && t <= 3) {
+                is[t] = cell_start(p, b, row + (uint32_t)i0);
+                ie[t] = cell_start(p, b, row + (uint32_t)i1 + 1);
+                if (ie[t] < is[t]) ie[t] = is[t];
+                tot3 += ie[t] - is[t];
+            }
+        }
+        const bool feats = tot5 <= kFeatBudget;
+        const bool sums = a.sums != nullptr && tot3 <= kStencilBudget;
+        unsigned long long key[kNbrs];
+#pragma unroll
+        for (int r = 0; r < kNbrs; ++r) key[r] = ~0ull;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            const bool inner = t >= 1 && t <= 3;
+            const uint32_t lo = feats ? rs[t] : (sums && inner ? is[t] : 0u);
+            const uint32_t hi = feats ? re[t] : (sums && inner ? ie[t] : 0u);
+            for (uint32_t j = lo; j < hi; ++j) {
+                if (j == i) continue;
+                const float2 o = a.pos[j];
+                if (sums && inner && j >= is[t] && j < ie[t]) {
+                    float ex, ey, ov;
+                    if (stencil_pair(q.x, q.y, o.x, o.y, ex, ey, ov)) {
+                        osum += (uint32_t)rintf(ov * kOverlapScale);
+                        ocnt += 1;
+                    }
+                }
+                if (feats) {
+                    const float dx = o.x - q.x;
+                    const float dy = o.y - q.y;
+                    float sq = dx * dx;
+                    sq = sq + dy * dy;
+                    insert16(key, ((unsigned long long)__float_as_uint(sq) << 32) | j);
+                }
+            }
+        }
+        uint4 *row = reinterpret_cast<uint4 *>(a.feat + (size_t)(a.row0 + i) * kFeat);
+#pragma unroll
+        for (int r = 0; r < kNbrs; r += 2) {
+            uint32_t h[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (key[r + u] != ~0ull) {
+                    const uint32_t j = (uint32_t)key[r + u];
+                    const float2 o = a.pos[j];
+                    const float2 ov = a.vel[j];
+                    h[2 * u] = (uint32_t)bf16_bits(o.x - q.x) | ((uint32_t)bf16_bits(o.y - q.y) << 16);
+                    h[2 * u + 1] = (uint32_t)bf16_bits(ov.x - w.x) | ((uint32_t)bf16_bits(ov.y - w.y) << 16);
+                }
+            }
+            row[r / 2] = make_uint4(h[0], h[1], h[2], h[3]);
+        }
+    }
+    if (a.sums) {
+        __shared__ unsigned long long s_sum[4], s_cnt[4];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            osum += __shfl_xor(osum, o, 64);
+            ocnt += __shfl_xor(ocnt, o, 64);
+        }
+        if (lane_id() == 0) {
+            s_sum[threadIdx.x >> 6] = osum;
+            s_cnt[threadIdx.x >> 6] = ocnt;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+            if (cnt) {
+                unsigned long long *slot = a.sums + (blockIdx.x % kStencilSumSlots) * kStencilSumStride;
+                atomicAdd(slot, s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3]);
+                atomicAdd(slot + 1, cnt);
+            }
+        }
+    }
 }
 
 }  // namespace
@@ -2098,6 +2269,13 @@ hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, cons
                                hipStream_t stream) {
     if (p.n == 0) return hipSuccess;
     hipLaunchKernelGGL(stencil_move_kernel, dim3((p.n + 255) / 256), dim3(256), 0, stream, p, b, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_learn_features(const FrameParams &p, const FrameBuffers &b, const FeatureArgs &a,
+                                 hipStream_t stream) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(learn_features_kernel, dim3((p.n + 255) / 256), dim3(256), 0, stream, p, b, a);
     return hipGetLastError();
 }
 
