@@ -247,6 +247,157 @@ def bench_many_worlds(args) -> int:
     return 0
 
 
+LEARNED_METRIC = "learned/plain speedup at equal mean overlap"
+# bf16 MFMA peak of MI355X (dense), and the network's MFMA work per particle:
+# 2 x (64 x 128 + 128 x 128) multiply-adds; the 128 -> 2 layer (512 flop)
+# runs on the vector ALUs.
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+NET_MFMA_FLOP = 2 * (64 * 128 + 128 * 128)
+
+
+def _frame_stats(world):
+    """Mean overlap after the passes of the last frame (exact sums)."""
+    _, _, _, (sb, nb, sa, na) = world.solver_stats()
+    return sa / 2**24 / na if na else 0.0
+
+
+def _timed(world, frames: int) -> float:
+    world.prepare(frames)
+    world.sync()
+    t0 = time.perf_counter()
+    world.step(frames)
+    world.sync()
+    return time.perf_counter() - t0
+
+
+def bench_learned(args) -> int:
+    """The learned warm start on the 1M lattice (config 1's setup):
+    training pairs from the world's own mode S rollout (state_passes passes
+    per frame, targets after target_passes passes), sampled at two frames
+    (2M pairs); Adam on the device; then, from one checkpoint of the same
+    rollout, the plain step (mode S, K passes, then the reference step)
+    against the learned step (features + network warm start, k passes,
+    then the reference step), k the fewest passes whose mean overlap after
+    the first frame is at most the plain step's.  Both are timed over the
+    same number of frames from that checkpoint, graphs captured first, and
+    their per-frame mean overlaps are reported over those frames."""
+    from wrach_amd import World, workloads
+    from wrach_amd import learn as wlearn
+
+    wl = workloads.lattice_1m()
+    n = wl.n
+    K = args.plain_passes
+    print(f"[learned] {wl.name}: N={n}, plain K={K}", file=sys.stderr, flush=True)
+    world = World(wl.config)
+    world.add_particles(wl.pos, wl.vel)
+    t0 = time.perf_counter()
+    frames = [int(f) for f in args.sample_frames.split(",")]
+    rows = wlearn.collect(world, frames, args.state_passes, args.target_passes)
+    world.sync()
+    collect_s = time.perf_counter() - t0
+    print(f"[learned] {rows} training pairs in {collect_s:.2f} s", file=sys.stderr, flush=True)
+    untrained = wlearn.init_params(args.seed)
+    world.learn_set_params(untrained)
+    t0 = time.perf_counter()
+    losses = wlearn.train(world, rows, args.train_steps, batch=8192, lr=1e-3, seed=args.seed)
+    world.sync()
+    train_s = time.perf_counter() - t0
+    trained = world.learn_params()
+    print(f"[learned] {len(losses)} Adam steps in {train_s:.2f} s, loss {losses[0]:.5f} -> "
+          f"{losses[-1]:.5f}", file=sys.stderr, flush=True)
+    # The evaluation checkpoint: the same rollout, continued.
+    world.set_solver("S", args.state_passes, quiet=True)
+    world.step(args.eval_frame - frames[-1])
+    ck = world.read()
+
+    def first_frame_mean(mode, passes, params=None):
+        world.load_packed(*ck)
+        if params is not None:
+            world.learn_set_params(params)
+        world.set_solver(mode, passes)
+        world.step(1)
+        return _frame_stats(world)
+
+    plain_mean = first_frame_mean("S", K)
+    ks = {}
+    for name, params in (("trained", trained), ("untrained", untrained)):
+        ks[name] = None
+        for k in range(1, K + 1):
+            m = first_frame_mean("L", k, params)
+            if m <= plain_mean:
+                ks[name] = k
+                break
+    k = ks["trained"] or K
+    print(f"[learned] plain {K} passes: mean overlap {plain_mean:.5f}; passes needed: {ks}",
+          file=sys.stderr, flush=True)
+    # Timing, from the checkpoint, graphs captured first.
+    T = args.steps
+    world.load_packed(*ck)
+    world.set_solver("S", K, quiet=True)
+    t_plain = _timed(world, T)
+    world.load_packed(*ck)
+    world.learn_set_params(trained)
+    world.set_solver("L", k, quiet=True)
+    t_learned = _timed(world, T)
+    # Per-frame mean overlaps over the same frames (measured runs, untimed).
+    per = {}
+    for name, mode, passes in (("plain", "S", K), ("learned", "L", k)):
+        world.load_packed(*ck)
+        world.set_solver(mode, passes)
+        means, falls = [], 0
+        for _ in range(T):
+            world.step(1)
+            means.append(_frame_stats(world))
+            if mode == "L":
+                falls += int(world.learned_stats()["fell_back"])
+        per[name] = {"mean_overlap_avg": float(np.mean(means)), "mean_overlap_max": float(np.max(means)),
+                     "fallbacks": falls}
+    # The learned kernels' times on the checkpoint state.
+    world.load_packed(*ck)
+    reps = 20
+    prof = world.learn_profile(reps)
+    net_ms = prof["network"] / reps
+    feat_ms = prof["features"] / reps
+    world.close()
+    achieved = n * NET_MFMA_FLOP / (net_ms * 1e-3) / 1e12
+    speedup = t_plain / t_learned
+    print(f"[learned] plain {t_plain / T * 1e3:.3f} ms/frame, learned {t_learned / T * 1e3:.3f} ms/frame, "
+          f"speedup {speedup:.3f}; network {net_ms * 1e3:.1f} us, features {feat_ms * 1e3:.1f} us",
+          file=sys.stderr, flush=True)
+    line = {
+        "metric": LEARNED_METRIC,
+        "value": speedup,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": 0,
+        "ms_per_step": t_learned * 1e3 / T,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "particles": n, "plain_passes": K, "learned_passes": k,
+                   "passes_needed": ks, "state_passes": args.state_passes,
+                   "target_passes": args.target_passes, "sample_frames": frames, "training_pairs": rows,
+                   "train_steps": len(losses), "batch": 8192, "lr": 1e-3, "eval_frame": args.eval_frame,
+                   "loss_first_last": [float(losses[0]), float(losses[-1])], "train_s": train_s,
+                   "collect_s": collect_s, "equal_overlap": "mean overlap (1 - d over the overlapping "
+                   "pairs of the 3x3 cells, exact sums) after the passes of the first frame from the "
+                   "checkpoint, learned <= plain",
+                   "plain_first_frame_mean_overlap": plain_mean,
+                   "plain_ms_per_frame": t_plain * 1e3 / T, "learned_ms_per_frame": t_learned * 1e3 / T,
+                   "over_timed_frames": per},
+        "roofline": {"bound": "mfma", "kernel": "learn_infer", "achieved": achieved,
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MFMA_BF16_PEAK_TFLOPS,
+                     "traffic": None, "flop_per_launch": n * NET_MFMA_FLOP, "kernel_avg_us": net_ms * 1e3,
+                     "features_avg_us": feat_ms * 1e3},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +415,16 @@ def main() -> int:
                          "before the reference step (an extension; default 0, the reference's step)")
     ap.add_argument("--quiet", action="store_true",
                     help="with --passes: no overlap measure after the passes (WRACH_SOLVER_QUIET)")
+    ap.add_argument("--learned", action="store_true",
+                    help="the learned warm start against the plain step at equal mean overlap "
+                         "(trains the network on the device first)")
+    ap.add_argument("--plain-passes", type=int, default=16, help="--learned: passes of the plain step")
+    ap.add_argument("--state-passes", type=int, default=4, help="--learned: passes per rollout frame")
+    ap.add_argument("--target-passes", type=int, default=16, help="--learned: passes behind a target")
+    ap.add_argument("--sample-frames", default="6,12", help="--learned: frames sampled for training")
+    ap.add_argument("--train-steps", type=int, default=3000, help="--learned: Adam steps")
+    ap.add_argument("--eval-frame", type=int, default=20, help="--learned: the checkpoint frame")
+    ap.add_argument("--seed", type=int, default=0, help="--learned: initialisation and batch order")
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="only measure a device-to-device copy's bandwidth (the bench's child process)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -281,6 +442,8 @@ def main() -> int:
 
     if args.workload == MANY_WORLDS:
         return bench_many_worlds(args)
+    if args.learned:
+        return bench_learned(args)
 
     from wrach_amd import World, workloads
 

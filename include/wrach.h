@@ -405,6 +405,32 @@ typedef struct wrach_learned_report {
 
 int wrach_learned_stats(wrach_ctx *ctx, wrach_learned_report *out);
 
+/* Runs the learned frame's two kernels -- features of the current state,
+ * then the network's warm start into a scratch buffer -- reps times with
+ * events around each, synchronises, and adds their summed elapsed
+ * milliseconds to ms[0] (features) and ms[1] (network).  The state does
+ * not change. */
+int wrach_learn_profile(wrach_ctx *ctx, uint32_t reps, double *ms);
+
+/* One frame with no allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It is wrach_step
+ * of one frame followed by the canonical order of wrach_read, on a packed
+ * triple the caller holds in device memory and that is replaced in place:
+ * indices (grid cells + 2 offsets, the layout of wrach_read), pos_xy and
+ * vel_xy (2n floats each).  ws gives the view, grid, cell size and
+ * particles_in_frame_count (= n); gravity and dt as in wrach_config.  The
+ * frames' other buffers live in one device array of
+ * WRACH_NOALLOC_SCRATCH_BYTES per GPU that the library holds until the
+ * process ends; a frame that needs more (about 8 bytes per cell plus 92 per
+ * particle) returns WRACH_E_CAPACITY before any work.  Asynchronous on
+ * `stream` (a hipStream_t; NULL: the null stream); calls no hipMalloc,
+ * hipFree or allocating routine.  Calls that overlap on one GPU share that
+ * array and race: serialise them (one stream, or wait between). */
+#define WRACH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int wrach_frame_noalloc(const wrach_world_settings *ws, const float gravity[2], float dt,
+                        uint32_t *indices, float *pos_xy, float *vel_xy, uint32_t n, void *stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *wrach_last_error(void);
 

@@ -132,3 +132,48 @@ def test_product_does_not_reference_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower(), f
+
+
+RUST_LIB = os.path.join(ROOT, "wrach-hip-sys", "src", "lib.rs")
+
+
+def test_rust_crate_binds_every_entry_point():
+    """wrach-hip-sys (not compiled here: no cargo in this image) declares
+    each function of the header once, with the same parameter count."""
+    rust = open(RUST_LIB).read()
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for name in declared_functions():
+        m = re.search(rf"pub fn {name}\((.*?)\)", rust, flags=re.S)
+        assert m, name
+        c = re.search(rf"\b{name}\s*\((.*?)\)\s*;", text, flags=re.S)
+        cparams = [p for p in c.group(1).split(",") if p.strip() and p.strip() != "void"]
+        rparams = [p for p in m.group(1).split(",") if p.strip()]
+        assert len(cparams) == len(rparams), name
+
+
+def test_rust_crate_layouts_match_the_header():
+    """The sizes and offsets the crate asserts at compile time are the
+    header's, as a C compiler lays it out."""
+    rust = open(RUST_LIB).read()
+    sizes = dict((k, int(v)) for k, v in re.findall(r"size_of::<(\w+)>\(\) == (\d+)", rust))
+    offsets = [(s, f, int(v)) for s, f, v in re.findall(r"offset_of!\((\w+), (\w+)\) == (\d+)", rust)]
+    assert len(sizes) >= 6 and len(offsets) >= 4
+    lines = [f'printf("{k} %zu\\n", sizeof({k}));' for k in sizes]
+    lines += [f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));' for s, f, _ in offsets]
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"wrach.h\"\nint main(void) {\n" + \
+          "\n".join(lines) + "\nreturn 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        got = dict(line.split() for line in subprocess.run([exe], capture_output=True, text=True,
+                                                           check=True).stdout.splitlines())
+    for k, v in sizes.items():
+        assert int(got[k]) == v, k
+    for s, f, v in offsets:
+        assert int(got[f"{s}.{f}"]) == v, f"{s}.{f}"
+    for name in ("WRACH_SOLVER_L", "WRACH_LEARN_PARAMS", "WRACH_LEARN_FEATURES", "WRACH_MAX_PASSES"):
+        value = int(re.search(rf"#define {name} (\d+)u", open(HEADER).read()).group(1))
+        assert int(re.search(rf"pub const {name}: u32 = (\d+);", rust).group(1)) == value, name
+        assert getattr(_lib, name) == value, name

@@ -26,6 +26,7 @@ WRACH_SOLVER_L = 2
 WRACH_LEARN_FEATURES = 64
 WRACH_LEARN_PARAMS = 25090
 WRACH_LEARN_BUDGET = 4096
+WRACH_NOALLOC_SCRATCH_BYTES = 32 << 20
 WRACH_SOLVER_QUIET = 0x100
 WRACH_MAX_PASSES = 64
 WRACH_STENCIL_BUDGET = 4096
@@ -70,6 +71,8 @@ EXPORTS = (
     "wrach_learn_clear_samples",
     "wrach_learn_train",
     "wrach_learned_stats",
+    "wrach_learn_profile",
+    "wrach_frame_noalloc",
     "wrach_test_scan_wait",
     "wrach_last_error",
 )
@@ -208,6 +211,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_learn_clear_samples.argtypes = [vp]
     L.wrach_learn_train.argtypes = [vp, u32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_float, f32p]
     L.wrach_learned_stats.argtypes = [vp, ctypes.POINTER(LearnedReportC)]
+    L.wrach_learn_profile.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    L.wrach_frame_noalloc.argtypes = [ctypes.POINTER(WorldSettingsC), f32p, ctypes.c_float, vp, vp, vp,
+                                      ctypes.c_uint32, vp]
     L.wrach_test_scan_wait.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
     L.wrach_last_error.argtypes = []
     L.wrach_last_error.restype = ctypes.c_char_p

@@ -306,6 +306,13 @@ class World:
                                         float(lr), losses.ctypes.data_as(_f32p)))
         return losses[:steps]
 
+    def learn_profile(self, reps: int) -> dict:
+        """Summed milliseconds of the features and network kernels over
+        reps runs on the current state (wrach_learn_profile)."""
+        ms = (ctypes.c_double * 2)()
+        check(self._L.wrach_learn_profile(self._h, int(reps), ms))
+        return {"features": ms[0], "network": ms[1]}
+
     def learned_stats(self) -> dict:
         r = _lib.LearnedReportC()
         check(self._L.wrach_learned_stats(self._h, ctypes.byref(r)))

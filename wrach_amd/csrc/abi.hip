@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -911,6 +912,9 @@ int wrach_sync(wrach_ctx *c) {
 
 static_assert(WRACH_MAX_PASSES == wrach::kMaxStencilPasses, "one pass limit");
 static_assert(WRACH_STENCIL_BUDGET == wrach::kStencilBudget, "one budget");
+static_assert(WRACH_LEARN_PARAMS == wrach::kParams, "one parameter count");
+static_assert(WRACH_LEARN_FEATURES == (unsigned)wrach::kFeat, "one feature width");
+static_assert(WRACH_LEARN_BUDGET == wrach::kFeatBudget, "one feature budget");
 
 namespace {
 
@@ -1913,6 +1917,51 @@ int wrach_learn_train(wrach_ctx *c, const uint32_t *order, size_t n_order, uint3
     return WRACH_OK;
 }
 
+int wrach_learn_profile(wrach_ctx *c, uint32_t reps, double *ms) {
+    if (!c || !ms) return fail(WRACH_E_INVALID, "null argument");
+    if (!c->lparams) return fail(WRACH_E_STATE, "no parameters set");
+    ms[0] = ms[1] = 0.0;
+    if (reps == 0 || c->fp.n == 0) return WRACH_OK;
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    {
+        const int rc = ensure_learned_frame(c);
+        if (rc) return rc;
+    }
+    WRACH_SETTLE(c);
+    std::vector<hipEvent_t> ev(3 * (size_t)reps, nullptr);
+    for (auto &e : ev) WRACH_HIP(hipEventCreate(&e));
+    int rc = WRACH_OK;
+    wrach::FeatureArgs f{};
+    f.pos = c->buf.pos;
+    f.vel = c->buf.vel;
+    f.feat = c->lfeat;
+    wrach::InferArgs in{};
+    in.feat = c->lfeat;
+    in.n = c->fp.n;
+    in.pin = c->buf.pos;
+    in.pout = c->lbuf;  // the state does not change
+    for (uint32_t r = 0; r < reps && rc == WRACH_OK; ++r) {
+        hipError_t e = hipEventRecord(ev[3 * r], c->stream);
+        if (e == hipSuccess) e = wrach::launch_learn_features(c->fp, c->buf, f, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 1], c->stream);
+        if (e == hipSuccess) e = wrach::launch_learn_infer(mlp_weights(c), in, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * r + 2], c->stream);
+        if (e != hipSuccess) rc = fail(WRACH_E_DEVICE, std::string("learn profile: ") + hipGetErrorString(e));
+    }
+    if (rc == WRACH_OK) rc = settle(c);
+    for (uint32_t r = 0; r < reps && rc == WRACH_OK; ++r) {
+        float a = 0.0f, b = 0.0f;
+        hipError_t e = hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]);
+        if (e != hipSuccess) rc = fail(WRACH_E_DEVICE, std::string("hipEventElapsedTime: ") + hipGetErrorString(e));
+        ms[0] += a;
+        ms[1] += b;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    c->canonical_valid = false;
+    return rc;
+}
+
 int wrach_learned_stats(wrach_ctx *c, wrach_learned_report *out) {
     if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
     wrach_learned_report r{};
@@ -1934,6 +1983,176 @@ int wrach_learned_stats(wrach_ctx *c, wrach_learned_report *out) {
         r.fell_back = fb;
     }
     *out = r;
+    return WRACH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One frame with no allocation (no counterpart in the reference): for
+// callers that cannot allocate device memory.  Every buffer beyond the
+// caller's packed triple lives in `scratch`, a file-scope device array that
+// each GPU holds until the process ends; the host reaches it only through
+// the address hipGetSymbolAddress returns for the current device.
+
+}  // extern "C"
+
+namespace {
+
+constexpr size_t kNoAllocBytes = WRACH_NOALLOC_SCRATCH_BYTES;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocDevices] = {};
+
+// The scratch array's address on the current device (looked up once per
+// device, with that device current).
+int scratch_address(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(WRACH_E_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(WRACH_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+    return WRACH_OK;
+}
+
+// Carves the frame's buffers for n particles and ncell cells out of `base`
+// (256-byte aligned pieces); returns the bytes used (base may be null to
+// size only).
+size_t carve_noalloc(unsigned char *base, uint32_t n, uint64_t ncell, wrach::FrameBuffers &b,
+                     size_t &ncells_total) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t np = n ? n : 1;
+    b.idx = reinterpret_cast<uint32_t *>(take((ncell + 2 + 64) * sizeof(uint32_t)));
+    if (b.idx) b.idx += 3;  // &idx[1] on a 16-byte boundary
+    b.pos = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
+    b.vel = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
+    b.src = reinterpret_cast<uint32_t *>(take(np * sizeof(uint32_t)));
+    b.pos_out = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
+    b.vel_out = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
+    b.kslot = reinterpret_cast<uint2 *>(take(np * sizeof(uint2)));
+    b.sortbuf = reinterpret_cast<unsigned long long *>(take(np * sizeof(unsigned long long)));
+    b.cnt = reinterpret_cast<uint32_t *>(take((ncell + 16) * sizeof(uint32_t)));
+    const size_t tiles = (ncell + wrach::kMinScanTile - 1) / wrach::kMinScanTile;
+    b.tile_state = reinterpret_cast<unsigned long long *>(take(tiles * sizeof(unsigned long long)));
+    const size_t chunks = (ncell + wrach::kChunkCells - 1) / wrach::kChunkCells;
+    b.chunk_next = take(chunks);
+    b.chunk_live = take(chunks);
+    b.chunk_prefix = reinterpret_cast<uint32_t *>(take(chunks * sizeof(uint32_t)));
+    b.epoch = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
+    b.error = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
+    b.class_ctr = reinterpret_cast<uint32_t *>(take(wrach::kCtrWords * sizeof(uint32_t)));
+    ncells_total = 0;
+    for (int k = 0; k < wrach::kClasses; ++k) {
+        b.class_cap[k] = (uint32_t)(np / wrach::kClassMinN[k] + 64);
+        b.slice_cap[k] = (b.class_cap[k] + wrach::kSlices - 1) / wrach::kSlices + 16;
+        b.class_off[k] = (uint32_t)ncells_total;
+        ncells_total += (size_t)wrach::kSlices * b.slice_cap[k] + b.class_cap[k];
+    }
+    b.cells = reinterpret_cast<uint4 *>(take(ncells_total * sizeof(uint4)));
+    return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wrach_frame_noalloc(const wrach_world_settings *ws, const float gravity[2], float dt,
+                        uint32_t *indices, float *pos_xy, float *vel_xy, uint32_t n, void *stream) {
+    if (!ws || !gravity || !indices || (n && (!pos_xy || !vel_xy)))
+        return fail(WRACH_E_INVALID, "null argument");
+    if (n != ws->particles_in_frame_count)
+        return fail(WRACH_E_INVALID, "n differs from the settings' particles_in_frame_count");
+    if (ws->cell_size == 0 || ws->grid_dimensions[0] == 0 || ws->grid_dimensions[1] == 0)
+        return fail(WRACH_E_INVALID, "cell size and grid dimensions must be >= 1");
+    if (!std::isfinite(gravity[0]) || !std::isfinite(gravity[1]) || !std::isfinite(dt) ||
+        !std::isfinite(ws->view_anchor[0]) || !std::isfinite(ws->view_anchor[1]) ||
+        !std::isfinite(ws->view_dimensions[0]) || !std::isfinite(ws->view_dimensions[1]))
+        return fail(WRACH_E_INVALID, "gravity, dt and the view must be finite");
+    if (n > 0x7fffffffu) return fail(WRACH_E_CAPACITY, "more than 2^31 - 1 particles");
+    const uint64_t gx = ws->grid_dimensions[0], gy = ws->grid_dimensions[1];
+    const uint64_t ncell = gx * gy;
+    if (ncell > kMaxCells) return fail(WRACH_E_TOO_MANY_CELLS, "grid has more cells than the 32-bit offsets allow");
+    wrach::FrameBuffers b{};
+    size_t ncells_total = 0;
+    const size_t need = carve_noalloc(nullptr, n, ncell, b, ncells_total);
+    if (need > kNoAllocBytes)
+        return fail(WRACH_E_CAPACITY, "the frame needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                                          std::to_string(kNoAllocBytes) + " the library holds");
+    int dev = 0;
+    WRACH_HIP(hipGetDevice(&dev));
+    unsigned char *base = nullptr;
+    {
+        const int rc = scratch_address(dev, &base);
+        if (rc) return rc;
+    }
+    carve_noalloc(base, n, ncell, b, ncells_total);
+    b.scan_shape = 0;
+    b.scatter_shape = 0;
+    b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell, 0);
+    {
+        const uint32_t want = (b.class_cap[0] * 2 / 64 + wrach::kWavesPerBlock) / wrach::kWavesPerBlock;
+        const uint32_t resident = wrach::resident_blocks_cells();
+        if (resident == 0) return fail(WRACH_E_DEVICE, "occupancy query failed");
+        b.cells_grid = want < resident ? want : resident;
+        if (b.cells_grid == 0) b.cells_grid = 1;
+    }
+    wrach::FrameParams fp{};
+    fp.ax = ws->view_anchor[0];
+    fp.ay = ws->view_anchor[1];
+    fp.x1 = fp.ax + ws->view_dimensions[0];
+    fp.y1 = fp.ay + ws->view_dimensions[1];
+    fp.cs = (float)ws->cell_size;
+    fp.gdx = gravity[0] != 0.0f ? gravity[0] * dt : -0.0f;
+    fp.gdy = gravity[1] != 0.0f ? gravity[1] * dt : -0.0f;
+    fp.dt = dt;
+    fp.gx = (uint32_t)gx;
+    fp.gy = (uint32_t)gy;
+    fp.ncell = (uint32_t)ncell;
+    fp.n = n;
+    fp.world_cells = (uint32_t)ncell;
+    fp.wait_limit = wrach::kDefaultWaitLimit;
+    fp.epoch_skew = 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t chunks = (ncell + wrach::kChunkCells - 1) / wrach::kChunkCells;
+    const size_t tiles = (ncell + wrach::kMinScanTile - 1) / wrach::kMinScanTile;
+    // The array keeps the last call's contents: clear what starts at zero.
+    WRACH_HIP(wrach::launch_fill32(b.cnt, 0u, ncell + 16, st));
+    WRACH_HIP(wrach::launch_fill32(reinterpret_cast<uint32_t *>(b.tile_state), 0u, 2 * tiles, st));
+    WRACH_HIP(wrach::launch_fill32(reinterpret_cast<uint32_t *>(b.chunk_next), 0u, (chunks + 3) / 4, st));
+    WRACH_HIP(wrach::launch_fill32(reinterpret_cast<uint32_t *>(b.chunk_live), 0x01010101u, (chunks + 3) / 4, st));
+    WRACH_HIP(wrach::launch_fill32(b.epoch, 1u, 1, st));  // status words start at 0, never valid
+    WRACH_HIP(wrach::launch_fill32(b.error, 0u, 1, st));
+    WRACH_HIP(wrach::launch_fill32(b.class_ctr, 0u, wrach::kCtrWords, st));
+    // The caller's packed triple becomes the state (each cell one run).
+    WRACH_HIP(wrach::launch_copy32(b.idx, indices, ncell + 2, st));
+    if (n) {
+        WRACH_HIP(wrach::launch_copy32(reinterpret_cast<uint32_t *>(b.pos), reinterpret_cast<const uint32_t *>(pos_xy),
+                                       2 * (size_t)n, st));
+        WRACH_HIP(wrach::launch_copy32(reinterpret_cast<uint32_t *>(b.vel), reinterpret_cast<const uint32_t *>(vel_xy),
+                                       2 * (size_t)n, st));
+        WRACH_HIP(wrach::launch_iota(b.src, n, st));
+        WRACH_HIP(wrach::launch_collide(fp, b, st));
+        WRACH_HIP(wrach::launch_cells(fp, b, st));
+        WRACH_HIP(wrach::launch_scan(fp, b, st));
+        WRACH_HIP(wrach::launch_scatter(fp, b, st));
+        WRACH_HIP(wrach::launch_materialize(fp, b, st));
+        WRACH_HIP(wrach::launch_canonical(fp, b, st));
+        WRACH_HIP(wrach::launch_copy32(reinterpret_cast<uint32_t *>(pos_xy), reinterpret_cast<const uint32_t *>(b.pos_out),
+                                       2 * (size_t)n, st));
+        WRACH_HIP(wrach::launch_copy32(reinterpret_cast<uint32_t *>(vel_xy), reinterpret_cast<const uint32_t *>(b.vel_out),
+                                       2 * (size_t)n, st));
+        WRACH_HIP(wrach::launch_copy32(indices, b.idx, ncell + 2, st));
+    }
     return WRACH_OK;
 }
 

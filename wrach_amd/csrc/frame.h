@@ -146,6 +146,9 @@ hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStre
 // Fills the idx entries of chunks the last scan found empty (before a readback).
 hipError_t launch_materialize(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream);
+// n words of v / of src (kernels, so that a captured stream records no memset).
+hipError_t launch_fill32(uint32_t *dst, uint32_t v, size_t n, hipStream_t stream);
+hipError_t launch_copy32(uint32_t *dst, const uint32_t *src, size_t n, hipStream_t stream);
 
 // Upload binning on the device: the store (every particle added, in
 // insertion order) and where its particles go.

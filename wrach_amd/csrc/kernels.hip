@@ -1605,6 +1605,16 @@ __global__ __launch_bounds__(256) void materialize_kernel(FrameParams p, FrameBu
     }
 }
 
+__global__ void fill32_kernel(uint32_t *dst, uint32_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+__global__ void copy32_kernel(uint32_t *dst, const uint32_t *src, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = i;
@@ -2222,6 +2232,18 @@ hipError_t launch_materialize(const FrameParams &p, const FrameBuffers &b, hipSt
     const uint32_t chunks = (p.ncell + kChunkCells - 1) / kChunkCells;
     if (chunks == 0) return hipSuccess;
     hipLaunchKernelGGL(materialize_kernel, dim3(chunks), dim3(256), 0, stream, p, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill32(uint32_t *dst, uint32_t v, size_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, dst, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy32(uint32_t *dst, const uint32_t *src, size_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, dst, src, n);
     return hipGetLastError();
 }
 
