@@ -535,6 +535,29 @@ def test_many_worlds_batch_full_size(oracle):
                               f"world {k} of {len(ws)}")
 
 
+@pytest.mark.parametrize("side", [512, 1024])
+def test_many_worlds_batches_of_sides_512_and_1024(oracle, side):
+    """The other two batches of the 512-world workload at default launch
+    shapes (no autotune): 184 worlds of side 512 (48.4M cells) and 159 of
+    side 1024 (167M cells).  The first, a middle and the last world of each
+    bit for bit against the single-world oracle after 10 frames, and the
+    batch's particle total."""
+    ws = [w for w in workloads.many_worlds() if w.config.dimensions[0] == side]
+    say(f"[batch] many-worlds side {side}: {len(ws)} worlds, {sum(w.n for w in ws)} particles")
+    with World(ws[0].config, worlds=len(ws)) as b:
+        for k, w in enumerate(ws):
+            b.add_particles(w.pos, w.vel, world=k)
+        assert b.n_particles == sum(w.n for w in ws)
+        b.step(10)
+        b.sync()
+        say(f"[batch] side {side}: 10 frames done")
+        for k in (0, len(ws) // 2, len(ws) - 1):
+            P, ref = oracle_world(oracle, ws[k].config, ws[k].pos, ws[k].vel)
+            assert_same_state(b.read_world(k), oracle.step(P, *ref, nsteps=10, nthreads=THREADS),
+                              f"world {k} of {len(ws)}, side {side}")
+        assert b.read()[0][-1] == sum(w.n for w in ws)
+
+
 def test_batch_argument_errors(oracle):
     say("[batch] errors")
     cfg = WrachConfig(dimensions=(16, 16), cell_size=1)

@@ -344,6 +344,7 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.chunk_prefix);
     (void)hipFree(c->buf.epoch);
     (void)hipFree(c->buf.error);
+    (void)hipFree(c->buf.canon_ctr);
     if (c->err_host) (void)hipHostFree(c->err_host);
     (void)hipFree(c->dstore_pos);
     (void)hipFree(c->dstore_vel);
@@ -726,6 +727,7 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.chunk_prefix, c->chunks));
     WRACH_HIP_C(dmalloc(&b.epoch, 1));
     WRACH_HIP_C(dmalloc(&b.error, 1));
+    WRACH_HIP_C(dmalloc(&b.canon_ctr, 1));
     WRACH_HIP_C(hipHostMalloc((void **)&c->err_host, sizeof(uint32_t), hipHostMallocDefault));
     *c->err_host = 0;
     WRACH_HIP_C(dmalloc(&b.cells, ncells_total));
@@ -748,6 +750,7 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     }
     WRACH_HIP_C(hipMemsetAsync(b.class_ctr, 0, wrach::kCtrWords * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.error, 0, sizeof(uint32_t), c->stream));
+    WRACH_HIP_C(hipMemsetAsync(b.canon_ctr, 0, sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipStreamSynchronize(c->stream));
 #undef WRACH_HIP_C
     *out = c;
@@ -1591,11 +1594,11 @@ int ensure_batch(wrach_ctx *c, uint32_t bsz) {
     return WRACH_OK;
 }
 
-// k chunk of the weight GEMMs: at most 64 chunks, each a multiple of 32.
+// k chunk of the weight GEMMs over a batch of bsz rows (a multiple of 64):
+// at most 64 chunks, each a multiple of 32, the last one possibly shorter.
 uint32_t gemm_chunk(uint32_t bsz) {
-    uint32_t kc = 32;
-    while (bsz / kc > 64 || bsz % kc) kc *= 2;
-    return kc;
+    const uint32_t chunks = std::min<uint32_t>(64u, bsz / 32u);
+    return (bsz / chunks + 31u) / 32u * 32u;
 }
 
 // One training step on the batch rows sel[0..bsz) of (feat, target):
@@ -1616,7 +1619,7 @@ int enqueue_gradient(wrach_ctx *c, const uint16_t *feat, const float2 *target, c
     t.dyt = c->tdyt;
     t.sqerr = c->tsq;
     WRACH_HIP(wrach::launch_learn_rows(mlp_weights(c), t, c->stream));
-    const uint32_t kc = gemm_chunk(bsz), chunks = bsz / kc;
+    const uint32_t kc = gemm_chunk(bsz), chunks = (bsz + kc - 1) / kc;
     WRACH_HIP(wrach::launch_learn_gemm_nt(c->txt, c->tdz1t, wrach::kFeat, wrach::kHid, bsz, kc, chunks,
                                           c->tpart1, c->stream));
     WRACH_HIP(wrach::launch_learn_gemm_nt(c->ta1t, c->tdz2t, wrach::kHid, wrach::kHid, bsz, kc, chunks,
@@ -2050,6 +2053,7 @@ size_t carve_noalloc(unsigned char *base, uint32_t n, uint64_t ncell, wrach::Fra
     b.chunk_prefix = reinterpret_cast<uint32_t *>(take(chunks * sizeof(uint32_t)));
     b.epoch = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
     b.error = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
+    b.canon_ctr = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
     b.class_ctr = reinterpret_cast<uint32_t *>(take(wrach::kCtrWords * sizeof(uint32_t)));
     ncells_total = 0;
     for (int k = 0; k < wrach::kClasses; ++k) {
@@ -2132,6 +2136,7 @@ int wrach_frame_noalloc(const wrach_world_settings *ws, const float gravity[2], 
     WRACH_HIP(wrach::launch_fill32(reinterpret_cast<uint32_t *>(b.chunk_live), 0x01010101u, (chunks + 3) / 4, st));
     WRACH_HIP(wrach::launch_fill32(b.epoch, 1u, 1, st));  // status words start at 0, never valid
     WRACH_HIP(wrach::launch_fill32(b.error, 0u, 1, st));
+    WRACH_HIP(wrach::launch_fill32(b.canon_ctr, 0u, 1, st));
     WRACH_HIP(wrach::launch_fill32(b.class_ctr, 0u, wrach::kCtrWords, st));
     // The caller's packed triple becomes the state (each cell one run).
     WRACH_HIP(wrach::launch_copy32(b.idx, indices, ncell + 2, st));

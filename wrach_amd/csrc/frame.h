@@ -64,6 +64,7 @@ struct FrameBuffers {
     uint32_t *chunk_prefix;
     uint32_t *epoch;      // frame counter tagging the scan's status words; the scatter advances it
     uint32_t *error;      // kErr* bits raised by a kernel that gave up a wait; zero otherwise
+    uint32_t *canon_ctr;  // the readback's queue of large cells (zero outside launch_canonical)
     uint4 *cells;         // queued cells (s, n, cell, 0), one region per size class
     uint32_t *class_ctr;  // fill of every slice and overflow list (ctr_index); reset by the scan
     uint32_t class_off[8];  // first entry of each class's region in `cells`

@@ -609,12 +609,16 @@ __device__ __forceinline__ void collide_chunk(const FrameParams &p, const FrameB
         cid[j] = w0 + 64 * j + ln;
         st[j] = cid[j] <= p.ncell ? b.idx[cid[j] + 1] : p.n;
     }
+    // A frame after a scan that gave up its look-back (b.error) runs no
+    // kernel that indexes by the offsets' contents: the context is dead.
+    // Vector loads (the lane term is 0 but opaque), queued behind the
+    // offsets: waiting for them waits for nothing the offsets do not.
+    const uint32_t dead = b.error[lane_id() >> 6];
     if (!BATCH) {
-        // A vector load (the lane term is 0 but opaque), queued behind the
-        // offsets: waiting for it waits for nothing they do not.
         const uint32_t live = b.chunk_live[chunk + (lane_id() >> 6)];
         if (!__builtin_amdgcn_readfirstlane(live)) return;  // the whole block: no barrier follows
     }
+    if (__builtin_amdgcn_readfirstlane(dead)) return;
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
         uint32_t end = (uint32_t)__shfl_down((int)st[j], 1, 64);
@@ -1334,38 +1338,41 @@ __global__ __launch_bounds__(kCollideBlock) void cells_kernel(FrameParams p, Fra
 
 // Canonical copy of the state into pos_out/vel_out (readback): the same
 // ordering code as the step, no arithmetic.
+// The readback's canonical order: cells of up to kRegCell particles here,
+// larger ones queued (b.canon_ctr, entries in b.cells, which the frame's
+// cell lists leave free between frames) for canonical_big_kernel, one block
+// per cell over the whole GPU: at the collapsed lattice about 2,000 floor
+// cells of ~460 particles are neighbours, and a block that sorted every big
+// cell among its own 256 would hold the readback for milliseconds.
 __global__ __launch_bounds__(kCollideBlock) void canonical_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ unsigned long long sk[kLdsSort];
-    __shared__ uint32_t big_s[kCollideBlock];
-    __shared__ uint32_t big_n[kCollideBlock];
-    __shared__ uint32_t nbig;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t c = blockIdx.x * kCollideBlock + tid;
-    if (tid == 0) nbig = 0;
-    __syncthreads();
-    if (c < p.ncell) {
-        const uint32_t s = b.idx[c + 1];
-        uint32_t n = b.idx[c + 2] - s;
-        clamp_range(p, s, n);
-        if (n == 1) {
-            one_particle<false>(p, b, c, s);
-        } else if (n == 2) {
-            small_cell<2, false>(p, b, c, s, n);
-        } else if (n > 2 && n <= 4) {
-            small_cell<4, false>(p, b, c, s, n);
-        } else if (n > 4 && n <= 8) {
-            small_cell<8, false>(p, b, c, s, n);
-        } else if (n > 8 && n <= (uint32_t)kRegCell) {
-            small_cell<kRegCell, false>(p, b, c, s, n);
-        } else if (n > (uint32_t)kRegCell) {
-            const uint32_t q = atomicAdd(&nbig, 1u);
-            big_s[q] = s;
-            big_n[q] = n;
-        }
+    const uint32_t c = blockIdx.x * kCollideBlock + threadIdx.x;
+    if (c >= p.ncell || *b.error) return;  // a dead context's offsets are not read
+    const uint32_t s = b.idx[c + 1];
+    uint32_t n = b.idx[c + 2] - s;
+    clamp_range(p, s, n);
+    if (n == 1) {
+        one_particle<false>(p, b, c, s);
+    } else if (n == 2) {
+        small_cell<2, false>(p, b, c, s, n);
+    } else if (n > 2 && n <= 4) {
+        small_cell<4, false>(p, b, c, s, n);
+    } else if (n > 4 && n <= 8) {
+        small_cell<8, false>(p, b, c, s, n);
+    } else if (n > 8 && n <= (uint32_t)kRegCell) {
+        small_cell<kRegCell, false>(p, b, c, s, n);
+    } else if (n > (uint32_t)kRegCell) {
+        const uint32_t q = atomicAdd(b.canon_ctr, 1u);
+        b.cells[q] = make_uint4(s, n, c, 0u);
     }
-    __syncthreads();
-    const uint32_t nb = nbig;
-    for (uint32_t q = 0; q < nb; ++q) big_cell<false>(p, b, big_s[q], big_n[q], 0, sk);
+}
+
+__global__ __launch_bounds__(kCollideBlock) void canonical_big_kernel(FrameParams p, FrameBuffers b) {
+    __shared__ unsigned long long sk[kLdsSort];
+    const uint32_t total = *b.error ? 0u : *b.canon_ctr;
+    for (uint32_t q = blockIdx.x; q < total; q += gridDim.x) {
+        const uint4 e = b.cells[q];
+        big_cell<false>(p, b, e.x, e.y, 0, sk);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1576,9 +1583,14 @@ __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, F
             v[j] = b.vel_out[i];
         }
     }
+    // A scan that gave up its look-back (b.error) left offsets that are not
+    // valid: the scatter then writes nothing (a vector load beside the
+    // others, checked before the stores).
+    const uint32_t dead = b.error[lane_id() >> 6];
 #pragma unroll
     for (int j = 0; j < PER; ++j)
         if (w0 + 64 * j + ln < p.n) start[j] = b.idx[ks[j].x + 1];
+    if (__builtin_amdgcn_readfirstlane(dead)) return;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         const uint32_t i = w0 + 64 * j + ln;
@@ -2029,14 +2041,39 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) {
     return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-// Inserts key c into the ascending list k[0..15] (dropping the largest).
-__device__ __forceinline__ void insert16(unsigned long long (&k)[kNbrs], unsigned long long c) {
-    if (!(c < k[kNbrs - 1])) return;
+// Candidate order: key = (squared distance bits << 32) | packed index; equal
+// distances are ordered by the bits of the candidates' (x, y, vx, vy) as
+// unsigned integers, not by their index, so that the choice does not depend
+// on the order of the particles inside a cell (two candidates equal in all
+// four are interchangeable).  ~0 (no candidate) is last.
+__device__ __forceinline__ bool cand_less(const float2 *pos, const float2 *vel, unsigned long long x,
+                                          unsigned long long y) {
+    const uint32_t hx = (uint32_t)(x >> 32), hy = (uint32_t)(y >> 32);
+    if (hx != hy) return hx < hy;
+    if (x == y) return false;
+    const uint32_t jx = (uint32_t)x, jy = (uint32_t)y;
+    const float2 px = pos[jx], py = pos[jy], vx = vel[jx], vy = vel[jy];
+    const uint32_t a[4] = {__float_as_uint(px.x), __float_as_uint(px.y), __float_as_uint(vx.x),
+                           __float_as_uint(vx.y)};
+    const uint32_t c[4] = {__float_as_uint(py.x), __float_as_uint(py.y), __float_as_uint(vy.x),
+                           __float_as_uint(vy.y)};
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (a[t] != c[t]) return a[t] < c[t];
+    return false;
+}
+
+// Inserts candidate c into the ascending list k[0..15] (dropping the last).
+// Squared distances are finite, so a real key's high word never equals the
+// empty key's (0xffffffff), and cand_less reads particles only for real keys.
+__device__ __forceinline__ void insert16(const float2 *pos, const float2 *vel, unsigned long long (&k)[kNbrs],
+                                         unsigned long long c) {
+    if (!cand_less(pos, vel, c, k[kNbrs - 1])) return;
     k[kNbrs - 1] = c;
 #pragma unroll
     for (int t = kNbrs - 1; t > 0; --t) {
         const unsigned long long lo = k[t - 1], hi = k[t];
-        const bool sw = hi < lo;
+        const bool sw = cand_less(pos, vel, hi, lo);
         k[t - 1] = sw ? hi : lo;
         k[t] = sw ? lo : hi;
     }
@@ -2101,7 +2138,7 @@ __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, Fram
                     const float dy = o.y - q.y;
                     float sq = dx * dx;
                     sq = sq + dy * dy;
-                    insert16(key, ((unsigned long long)__float_as_uint(sq) << 32) | j);
+                    insert16(a.pos, a.vel, key, ((unsigned long long)__float_as_uint(sq) << 32) | j);
                 }
             }
         }
@@ -2185,7 +2222,22 @@ uint32_t resident_blocks_cells() { return resident_blocks(reinterpret_cast<const
 hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStream_t stream) {
     const uint32_t grid = (p.ncell + kCollideBlock - 1) / kCollideBlock;
     hipLaunchKernelGGL(canonical_kernel, dim3(grid), dim3(kCollideBlock), 0, stream, p, b);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    static uint32_t big_grid = 0;
+    if (!big_grid) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, canonical_big_kernel, kCollideBlock, 0) ==
+                hipSuccess)
+            big_grid = (uint32_t)(cus * (per > 0 ? per : 1));
+        if (!big_grid) big_grid = 1024;
+    }
+    hipLaunchKernelGGL(canonical_big_kernel, dim3(big_grid), dim3(kCollideBlock), 0, stream, p, b);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_fill32(b.canon_ctr, 0u, 1, stream);  // for the next readback
 }
 
 template <int S>

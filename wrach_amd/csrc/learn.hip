@@ -363,8 +363,8 @@ __global__ __launch_bounds__(256) void learn_rows_kernel(MlpWeights w, TrainArgs
             a.xt[(size_t)(32 * s + 8 * g + e) * a.bsz + p] = __builtin_bit_cast(uint16_t, xb[0][s][e]);
 }
 
-// part[c][m][n] = sum over k in chunk c of A[m][k] B[n][k]: a 32 x 32 output
-// tile per block, one 16 x 16 MFMA tile per wavefront.
+// part[c][m][n] = sum over k in chunk c (k < ldk) of A[m][k] B[n][k]: a
+// 32 x 32 output tile per block, one 16 x 16 MFMA tile per wavefront.
 __global__ __launch_bounds__(256) void learn_gemm_nt_kernel(const uint16_t *A, const uint16_t *B,
                                                             uint32_t M, uint32_t N, uint32_t ldk,
                                                             uint32_t kc, float *part) {
@@ -374,7 +374,8 @@ __global__ __launch_bounds__(256) void learn_gemm_nt_kernel(const uint16_t *A, c
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     const uint16_t *ar = A + (size_t)(m0 + (l & 15)) * ldk + (size_t)chunk * kc + 8 * g;
     const uint16_t *br = B + (size_t)(n0 + (l & 15)) * ldk + (size_t)chunk * kc + 8 * g;
-    for (uint32_t k = 0; k < kc; k += 32)
+    const uint32_t kend = min(kc, ldk - chunk * kc);  // the last chunk may be shorter
+    for (uint32_t k = 0; k < kend; k += 32)
         acc = mfma(*reinterpret_cast<const bf16x8 *>(ar + k), *reinterpret_cast<const bf16x8 *>(br + k), acc);
     float *out = part + (size_t)chunk * M * N;
 #pragma unroll
@@ -541,7 +542,9 @@ hipError_t launch_learn_rows(const MlpWeights &w, const TrainArgs &a, hipStream_
 hipError_t launch_learn_gemm_nt(const uint16_t *A, const uint16_t *B, uint32_t M, uint32_t N,
                                 uint32_t ldk, uint32_t kc, uint32_t chunks, float *part,
                                 hipStream_t stream) {
-    if (M % 32 || N % 32 || kc % 32 || (size_t)kc * chunks > ldk) return hipErrorInvalidValue;
+    if (M % 32 || N % 32 || kc % 32 || ldk % 32 || chunks == 0 || (size_t)kc * (chunks - 1) >= ldk ||
+        (size_t)kc * chunks < ldk)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(learn_gemm_nt_kernel, dim3(M / 32, N / 32, chunks), dim3(256), 0, stream, A, B, M,
                        N, ldk, kc, part);
     return hipGetLastError();
