@@ -15,6 +15,7 @@ Bars:
   fallback), and a trained network must need fewer passes than an untrained
   one to reach the plain solver's mean overlap.
 """
+import os
 import sys
 
 import numpy as np
@@ -49,7 +50,7 @@ def test_features_match_the_oracle(oracle):
     lw = lattice()
     cases.append(("lattice", lw.config, lw.pos, lw.vel, 6))
     r = np.random.Generator(np.random.PCG64(21))
-    n = 20000
+    n = 8000  # about 19 per 3x3 cell in the lower 40 rows; capacity 10,791
     pos = np.stack([r.uniform(0, 96, n), r.uniform(0, 40, n)], 1).astype(np.float32)
     vel = r.uniform(-1, 1, (n, 2)).astype(np.float32)
     cases.append(("crowded cs 3", WrachConfig(dimensions=(96, 96), cell_size=3, gravity=(0.0, -0.1)),
@@ -113,13 +114,18 @@ def test_gradient_matches_the_oracle(n):
         g, loss = w.learn_gradient(rows, t)
         np.testing.assert_array_equal(w.learn_params(), params)  # unchanged
     ref, ref_loss = ol.gradient(params, rows, t)
+    if os.path.isdir("gpurun_out"):
+        np.savez(f"gpurun_out/gradient_{n}.npz", gpu=g, ref=ref, loss=loss, ref_loss=ref_loss)
     assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    bad = []
     for name, a, b in BLOCKS:
         d = np.linalg.norm(g[a:b] - ref[a:b])
         nrm = np.linalg.norm(ref[a:b])
         cos = float(g[a:b] @ ref[a:b] / (np.linalg.norm(g[a:b]) * nrm))
         say(f"[learn] {name}: rel L2 err {d / nrm:.2e}, cos {cos:.6f}")
-        assert d <= 1e-2 * nrm and cos > 0.9999, name
+        if not (d <= 1e-2 * nrm and cos > 0.9999):
+            bad.append(name)
+    assert not bad, bad
 
 
 def test_adam_is_bit_exact():
@@ -193,12 +199,28 @@ def test_learned_frame_with_constant_network(oracle, passes):
             assert w.learned_stats()["fell_back"] == fell
 
 
+def _onto_nearest_network():
+    """A network that moves each particle onto its nearest neighbour: hidden
+    units 0 and 1 carry the nearest's (dx, dy) shifted by 20 (where SiLU is
+    the identity to 1e-8), and the output removes the shift; the bf16
+    rounding of the hidden values (1/16 at 20) leaves the particle within
+    1/16 of that neighbour."""
+    P = {k: np.zeros(shape, np.float32) for k, shape in wl.SHAPES}
+    P["W1"][0, 0] = P["W1"][1, 1] = 1.0
+    P["b1"][:2] = 20.0
+    P["W2"][0, 0] = P["W2"][1, 1] = 1.0
+    P["W3"][0, 0] = P["W3"][1, 1] = 1.0
+    P["b3"][:] = -20.0
+    return np.concatenate([P[k].ravel() for k, _ in wl.SHAPES])
+
+
 def test_learned_frame_falls_back_on_a_bad_warm_start(oracle):
-    """Weights scaled up 50x: displacements near +-1 everywhere, far more
-    overlap than the frame start -- the frame falls back, and its positions
-    are those of passes 2..k from the frame start (k - 1 plain passes)."""
+    """A network that moves every particle onto its nearest neighbour leaves
+    far more mean overlap than the frame start had: the frame falls back,
+    and its positions are those of passes 2..k from the frame start (k - 1
+    plain passes)."""
     lw = lattice(128, 64, seed=5)
-    params = wl.init_params(10) * 50.0
+    params = _onto_nearest_network()
     k = 4
     say("[learn] fallback")
     P, ref = oracle_world(oracle, lw.config, lw.pos, lw.vel)
