@@ -247,6 +247,59 @@ def bench_many_worlds(args) -> int:
     return 0
 
 
+def traffic_summary(args) -> int:
+    """profiles/traffic_<workload>.json from two rocprofv3 --pmc runs of
+    this bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) and the same
+    two counters over a plain device copy of known size (--copy-probe): per
+    frame kernel, the median over its last --profile-steps dispatches (the
+    frames bench.py times with events, after --warmup + --steps), scaled by
+    what the counters read for the copy (bytes read and written)."""
+    import csv
+    fetch_csv, write_csv, copy_fetch_csv, copy_write_csv = args.traffic_summary
+
+    def rows(path):
+        with open(path, newline="") as f:
+            return list(csv.DictReader(f))
+
+    def per_kernel(path, counter):
+        out = {}
+        for r in rows(path):
+            if r.get("Counter_Name") != counter:
+                continue
+            out.setdefault(r["Kernel_Name"], []).append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+        return {k: [v for _, v in sorted(vs)] for k, vs in out.items()}
+
+    def copy_value(path, counter):
+        vals = [v for k, vs in per_kernel(path, counter).items() if "copyBuffer" in k for v in vs]
+        return max(vals)
+
+    probe_kib = args.copy_probe_bytes / 1024.0
+    fetch_scale = probe_kib / copy_value(copy_fetch_csv, "FETCH_SIZE")   # copy reads probe bytes
+    write_scale = probe_kib / copy_value(copy_write_csv, "WRITE_SIZE")   # and writes them
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    names = {"collide": "collide_kernel", "cells": "cells_kernel", "scan": "scan_kernel",
+             "scatter": "scatter_kernel"}
+    kernels = {}
+    for short, needle in names.items():
+        fk = next(k for k in fetch if needle in k)
+        wk = next(k for k in write if needle in k)
+        f = float(np.median(fetch[fk][-args.profile_steps:])) * 1024 * fetch_scale
+        w = float(np.median(write[wk][-args.profile_steps:])) * 1024 * write_scale
+        kernels[short] = {"hbm_bytes_per_launch": int(f + w), "fetch_bytes": int(f), "write_bytes": int(w),
+                          "dispatches": len(fetch[fk])}
+    out = {"_about": "HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch), "
+                     f"scaled by a {args.copy_probe_bytes} B device copy (FETCH x{fetch_scale:.3f}, WRITE "
+                     f"x{write_scale:.3f}); median over each frame kernel's last {args.profile_steps} "
+                     "dispatches, the frames bench.py times with events",
+           "frames": [args.warmup + args.steps, args.warmup + args.steps + args.profile_steps],
+           "fetch_scale": fetch_scale, "write_scale": write_scale, "kernels": kernels}
+    path = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 LEARNED_METRIC = "learned/plain speedup at equal mean overlap"
 # bf16 MFMA peak of MI355X (dense), and the network's MFMA work per particle:
 # 2 x (64 x 128 + 128 x 128) multiply-adds; the 128 -> 2 layer (512 flop)
@@ -425,6 +478,12 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=3000, help="--learned: Adam steps")
     ap.add_argument("--eval-frame", type=int, default=20, help="--learned: the checkpoint frame")
     ap.add_argument("--seed", type=int, default=0, help="--learned: initialisation and batch order")
+    ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
+                                                          "COPY_WRITE_CSV"),
+                    help="write profiles/traffic_<workload>.json from rocprofv3 counter CSVs of this bench "
+                         "run with the same --steps/--warmup/--profile-steps, and of --copy-probe")
+    ap.add_argument("--copy-probe-bytes", type=int, default=268435456,
+                    help="--traffic-summary: the bytes of the --copy-probe run")
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="only measure a device-to-device copy's bandwidth (the bench's child process)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -432,6 +491,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.traffic_summary:
+        return traffic_summary(args)
     if args.copy_bandwidth:
         return copy_bandwidth()
     if args.gpus != 1:

@@ -55,6 +55,13 @@ def test_features_match_the_oracle(oracle):
     vel = r.uniform(-1, 1, (n, 2)).astype(np.float32)
     cases.append(("crowded cs 3", WrachConfig(dimensions=(96, 96), cell_size=3, gravity=(0.0, -0.1)),
                   pos, vel, 2))
+    # Exact ties: coincident copies, half with their original's velocity.
+    n = 3000
+    pos = np.stack([r.uniform(0, 64, n), r.uniform(0, 32, n)], 1).astype(np.float32)
+    vel = r.uniform(-1, 1, (n, 2)).astype(np.float32)
+    pos[-400:] = pos[:400]
+    vel[-200:] = vel[:200]
+    cases.append(("coincident copies", WrachConfig(dimensions=(64, 64), cell_size=1), pos, vel, 0))
     for name, cfg, pos, vel, frames in cases:
         say(f"[learn] features: {name}")
         with World(cfg) as w:

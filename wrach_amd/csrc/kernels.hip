@@ -644,80 +644,31 @@ __device__ __forceinline__ void collide_chunk(const FrameParams &p, const FrameB
         clamp_range(p, st[j], nn[j]);
     }
 
-    // Cells of one and two particles, all their loads in flight together.
-    // Two: canonical order is ascending source index; the one pair of the
-    // reference's loop (particles.rs:62-81) is symmetric in its operands,
-    // so only the output positions depend on that order.  Both particles
-    // land in one new cell: one claim of two slots (a run, flagged at its
-    // first); else one claim each (runs of one).
-    float px[kCellsPerLane][2], py[kCellsPerLane][2], vx[kCellsPerLane][2], vy[kCellsPerLane][2];
-    uint32_t sr[kCellsPerLane][2];
+    // Single-particle cells.
+    float px[kCellsPerLane], py[kCellsPerLane], vx[kCellsPerLane], vy[kCellsPerLane];
+    uint32_t key[kCellsPerLane], slot[kCellsPerLane];
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            px[j][u] = py[j][u] = vx[j][u] = vy[j][u] = 0.0f;
-            sr[j][u] = 0;
-            if (nn[j] == 1 + u || (u == 0 && nn[j] == 2)) {
-                const float2 q = b.pos[st[j] + u];
-                const float2 w = b.vel[st[j] + u];
-                px[j][u] = q.x; py[j][u] = q.y; vx[j][u] = w.x; vy[j][u] = w.y;
-            }
-        }
-        if (nn[j] == 2) {
-            sr[j][0] = b.src[st[j]] & kSrcMask;
-            sr[j][1] = b.src[st[j] + 1] & kSrcMask;
+        key[j] = 0;
+        slot[j] = 0;
+        px[j] = py[j] = vx[j] = vy[j] = 0.0f;
+        if (nn[j] == 1) {
+            const float2 q = b.pos[st[j]];
+            const float2 w = b.vel[st[j]];
+            px[j] = q.x; py[j] = q.y; vx[j] = w.x; vy[j] = w.y;
+            finish_particle(p, px[j], py[j], vx[j], vy[j]);
+            key[j] = cell_key(p, px[j], py[j], world_base(p, cid[j]));
         }
     }
-    uint32_t key[kCellsPerLane][2], slot[kCellsPerLane][2];
 #pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        key[j][0] = key[j][1] = 0;
-        slot[j][0] = slot[j][1] = 0;
-        if (nn[j] == 2) {
-            if (sr[j][1] < sr[j][0]) {  // canonical order
-                float t = px[j][0]; px[j][0] = px[j][1]; px[j][1] = t;
-                t = py[j][0]; py[j][0] = py[j][1]; py[j][1] = t;
-                t = vx[j][0]; vx[j][0] = vx[j][1]; vx[j][1] = t;
-                t = vy[j][0]; vy[j][0] = vy[j][1]; vy[j][1] = t;
-            }
-            collide_pair(px[j][0], py[j][0], px[j][1], py[j][1]);
-        }
-        if (nn[j] == 1 || nn[j] == 2) {
-            const uint32_t base = world_base(p, cid[j]);
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                if (u < (int)nn[j]) {
-                    finish_particle(p, px[j][u], py[j][u], vx[j][u], vy[j][u]);
-                    key[j][u] = cell_key(p, px[j][u], py[j][u], base);
-                }
-            }
-        }
-    }
+    for (int j = 0; j < kCellsPerLane; ++j)
+        if (nn[j] == 1) slot[j] = claim(b, key[j], 1u);
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
         if (nn[j] == 1) {
-            slot[j][0] = claim(b, key[j][0], 1u) | kRunFlag;
-        } else if (nn[j] == 2) {
-            if (key[j][0] == key[j][1]) {
-                const uint32_t f = claim(b, key[j][0], 2u);
-                slot[j][0] = f | kRunFlag;
-                slot[j][1] = f + 1;
-            } else {
-                slot[j][0] = claim(b, key[j][0], 1u) | kRunFlag;
-                slot[j][1] = claim(b, key[j][1], 1u) | kRunFlag;
-            }
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            if (u < (int)nn[j] && nn[j] <= 2) {
-                b.pos_out[st[j] + u] = make_float2(px[j][u], py[j][u]);
-                b.vel_out[st[j] + u] = make_float2(vx[j][u], vy[j][u]);
-                b.kslot[st[j] + u] = make_uint2(key[j][u], slot[j][u]);
-            }
+            b.pos_out[st[j]] = make_float2(px[j], py[j]);
+            b.vel_out[st[j]] = make_float2(vx[j], vy[j]);
+            b.kslot[st[j]] = make_uint2(key[j], slot[j] | kRunFlag);
         }
     }
 
@@ -726,7 +677,7 @@ __device__ __forceinline__ void collide_chunk(const FrameParams &p, const FrameB
     uint32_t cls[kCellsPerLane], at[kCellsPerLane];
 #pragma unroll
     for (int j = 0; j < kCellsPerLane; ++j) {
-        cls[j] = nn[j] == 2 ? (uint32_t)kClasses : size_class(nn[j]);
+        cls[j] = size_class(nn[j]);
         at[j] = cls[j] < (uint32_t)kClasses ? atomicAdd(&s_cnt[cls[j]], 1u) : 0u;
     }
     __syncthreads();

@@ -356,11 +356,17 @@ __global__ __launch_bounds__(256) void learn_rows_kernel(MlpWeights w, TrainArgs
             a.dz1t[(size_t)o * a.bsz + p] = bf16_bits(acc[r] * silu_grad(z));
         }
     }
+    // The inputs, feature-major, from the raw bits of the B operands (a
+    // whole-vector cast: subscripting the bf16 vector type here read
+    // element 0 for every element).
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < 2; ++s) {
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 w4 = __builtin_bit_cast(u32x4, xb[0][s]);
 #pragma unroll
         for (int e = 0; e < 8; ++e)
-            a.xt[(size_t)(32 * s + 8 * g + e) * a.bsz + p] = __builtin_bit_cast(uint16_t, xb[0][s][e]);
+            a.xt[(size_t)(32 * s + 8 * g + e) * a.bsz + p] = (uint16_t)(w4[e >> 1] >> (16 * (e & 1)));
+    }
 }
 
 // part[c][m][n] = sum over k in chunk c (k < ldk) of A[m][k] B[n][k]: a
