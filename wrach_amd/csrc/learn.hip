@@ -49,9 +49,11 @@ __host__ __device__ constexpr uint32_t kmap(uint32_t s, uint32_t g, uint32_t e) 
     return 32u * s + (e < 4u ? 4u * g + e : 16u + 4u * g + (e - 4u));
 }
 
-// SiLU and its derivative, in f32.  exp via the hardware exp2.
+// SiLU and its derivative, in f32, from the hardware exp2 and reciprocal
+// (v_exp_f32, v_rcp_f32: about 1 ulp each; the network is trained and run
+// with the same function).
 __device__ __forceinline__ float sigmoid(float z) {
-    return 1.0f / (1.0f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
 }
 __device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
@@ -59,10 +61,11 @@ __device__ __forceinline__ float silu_grad(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
+// Eight f32 to bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ __forceinline__ bf16x8 pack8(const float (&v)[8]) {
     bf16x8 r;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) r[e] = __builtin_bit_cast(__bf16, bf16_bits(v[e]));
+    for (int e = 0; e < 8; ++e) r[e] = (__bf16)v[e];
     return r;
 }
 
@@ -187,7 +190,10 @@ __device__ __forceinline__ void mlp_infer(const bf16x8 (&xb)[NC][2], FragPtr fra
     float y0[NC], y1[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) y0[c] = y1[c] = 0.0f;
-#pragma unroll
+    // One output tile at a time (not unrolled): its 4 fragments and 8 W3
+    // entries are the only weights live, which keeps the kernel at 4
+    // wavefronts per SIMD without spilling.
+#pragma unroll 1
     for (int t = 0; t < 8; ++t) {
         f32x4 bias;
 #pragma unroll
@@ -248,12 +254,15 @@ __device__ __forceinline__ void load_rows(const uint16_t *feat, const uint32_t *
 // Inference: persistent blocks of 4 wavefronts, weights staged in LDS once
 // per block, 32 particles per wavefront per step.
 
-constexpr int kInferBlock = 256;
+constexpr int kInferBlock = 512;
 constexpr int kInferCols = 2;  // 16-particle column tiles per wavefront step
 
-__global__ __launch_bounds__(kInferBlock) void learn_infer_kernel(MlpWeights w, InferArgs a) {
+__global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(4))) void learn_infer_kernel(MlpWeights w, InferArgs a) {
     __shared__ bf16x8 s_frag[(kW1Frags + kW2Frags) * 64];  // 48 KiB
-    __shared__ float s_b1[kHid], s_b2[kHid], s_w3[kHid * kOutDim], s_b3[kOutDim];
+    __shared__ __align__(16) float s_b1[kHid];
+    __shared__ __align__(16) float s_b2[kHid];
+    __shared__ __align__(16) float s_w3[kHid * kOutDim];
+    __shared__ float s_b3[kOutDim];
     const bf16x8 *gfrag = reinterpret_cast<const bf16x8 *>(w.frag);
     for (uint32_t i = threadIdx.x; i < (kW1Frags + kW2Frags) * 64; i += kInferBlock) s_frag[i] = gfrag[i];
     for (uint32_t i = threadIdx.x; i < (uint32_t)kHid; i += kInferBlock) {
@@ -271,7 +280,13 @@ __global__ __launch_bounds__(kInferBlock) void learn_infer_kernel(MlpWeights w, 
         bf16x8 xb[kInferCols][2];
         load_rows<kInferCols>(a.feat, nullptr, row0, a.n, xb);
         float y[kInferCols][2];
-        mlp_infer<kInferCols>(xb, s_frag, s_b1, s_b2, s_w3, s_b3, y);
+        // The weights stay in LDS: an address the optimiser cannot see
+        // through keeps it from hoisting all 48 fragments into registers for
+        // the whole loop (192 VGPRs, one wavefront per SIMD).
+        const bf16x8 *fr = s_frag;
+        const float *b1 = s_b1, *b2 = s_b2, *w3 = s_w3;
+        asm volatile("" : "+v"(fr), "+v"(b1), "+v"(b2), "+v"(w3));
+        mlp_infer<kInferCols>(xb, fr, b1, b2, w3, s_b3, y);
         if (l < 16) {
 #pragma unroll
             for (int c = 0; c < kInferCols; ++c) {
