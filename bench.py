@@ -323,32 +323,24 @@ def _timed(world, frames: int) -> float:
     return time.perf_counter() - t0
 
 
-def bench_learned(args) -> int:
-    """The learned warm start on the 1M lattice (config 1's setup):
-    training pairs from the world's own mode S rollout (state_passes passes
-    per frame, targets after target_passes passes), sampled at two frames
-    (2M pairs); Adam on the device; then, from one checkpoint of the same
-    rollout, the plain step (mode S, K passes, then the reference step)
-    against the learned step (features + network warm start, k passes,
-    then the reference step), k the fewest passes whose mean overlap after
-    the first frame is at most the plain step's.  Both are timed over the
-    same number of frames from that checkpoint, graphs captured first, and
-    their per-frame mean overlaps are reported over those frames."""
-    from wrach_amd import World, workloads
+def learned_at(wl, K: int, args) -> dict:
+    """One plain step size K (passes): a fresh world of the workload, its
+    mode S rollout (state_passes passes per frame) sampled at the sample
+    frames with targets after K passes, Adam on the device, then the plain
+    step (K passes) against the learned step (k passes) from the checkpoint
+    of the same rollout at eval_frame; k is the fewest passes whose mean
+    overlap after the first frame is at most the plain step's."""
+    from wrach_amd import World
     from wrach_amd import learn as wlearn
 
-    wl = workloads.lattice_1m()
     n = wl.n
-    K = args.plain_passes
-    print(f"[learned] {wl.name}: N={n}, plain K={K}", file=sys.stderr, flush=True)
     world = World(wl.config)
     world.add_particles(wl.pos, wl.vel)
     t0 = time.perf_counter()
     frames = [int(f) for f in args.sample_frames.split(",")]
-    rows = wlearn.collect(world, frames, args.state_passes, args.target_passes)
+    rows = wlearn.collect(world, frames, args.state_passes, K)
     world.sync()
     collect_s = time.perf_counter() - t0
-    print(f"[learned] {rows} training pairs in {collect_s:.2f} s", file=sys.stderr, flush=True)
     untrained = wlearn.init_params(args.seed)
     world.learn_set_params(untrained)
     t0 = time.perf_counter()
@@ -356,9 +348,8 @@ def bench_learned(args) -> int:
     world.sync()
     train_s = time.perf_counter() - t0
     trained = world.learn_params()
-    print(f"[learned] {len(losses)} Adam steps in {train_s:.2f} s, loss {losses[0]:.5f} -> "
-          f"{losses[-1]:.5f}", file=sys.stderr, flush=True)
-    # The evaluation checkpoint: the same rollout, continued.
+    print(f"[learned] K={K}: {rows} pairs, {len(losses)} Adam steps in {train_s:.2f} s, loss "
+          f"{losses[0]:.5f} -> {losses[-1]:.5f}", file=sys.stderr, flush=True)
     world.set_solver("S", args.state_passes, quiet=True)
     world.step(args.eval_frame - frames[-1])
     ck = world.read()
@@ -376,14 +367,10 @@ def bench_learned(args) -> int:
     for name, params in (("trained", trained), ("untrained", untrained)):
         ks[name] = None
         for k in range(1, K + 1):
-            m = first_frame_mean("L", k, params)
-            if m <= plain_mean:
+            if first_frame_mean("L", k, params) <= plain_mean:
                 ks[name] = k
                 break
     k = ks["trained"] or K
-    print(f"[learned] plain {K} passes: mean overlap {plain_mean:.5f}; passes needed: {ks}",
-          file=sys.stderr, flush=True)
-    # Timing, from the checkpoint, graphs captured first.
     T = args.steps
     world.load_packed(*ck)
     world.set_solver("S", K, quiet=True)
@@ -392,7 +379,6 @@ def bench_learned(args) -> int:
     world.learn_set_params(trained)
     world.set_solver("L", k, quiet=True)
     t_learned = _timed(world, T)
-    # Per-frame mean overlaps over the same frames (measured runs, untimed).
     per = {}
     for name, mode, passes in (("plain", "S", K), ("learned", "L", k)):
         world.load_packed(*ck)
@@ -405,46 +391,61 @@ def bench_learned(args) -> int:
                 falls += int(world.learned_stats()["fell_back"])
         per[name] = {"mean_overlap_avg": float(np.mean(means)), "mean_overlap_max": float(np.max(means)),
                      "fallbacks": falls}
-    # The learned kernels' times on the checkpoint state.
     world.load_packed(*ck)
     reps = 20
     prof = world.learn_profile(reps)
-    net_ms = prof["network"] / reps
-    feat_ms = prof["features"] / reps
     world.close()
-    achieved = n * NET_MFMA_FLOP / (net_ms * 1e-3) / 1e12
-    speedup = t_plain / t_learned
-    print(f"[learned] plain {t_plain / T * 1e3:.3f} ms/frame, learned {t_learned / T * 1e3:.3f} ms/frame, "
-          f"speedup {speedup:.3f}; network {net_ms * 1e3:.1f} us, features {feat_ms * 1e3:.1f} us",
-          file=sys.stderr, flush=True)
+    net_ms, feat_ms = prof["network"] / reps, prof["features"] / reps
+    out = {"plain_passes": K, "learned_passes": k, "passes_needed": ks, "training_pairs": rows,
+           "train_steps": len(losses), "loss_first_last": [float(losses[0]), float(losses[-1])],
+           "train_s": train_s, "collect_s": collect_s, "plain_first_frame_mean_overlap": plain_mean,
+           "plain_ms_per_frame": t_plain * 1e3 / T, "learned_ms_per_frame": t_learned * 1e3 / T,
+           "speedup": t_plain / t_learned, "over_timed_frames": per,
+           "network_us": net_ms * 1e3, "features_us": feat_ms * 1e3,
+           "network_tflops": n * NET_MFMA_FLOP / (net_ms * 1e-3) / 1e12}
+    print(f"[learned] K={K}: plain {out['plain_ms_per_frame']:.3f} ms/frame, learned "
+          f"{out['learned_ms_per_frame']:.3f} ms/frame ({k} passes), speedup {out['speedup']:.3f}; "
+          f"network {out['network_us']:.1f} us, features {out['features_us']:.1f} us", file=sys.stderr,
+          flush=True)
+    return out
+
+
+def bench_learned(args) -> int:
+    """The learned warm start on the 1M lattice (config 1's setup), at each
+    plain step size in --plain-passes (see learned_at); value is the
+    speedup at the first of them."""
+    from wrach_amd import workloads
+
+    wl = workloads.lattice_1m()
+    Ks = [int(k) for k in str(args.plain_passes).split(",")]
+    print(f"[learned] {wl.name}: N={wl.n}, plain K in {Ks}", file=sys.stderr, flush=True)
+    res = [learned_at(wl, K, args) for K in Ks]
+    head = res[0]
     line = {
         "metric": LEARNED_METRIC,
-        "value": speedup,
+        "value": head["speedup"],
         "unit": "x",
         "n_gpus": 1,
-        "steps": T,
+        "steps": args.steps,
         "warmup": 0,
-        "ms_per_step": t_learned * 1e3 / T,
+        "ms_per_step": head["learned_ms_per_frame"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic",
-        "config": {"workload": wl.name, "particles": n, "plain_passes": K, "learned_passes": k,
-                   "passes_needed": ks, "state_passes": args.state_passes,
-                   "target_passes": args.target_passes, "sample_frames": frames, "training_pairs": rows,
-                   "train_steps": len(losses), "batch": 8192, "lr": 1e-3, "eval_frame": args.eval_frame,
-                   "loss_first_last": [float(losses[0]), float(losses[-1])], "train_s": train_s,
-                   "collect_s": collect_s, "equal_overlap": "mean overlap (1 - d over the overlapping "
-                   "pairs of the 3x3 cells, exact sums) after the passes of the first frame from the "
-                   "checkpoint, learned <= plain",
-                   "plain_first_frame_mean_overlap": plain_mean,
-                   "plain_ms_per_frame": t_plain * 1e3 / T, "learned_ms_per_frame": t_learned * 1e3 / T,
-                   "over_timed_frames": per},
-        "roofline": {"bound": "mfma", "kernel": "learn_infer", "achieved": achieved,
-                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MFMA_BF16_PEAK_TFLOPS,
-                     "traffic": None, "flop_per_launch": n * NET_MFMA_FLOP, "kernel_avg_us": net_ms * 1e3,
-                     "features_avg_us": feat_ms * 1e3},
+        "config": {"workload": wl.name, "particles": wl.n, "state_passes": args.state_passes,
+                   "sample_frames": args.sample_frames, "batch": 8192, "lr": 1e-3,
+                   "eval_frame": args.eval_frame,
+                   "equal_overlap": "mean overlap (1 - d over the overlapping pairs of the 3x3 cells, exact "
+                                    "sums) after the passes of the first frame from the checkpoint, "
+                                    "learned <= plain; targets after K passes",
+                   "by_plain_passes": res},
+        "roofline": {"bound": "mfma", "kernel": "learn_infer", "achieved": head["network_tflops"],
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": head["network_tflops"] / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "flop_per_launch": wl.n * NET_MFMA_FLOP, "kernel_avg_us": head["network_us"],
+                     "features_avg_us": head["features_us"]},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -471,9 +472,9 @@ def main() -> int:
     ap.add_argument("--learned", action="store_true",
                     help="the learned warm start against the plain step at equal mean overlap "
                          "(trains the network on the device first)")
-    ap.add_argument("--plain-passes", type=int, default=16, help="--learned: passes of the plain step")
+    ap.add_argument("--plain-passes", default="32,16",
+                    help="--learned: passes of the plain step, comma-separated (value: the first)")
     ap.add_argument("--state-passes", type=int, default=4, help="--learned: passes per rollout frame")
-    ap.add_argument("--target-passes", type=int, default=16, help="--learned: passes behind a target")
     ap.add_argument("--sample-frames", default="6,12", help="--learned: frames sampled for training")
     ap.add_argument("--train-steps", type=int, default=3000, help="--learned: Adam steps")
     ap.add_argument("--eval-frame", type=int, default=20, help="--learned: the checkpoint frame")

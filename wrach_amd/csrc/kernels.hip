@@ -2041,71 +2041,68 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) {
     return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-// Half a wavefront (32 lanes) per particle.  Its candidates are numbered
-// through the row ranges; each round, lanes 16..31 take 16 new ones and
-// lanes 0..15 hold the 16 best so far.  Every lane ranks its candidate
-// among the 32 by comparing with each other lane's (shuffles), and the
-// lanes are permuted by rank, so lanes 0..15 again hold the best 16, in
-// order.  The order is (squared distance, then the bits of x, y, vx, vy as
-// unsigned integers): candidates equal in all of them are interchangeable,
-// and there the lane order decides.  A lane holds its candidate's position
-// and velocity, which are also its features.
-constexpr uint32_t kFeatEmpty = 0xffffffffu;  // a squared distance no candidate has
-
-struct FeatCand {
-    uint32_t d2, x, y, vx, vy;
-};
-
-__device__ __forceinline__ FeatCand shfl_cand(const FeatCand &c, int src) {
-    FeatCand o;
-    o.d2 = (uint32_t)__shfl((int)c.d2, src, 32);
-    o.x = (uint32_t)__shfl((int)c.x, src, 32);
-    o.y = (uint32_t)__shfl((int)c.y, src, 32);
-    o.vx = (uint32_t)__shfl((int)c.vx, src, 32);
-    o.vy = (uint32_t)__shfl((int)c.vy, src, 32);
-    return o;
+// Candidate order: key = (squared distance bits << 32) | packed index; equal
+// distances are ordered by the bits of the candidates' (x, y, vx, vy) as
+// unsigned integers, not by their index, so that the choice does not depend
+// on the order of the particles inside a cell (two candidates equal in all
+// four are interchangeable).  ~0 (no candidate) is last.
+__device__ __forceinline__ bool cand_less(const float2 *pos, const float2 *vel, unsigned long long x,
+                                          unsigned long long y) {
+    const uint32_t hx = (uint32_t)(x >> 32), hy = (uint32_t)(y >> 32);
+    if (hx != hy) return hx < hy;
+    if (x == y) return false;
+    const uint32_t jx = (uint32_t)x, jy = (uint32_t)y;
+    const float2 px = pos[jx], py = pos[jy], vx = vel[jx], vy = vel[jy];
+    const uint32_t a[4] = {__float_as_uint(px.x), __float_as_uint(px.y), __float_as_uint(vx.x),
+                           __float_as_uint(vx.y)};
+    const uint32_t c[4] = {__float_as_uint(py.x), __float_as_uint(py.y), __float_as_uint(vy.x),
+                           __float_as_uint(vy.y)};
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (a[t] != c[t]) return a[t] < c[t];
+    return false;
 }
 
-__device__ __forceinline__ bool cand_before(const FeatCand &a, const FeatCand &b) {
-    if (a.d2 != b.d2) return a.d2 < b.d2;
-    if (a.x != b.x) return a.x < b.x;
-    if (a.y != b.y) return a.y < b.y;
-    if (a.vx != b.vx) return a.vx < b.vx;
-    return a.vy < b.vy;
-}
+constexpr int kSortLog2 = 5;
+constexpr int kSortCands = 1 << kSortLog2;  // candidates sorted by the network
 
-__device__ __forceinline__ bool cand_same(const FeatCand &a, const FeatCand &b) {
-    return a.d2 == b.d2 && a.x == b.x && a.y == b.y && a.vx == b.vx && a.vy == b.vy;
+// Inserts candidate c into the ascending list k[0..15] (dropping the last).
+// Squared distances are finite, so a real key's high word never equals the
+// empty key's (0xffffffff), and cand_less reads particles only for real keys.
+__device__ __forceinline__ void insert16(const float2 *pos, const float2 *vel, unsigned long long (&k)[kNbrs],
+                                         unsigned long long c) {
+    if (!cand_less(pos, vel, c, k[kNbrs - 1])) return;
+    k[kNbrs - 1] = c;
+#pragma unroll
+    for (int t = kNbrs - 1; t > 0; --t) {
+        const unsigned long long lo = k[t - 1], hi = k[t];
+        const bool sw = cand_less(pos, vel, hi, lo);
+        k[t - 1] = sw ? hi : lo;
+        k[t] = sw ? lo : hi;
+    }
 }
 
 __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, FrameBuffers b,
                                                              FeatureArgs a) {
-    const uint32_t lane = lane_id(), h = lane & 31u;
-    const uint32_t i = (blockIdx.x * 256u + threadIdx.x) >> 5;  // this half's particle
-    const bool valid = i < p.n;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     unsigned long long osum = 0, ocnt = 0;
-    float2 q = make_float2(0.0f, 0.0f), w = make_float2(0.0f, 0.0f);
-    uint32_t lo[5] = {0, 0, 0, 0, 0}, hi[5] = {0, 0, 0, 0, 0}, is[5] = {0, 0, 0, 0, 0},
-             ie[5] = {0, 0, 0, 0, 0};
-    bool sums = false, ranked = false;
-    uint32_t m = 0;  // candidates of this half
-    if (valid) {
-        q = a.pos[i];
-        w = a.vel[i];
+    if (i < p.n) {
+        const float2 q = a.pos[i];
+        const float2 w = a.vel[i];
         const uint32_t base = particle_world_base(p, b, i);
         const uint32_t k = cell_key(p, q.x, q.y, base);
-        if (a.cell && h == 0) a.cell[i] = k;
+        if (a.cell) a.cell[i] = k;
         const uint32_t local = k - base;
         const int cx = (int)(local % p.gx), cy = (int)(local / p.gx);
         const int gx = (int)p.gx, gy = (int)p.gy;
         const int x0 = cx > 1 ? cx - 2 : 0, x1 = cx + 2 < gx ? cx + 2 : gx - 1;
         const int i0 = cx > 0 ? cx - 1 : 0, i1 = cx + 1 < gx ? cx + 1 : gx - 1;
-        uint32_t rs[5], re[5];
+        uint32_t rs[5], re[5], is[5], ie[5];
         uint64_t tot5 = 0, tot3 = 0;
 #pragma unroll
         for (int t = 0; t < 5; ++t) {
             const int y = cy + t - 2;
-            rs[t] = re[t] = 0;
+            rs[t] = re[t] = is[t] = ie[t] = 0;
             if (y < 0 || y >= gy) continue;
             const uint32_t row = base + (uint32_t)(y * gx);
             rs[t] = cell_start(p, b, row + (uint32_t)x0);
@@ -2119,99 +2116,107 @@ __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, Fram
                 tot3 += ie[t] - is[t];
             }
         }
-        ranked = tot5 <= kFeatBudget;
-        sums = a.sums != nullptr && tot3 <= kStencilBudget;
-        // The candidates: the 5x5 cells, or for the sums alone the 3x3.
+        const bool feats = tot5 <= kFeatBudget;
+        const bool sums = a.sums != nullptr && tot3 <= kStencilBudget;
+        // The overlap sums: the 3x3 cells, as a separation pass visits them.
+        if (sums) {
 #pragma unroll
-        for (int t = 0; t < 5; ++t) {
-            const bool inner = t >= 1 && t <= 3;
-            lo[t] = ranked ? rs[t] : (sums && inner ? is[t] : 0u);
-            hi[t] = ranked ? re[t] : (sums && inner ? ie[t] : 0u);
-            m += hi[t] - lo[t];
-        }
-    }
-    // Rounds: the same count for both halves of the wavefront, which share
-    // the shuffles below.
-    uint32_t mm = m;
-    mm = max(mm, (uint32_t)__shfl_xor((int)mm, 32, 64));
-    FeatCand best{kFeatEmpty, 0u, 0u, 0u, 0u};
-    const uint32_t rounds = (mm + 15u) / 16u;
-    for (uint32_t r = 0; r < rounds; ++r) {
-        FeatCand c = best;
-        if (h >= 16) {
-            c = FeatCand{kFeatEmpty, 0u, 0u, 0u, 0u};
-            const uint32_t kk = r * 16u + (h - 16u);
-            if (kk < m) {
-                uint32_t tt = 0, acc = 0, j = 0;  // candidate kk: row tt, particle j
-#pragma unroll
-                for (int u = 0; u < 5; ++u) {
-                    const uint32_t len = hi[u] - lo[u];
-                    if (kk >= acc && kk < acc + len) {
-                        tt = (uint32_t)u;
-                        j = lo[u] + (kk - acc);
-                    }
-                    acc += len;
-                }
-                if (j != i) {
+            for (int t = 1; t <= 3; ++t) {
+                for (uint32_t j = is[t]; j < ie[t]; ++j) {
+                    if (j == i) continue;
                     const float2 o = a.pos[j];
-                    const bool inner = tt >= 1 && tt <= 3 && j >= is[tt] && j < ie[tt];
-                    if (sums && inner) {
-                        float ex, ey, ov;
-                        if (stencil_pair(q.x, q.y, o.x, o.y, ex, ey, ov)) {
-                            osum += (uint32_t)rintf(ov * kOverlapScale);
-                            ocnt += 1;
-                        }
+                    float ex, ey, ov;
+                    if (stencil_pair(q.x, q.y, o.x, o.y, ex, ey, ov)) {
+                        osum += (uint32_t)rintf(ov * kOverlapScale);
+                        ocnt += 1;
                     }
-                    if (ranked) {
-                        const float2 ov2 = a.vel[j];
+                }
+            }
+        }
+        // The 16 nearest: the first kSortCands candidates sorted in
+        // registers by a bitonic network (240 compare-exchanges for 32,
+        // where inserting each into a sorted list of 16 takes 15), then any
+        // further candidates inserted into the best 16.
+        unsigned long long key[kNbrs];
+#pragma unroll
+        for (int r = 0; r < kNbrs; ++r) key[r] = ~0ull;
+        if (feats) {
+            uint32_t cum[5], m5 = 0;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                cum[t] = m5;
+                m5 += re[t] - rs[t];
+            }
+            unsigned long long ck[kSortCands];
+#pragma unroll
+            for (int u = 0; u < kSortCands; ++u) {
+                ck[u] = ~0ull;
+                if ((uint32_t)u < m5) {
+                    uint32_t j = 0;
+#pragma unroll
+                    for (int t = 0; t < 5; ++t)
+                        if ((uint32_t)u >= cum[t] && (uint32_t)u < cum[t] + (re[t] - rs[t]))
+                            j = rs[t] + ((uint32_t)u - cum[t]);
+                    if (j != i) {
+                        const float2 o = a.pos[j];
                         const float dx = o.x - q.x;
                         const float dy = o.y - q.y;
                         float sq = dx * dx;
                         sq = sq + dy * dy;
-                        c = FeatCand{__float_as_uint(sq), __float_as_uint(o.x), __float_as_uint(o.y),
-                                     __float_as_uint(ov2.x), __float_as_uint(ov2.y)};
+                        ck[u] = ((unsigned long long)__float_as_uint(sq) << 32) | j;
                     }
                 }
             }
-        }
-        // Rank among the 32 lanes of the half, equal keys by lane.  By
-        // squared distance alone first; when two real candidates of a half
-        // tie on it, the whole key decides (all lanes redo the count).
-        uint32_t rank = 0;
-        bool tie = false;
-#pragma unroll 8
-        for (int s = 0; s < 32; ++s) {
-            const uint32_t od = (uint32_t)__shfl((int)c.d2, s, 32);
-            rank += od < c.d2 || (od == c.d2 && s < (int)h);
-            tie = tie || (od == c.d2 && c.d2 != kFeatEmpty && s != (int)h);
-        }
-        if (__any(tie)) {
-            rank = 0;
-#pragma unroll 1
-            for (int s = 0; s < 32; ++s) {
-                const FeatCand o = shfl_cand(c, s);
-                rank += cand_before(o, c) || (cand_same(o, c) && s < (int)h);
+#pragma unroll
+            for (int kb = 1; kb <= kSortLog2; ++kb) {
+#pragma unroll
+                for (int jb = kb - 1; jb >= 0; --jb) {
+                    const int kk = 1 << kb, jj = 1 << jb;
+#pragma unroll
+                    for (int u = 0; u < kSortCands; ++u) {
+                        const int l = u ^ jj;
+                        if (l > u) {
+                            const bool up = (u & kk) == 0;
+                            const unsigned long long x = ck[u], y = ck[l];
+                            const bool sw = up ? cand_less(a.pos, a.vel, y, x) : cand_less(a.pos, a.vel, x, y);
+                            ck[u] = sw ? y : x;
+                            ck[l] = sw ? x : y;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kNbrs; ++r) key[r] = ck[r];
+            for (uint32_t u = kSortCands; u < m5; ++u) {
+                uint32_t j = 0;
+#pragma unroll
+                for (int t = 0; t < 5; ++t)
+                    if (u >= cum[t] && u < cum[t] + (re[t] - rs[t])) j = rs[t] + (u - cum[t]);
+                if (j == i) continue;
+                const float2 o = a.pos[j];
+                const float dx = o.x - q.x;
+                const float dy = o.y - q.y;
+                float sq = dx * dx;
+                sq = sq + dy * dy;
+                insert16(a.pos, a.vel, key, ((unsigned long long)__float_as_uint(sq) << 32) | j);
             }
         }
-        // Lane (half base + rank) receives this lane's candidate.
-        const int dst = (int)(((lane & 32u) + rank) * 4u);
-        best.d2 = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)c.d2);
-        best.x = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)c.x);
-        best.y = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)c.y);
-        best.vx = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)c.vx);
-        best.vy = (uint32_t)__builtin_amdgcn_ds_permute(dst, (int)c.vy);
-    }
-    // Lane r < 16 of the half writes neighbour r (zeros past the last).
-    if (valid && h < 16) {
-        uint32_t w0 = 0, w1 = 0;
-        if (best.d2 != kFeatEmpty) {
-            const float ox = __uint_as_float(best.x), oy = __uint_as_float(best.y);
-            const float ovx = __uint_as_float(best.vx), ovy = __uint_as_float(best.vy);
-            w0 = (uint32_t)bf16_bits(ox - q.x) | ((uint32_t)bf16_bits(oy - q.y) << 16);
-            w1 = (uint32_t)bf16_bits(ovx - w.x) | ((uint32_t)bf16_bits(ovy - w.y) << 16);
+        uint4 *row = reinterpret_cast<uint4 *>(a.feat + (size_t)(a.row0 + i) * kFeat);
+#pragma unroll
+        for (int r = 0; r < kNbrs; r += 2) {
+            uint32_t h[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (key[r + u] != ~0ull) {
+                    const uint32_t j = (uint32_t)key[r + u];
+                    const float2 o = a.pos[j];
+                    const float2 ov = a.vel[j];
+                    h[2 * u] = (uint32_t)bf16_bits(o.x - q.x) | ((uint32_t)bf16_bits(o.y - q.y) << 16);
+                    h[2 * u + 1] = (uint32_t)bf16_bits(ov.x - w.x) | ((uint32_t)bf16_bits(ov.y - w.y) << 16);
+                }
+            }
+            row[r / 2] = make_uint4(h[0], h[1], h[2], h[3]);
         }
-        uint2 *row = reinterpret_cast<uint2 *>(a.feat + (size_t)(a.row0 + i) * kFeat);
-        row[h] = make_uint2(w0, w1);
     }
     if (a.sums) {
         __shared__ unsigned long long s_sum[4], s_cnt[4];
@@ -2403,8 +2408,7 @@ hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, cons
 hipError_t launch_learn_features(const FrameParams &p, const FrameBuffers &b, const FeatureArgs &a,
                                  hipStream_t stream) {
     if (p.n == 0) return hipSuccess;
-    // 32 lanes per particle: 8 particles per 256-thread block.
-    hipLaunchKernelGGL(learn_features_kernel, dim3((p.n + 7) / 8), dim3(256), 0, stream, p, b, a);
+    hipLaunchKernelGGL(learn_features_kernel, dim3((p.n + 255) / 256), dim3(256), 0, stream, p, b, a);
     return hipGetLastError();
 }
 
