@@ -472,11 +472,11 @@ def main() -> int:
     ap.add_argument("--learned", action="store_true",
                     help="the learned warm start against the plain step at equal mean overlap "
                          "(trains the network on the device first)")
-    ap.add_argument("--plain-passes", default="32,16",
+    ap.add_argument("--plain-passes", default="64,32,16",
                     help="--learned: passes of the plain step, comma-separated (value: the first)")
     ap.add_argument("--state-passes", type=int, default=4, help="--learned: passes per rollout frame")
     ap.add_argument("--sample-frames", default="6,12", help="--learned: frames sampled for training")
-    ap.add_argument("--train-steps", type=int, default=3000, help="--learned: Adam steps")
+    ap.add_argument("--train-steps", type=int, default=20000, help="--learned: Adam steps")
     ap.add_argument("--eval-frame", type=int, default=20, help="--learned: the checkpoint frame")
     ap.add_argument("--seed", type=int, default=0, help="--learned: initialisation and batch order")
     ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",

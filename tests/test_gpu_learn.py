@@ -15,7 +15,6 @@ Bars:
   fallback), and a trained network must need fewer passes than an untrained
   one to reach the plain solver's mean overlap.
 """
-import os
 import sys
 
 import numpy as np
@@ -121,8 +120,6 @@ def test_gradient_matches_the_oracle(n):
         g, loss = w.learn_gradient(rows, t)
         np.testing.assert_array_equal(w.learn_params(), params)  # unchanged
     ref, ref_loss = ol.gradient(params, rows, t)
-    if os.path.isdir("gpurun_out"):
-        np.savez(f"gpurun_out/gradient_{n}.npz", gpu=g, ref=ref, loss=loss, ref_loss=ref_loss)
     assert abs(loss - ref_loss) <= 1e-3 * ref_loss
     bad = []
     for name, a, b in BLOCKS:
