@@ -280,9 +280,12 @@ def traffic_summary(args) -> int:
     names = {"collide": "collide_kernel", "cells": "cells_kernel", "scan": "scan_kernel",
              "scatter": "scatter_kernel"}
     kernels = {}
+    def find(table, needle):
+        # "::scatter_kernel<" or "::scatter_kernel(", not upload_scatter_kernel
+        return next(k for k in table if f"::{needle}<" in k or f"::{needle}(" in k)
+
     for short, needle in names.items():
-        fk = next(k for k in fetch if needle in k)
-        wk = next(k for k in write if needle in k)
+        fk, wk = find(fetch, needle), find(write, needle)
         f = float(np.median(fetch[fk][-args.profile_steps:])) * 1024 * fetch_scale
         w = float(np.median(write[wk][-args.profile_steps:])) * 1024 * write_scale
         kernels[short] = {"hbm_bytes_per_launch": int(f + w), "fetch_bytes": int(f), "write_bytes": int(w),
