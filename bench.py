@@ -37,22 +37,34 @@ METRIC = "particle-steps/s & %HBM roofline @1M"
 def algorithmic_bytes(n: int, cells: int, split: dict) -> dict:
     """Bytes each kernel must move per launch.  Per particle: the collide
     pass reads position+velocity (16) and writes them (16) plus the cell key
-    (4) -- in collide_kernel for one-particle cells, in cells_kernel for the
-    rest; the scatter reads key (4) + position+velocity (16) and writes them
-    (16).  Per cell (C + 2 offsets): collide_kernel reads one offset (4); the
-    scan reads a count (4), writes an offset (4) and clears the count (4)."""
+    (4) -- in collide_kernel for the particles of cells it handles inside a
+    wavefront, in cells_kernel for the queued cells; the scatter reads key
+    (4) + position+velocity (16) and writes them (16).  Per cell (C + 2
+    offsets): the frame reads one offset (4), counted with collide_kernel,
+    which learns the cells from a 4-byte key per particle instead; the scan
+    reads a count (4), writes an offset (4) and clears the count (4)."""
     c2 = cells + 2
-    return {"collide": 36 * split["single"] + 4 * c2, "cells": 36 * (split["wide"] + split["group"]),
+    return {"collide": 36 * split["collide"] + 4 * c2, "cells": 36 * (n - split["collide"]),
             "scan": 12 * c2, "scatter": 36 * n}
 
 
+IN_WAVE_MAX = 3  # collide_kernel handles cells of up to 3 particles inside a 64-particle wavefront
+
+
 def particles_by_kernel(idx: np.ndarray) -> dict:
-    """Particles in one-particle cells, in cells of 65..1024 (the wide
-    path) and in the other cells of two or more."""
+    """Particles of the cells collide_kernel handles (at most IN_WAVE_MAX
+    particles, inside one 64-particle wavefront window of the packed order),
+    and of the cells it queues for cells_kernel: 2..64 particles (lane
+    groups), 65..704 (the wide path), more (a block each)."""
     cnt = np.diff(idx[1:].astype(np.int64))
-    single = int(cnt[cnt == 1].sum())
-    wide = int(cnt[(cnt >= 65) & (cnt <= 1024)].sum())
-    return {"single": single, "wide": wide, "group": int(cnt.sum()) - single - wide}
+    start = idx[1:-1].astype(np.int64)
+    occ = cnt > 0
+    c, s = cnt[occ], start[occ]
+    inside = (s // 64) == ((s + c - 1) // 64)
+    col = inside & (c <= IN_WAVE_MAX)
+    q = c[~col]
+    return {"collide": int(c[col].sum()), "group": int(q[q <= 64].sum()),
+            "wide": int(q[(q > 64) & (q <= 704)].sum()), "block": int(q[q > 704].sum())}
 
 
 def host_cores() -> dict:

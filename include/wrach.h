@@ -422,7 +422,7 @@ int wrach_learn_profile(wrach_ctx *ctx, uint32_t reps, double *ms);
  * particles_in_frame_count (= n); gravity and dt as in wrach_config.  The
  * frames' other buffers live in one device array of
  * WRACH_NOALLOC_SCRATCH_BYTES per GPU that the library holds until the
- * process ends; a frame that needs more (about 8 bytes per cell plus 92 per
+ * process ends; a frame that needs more (about 8 bytes per cell plus 96 per
  * particle) returns WRACH_E_CAPACITY before any work.  Asynchronous on
  * `stream` (a hipStream_t; NULL: the null stream); calls no hipMalloc,
  * hipFree or allocating routine.  Calls that overlap on one GPU share that

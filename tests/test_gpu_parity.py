@@ -365,6 +365,51 @@ def test_checkpoint_round_trip(oracle):
         assert_same_state(b.read(), (idx, pos, vel), "re-pack of the restored store")
 
 
+def _window_layout(seed: int, sizes):
+    """Positions inside a 64x64 world (cell size 1, cells row-major) whose
+    packed layout holds cells of the given particle counts in order, one
+    per cell from cell 0 on; velocities small and random."""
+    r = np.random.default_rng(seed)
+    gx = 65
+    pos, vel = [], []
+    for c, m in enumerate(sizes):
+        cx, cy = c % gx, c // gx
+        if m == 0 or cx >= 64 or cy >= 64:
+            continue
+        pos.append(np.stack([cx + r.uniform(0.05, 0.95, m), cy + r.uniform(0.05, 0.95, m)], 1))
+        vel.append(r.uniform(-0.6, 0.6, (m, 2)))
+    return f32(np.concatenate(pos)), f32(np.concatenate(vel))
+
+
+@pytest.mark.parametrize("case", ["straddle", "mixed", "wide"])
+def test_cells_across_wavefront_windows(oracle, case):
+    """collide_kernel handles cells of up to 3 particles inside a window of
+    64 packed particles and queues the others (larger, or running past the
+    window).  Cells of 1..5 particles placed across the window edges at 64,
+    128, ..., a cell spanning several windows and one filling a window
+    exactly, loaded as an exact packed triple and stepped against the oracle."""
+    if case == "straddle":
+        # 62 singles, then a 3-cell over 62..64 (across the edge at 64), a 2-cell, ...
+        sizes = [1] * 62 + [3, 2] + [1] * 59 + [4] + [2] * 3 + [5] + [1] * 50 + [2, 3, 4, 5] * 6
+    elif case == "mixed":
+        r = np.random.default_rng(7)
+        sizes = list(r.choice([0, 1, 1, 1, 2, 2, 3, 4, 5, 7], 700))
+    else:
+        # a cell of 150 over three windows, one of 64 filling the window
+        # 192..255 exactly, then cells of 700 and 900
+        sizes = [1] * 40 + [150] + [1] * 2 + [64] + [3] * 10 + [700] + [2] * 30 + [900] + [1] * 20
+    pos, vel = _window_layout(11, sizes)
+    cfg = WrachConfig(dimensions=(64, 64), cell_size=1, gravity=(0.0, -0.05))
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    say(f"[windows] {case}: {len(pos)} particles")
+    with World(cfg) as w:
+        w.load_packed(*ref)
+        for frames in (1, 1, 3, 10):
+            w.step(frames)
+            ref = oracle.step(P, *ref, nsteps=frames)
+            assert_same_state(w.read(), ref, f"{case} after {frames} more")
+
+
 def test_load_packed_rejects_bad_triples(oracle):
     say("[checkpoint] rejected triples leave the state")
     cfg = WrachConfig(dimensions=(20, 12), cell_size=1)

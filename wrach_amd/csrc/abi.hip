@@ -156,6 +156,9 @@ int settle(wrach_ctx *c) {
     if (*c->err_host & wrach::kErrLookBack)
         return fail(WRACH_E_DEVICE, "the exclusive scan's look-back waited past its bound: the offsets "
                                     "of a frame are not valid and the context must be destroyed");
+    if (*c->err_host & wrach::kErrLayout)
+        return fail(WRACH_E_DEVICE, "the collide kernel found particle cells that disagree with the "
+                                    "offsets: the layout is not valid and the context must be destroyed");
     return WRACH_OK;
 }
 
@@ -333,6 +336,7 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->buf.pos);
     (void)hipFree(c->buf.vel);
     (void)hipFree(c->buf.src);
+    (void)hipFree(c->buf.key);
     (void)hipFree(c->buf.pos_out);
     (void)hipFree(c->buf.vel_out);
     (void)hipFree(c->buf.kslot);
@@ -434,6 +438,7 @@ int add_host_packed(wrach_ctx *c, const float *pos_xy, const float *vel_xy, uint
                                  hipMemcpyHostToDevice, c->stream));
     }
     WRACH_HIP(wrach::launch_iota(c->buf.src, packed, c->stream));
+    WRACH_HIP(wrach::launch_keys_from_offsets(c->buf.idx, (uint32_t)c->ncell, packed, c->buf.key, c->stream));
     // every offset came from the host: no chunk's offsets are stale
     WRACH_HIP(hipMemsetAsync(c->buf.chunk_live, 1, c->chunks, c->stream));
     WRACH_SETTLE(c);
@@ -713,6 +718,7 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     WRACH_HIP_C(dmalloc(&b.pos, cap));
     WRACH_HIP_C(dmalloc(&b.vel, cap));
     WRACH_HIP_C(dmalloc(&b.src, cap));
+    WRACH_HIP_C(dmalloc(&b.key, cap));
     WRACH_HIP_C(dmalloc(&b.pos_out, cap));
     WRACH_HIP_C(dmalloc(&b.vel_out, cap));
     WRACH_HIP_C(dmalloc(&b.kslot, cap));
@@ -1266,6 +1272,7 @@ int wrach_load_packed(wrach_ctx *c, const uint32_t *indices, size_t n_indices, c
                                  c->stream));
     }
     WRACH_HIP(wrach::launch_iota(c->buf.src, n, c->stream));
+    WRACH_HIP(wrach::launch_keys_from_offsets(c->buf.idx, (uint32_t)c->ncell, n, c->buf.key, c->stream));
     WRACH_HIP(hipMemsetAsync(c->buf.chunk_live, 1, c->chunks, c->stream));  // offsets from the host
     if (c->host_pack) {
         c->store_pos.assign(pos_xy, pos_xy + 2 * (size_t)n);
@@ -1355,18 +1362,19 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
         if (out) *out = current_tuning(c);
         return WRACH_OK;
     }
-    // The state every trial starts from: offsets, positions, velocities and
-    // source words (counts, list counters and scan words are clean between
-    // frames).
+    // The state every trial starts from: offsets, positions, velocities,
+    // source words and cell keys (counts, list counters and scan words are
+    // clean between frames).
     const size_t nidx = c->ncell + 2;
-    void *s_idx = nullptr, *s_pos = nullptr, *s_vel = nullptr, *s_src = nullptr, *s_live = nullptr,
-         *s_pre = nullptr;
+    void *s_idx = nullptr, *s_pos = nullptr, *s_vel = nullptr, *s_src = nullptr, *s_key = nullptr,
+         *s_live = nullptr, *s_pre = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto cleanup = [&]() {
         (void)hipFree(s_idx);
         (void)hipFree(s_pos);
         (void)hipFree(s_vel);
         (void)hipFree(s_src);
+        (void)hipFree(s_key);
         (void)hipFree(s_live);
         (void)hipFree(s_pre);
         if (e0) (void)hipEventDestroy(e0);
@@ -1379,6 +1387,7 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
     if (e == hipSuccess) e = hipMalloc(&s_pos, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&s_vel, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&s_src, (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&s_key, (size_t)n * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&s_live, c->chunks);
     if (e == hipSuccess) e = hipMalloc(&s_pre, c->chunks * sizeof(uint32_t));
     if (e == hipSuccess) e = hipEventCreate(&e0);
@@ -1387,6 +1396,7 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
     if (e == hipSuccess) e = copy(s_pos, c->buf.pos, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = copy(s_vel, c->buf.vel, (size_t)n * sizeof(float2));
     if (e == hipSuccess) e = copy(s_src, c->buf.src, (size_t)n * sizeof(uint32_t));
+    if (e == hipSuccess) e = copy(s_key, c->buf.key, (size_t)n * sizeof(uint32_t));
     if (e == hipSuccess) e = copy(s_live, c->buf.chunk_live, c->chunks);
     if (e == hipSuccess) e = copy(s_pre, c->buf.chunk_prefix, c->chunks * sizeof(uint32_t));
     if (e != hipSuccess) {
@@ -1398,6 +1408,7 @@ int wrach_autotune(wrach_ctx *c, uint32_t frames, wrach_tuning *out) {
         if (r == hipSuccess) r = copy(c->buf.pos, s_pos, (size_t)n * sizeof(float2));
         if (r == hipSuccess) r = copy(c->buf.vel, s_vel, (size_t)n * sizeof(float2));
         if (r == hipSuccess) r = copy(c->buf.src, s_src, (size_t)n * sizeof(uint32_t));
+        if (r == hipSuccess) r = copy(c->buf.key, s_key, (size_t)n * sizeof(uint32_t));
         if (r == hipSuccess) r = copy(c->buf.chunk_live, s_live, c->chunks);
         if (r == hipSuccess) r = copy(c->buf.chunk_prefix, s_pre, c->chunks * sizeof(uint32_t));
         if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
@@ -2040,6 +2051,7 @@ size_t carve_noalloc(unsigned char *base, uint32_t n, uint64_t ncell, wrach::Fra
     b.pos = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
     b.vel = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
     b.src = reinterpret_cast<uint32_t *>(take(np * sizeof(uint32_t)));
+    b.key = reinterpret_cast<uint32_t *>(take(np * sizeof(uint32_t)));
     b.pos_out = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
     b.vel_out = reinterpret_cast<float2 *>(take(np * sizeof(float2)));
     b.kslot = reinterpret_cast<uint2 *>(take(np * sizeof(uint2)));
@@ -2146,6 +2158,7 @@ int wrach_frame_noalloc(const wrach_world_settings *ws, const float gravity[2], 
         WRACH_HIP(wrach::launch_copy32(reinterpret_cast<uint32_t *>(b.vel), reinterpret_cast<const uint32_t *>(vel_xy),
                                        2 * (size_t)n, st));
         WRACH_HIP(wrach::launch_iota(b.src, n, st));
+        WRACH_HIP(wrach::launch_keys_from_offsets(b.idx, (uint32_t)ncell, n, b.key, st));
         WRACH_HIP(wrach::launch_collide(fp, b, st));
         WRACH_HIP(wrach::launch_cells(fp, b, st));
         WRACH_HIP(wrach::launch_scan(fp, b, st));

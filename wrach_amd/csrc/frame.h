@@ -42,6 +42,9 @@ struct FrameParams {
 constexpr uint32_t kDefaultWaitLimit = 1u << 20;
 // Bits of FrameBuffers::error.
 constexpr uint32_t kErrLookBack = 1u;
+// The collide kernel found a cell whose keys and offsets disagree (a layout
+// no upload or frame produces); it queues nothing for that cell.
+constexpr uint32_t kErrLayout = 2u;
 
 // Device buffers of one world.  `idx` has ncell + 2 entries and is placed 3
 // words past a 256-byte boundary so that &idx[1] is 16-byte aligned (the
@@ -50,6 +53,7 @@ struct FrameBuffers {
     uint32_t *idx;        // [0, start_0, ..., start_{C-1}, N]
     float2 *pos, *vel;    // cell-sorted state; in-cell order is by `src`
     uint32_t *src;        // canonical index of each particle in the previous frame
+    uint32_t *key;        // cell of each packed particle (the segment of idx that holds it)
     float2 *pos_out, *vel_out;  // post-collide state in canonical order
     uint2 *kslot;         // (new cell key, slot inside the new cell)
     unsigned long long *sortbuf;  // merge-sort ping-pong for very full cells
@@ -147,6 +151,11 @@ hipError_t launch_canonical(const FrameParams &p, const FrameBuffers &b, hipStre
 // Fills the idx entries of chunks the last scan found empty (before a readback).
 hipError_t launch_materialize(const FrameParams &p, const FrameBuffers &b, hipStream_t stream);
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream);
+// key[i] = the cell whose range [idx[c + 1], idx[c + 2]) holds packed
+// particle i, for offsets given whole (an upload from the host, a
+// checkpoint): a binary search per particle.
+hipError_t launch_keys_from_offsets(const uint32_t *idx, uint32_t ncell, uint32_t n, uint32_t *key,
+                                    hipStream_t stream);
 // n words of v / of src (kernels, so that a captured stream records no memset).
 hipError_t launch_fill32(uint32_t *dst, uint32_t v, size_t n, hipStream_t stream);
 hipError_t launch_copy32(uint32_t *dst, const uint32_t *src, size_t n, hipStream_t stream);

@@ -1,23 +1,29 @@
 // Frame kernels of the particle step, written for gfx950 (wave64).
 //
 // One frame is four launches:
-//   collide_kernel     four cells per lane (one 128-bit offset load):
-//                      single-particle cells are finished here -- integrate
-//                      and limit (shaders/physics/src/particle.rs:40-82),
-//                      write pos_out/vel_out, compute the new cell key
-//                      (particles_per_cell.wgsl:14-27), claim a slot in the
-//                      new cell with one atomic.  Fuller cells are queued by
-//                      size class.
+//   collide_kernel     one packed particle per lane, 64 consecutive ones per
+//                      wavefront; `key` gives each particle's cell, so a
+//                      cell is a segment of equal keys.  A cell that lies
+//                      inside the wavefront is handled there: ranked by
+//                      `src` (the canonical order), its first 9 particles
+//                      separated (Gauss-Seidel, particles.rs:62-93), every
+//                      particle finished -- integrate and limit
+//                      (shaders/physics/src/particle.rs:40-82) -- and written
+//                      to pos_out/vel_out at its canonical index, the new cell
+//                      key computed (particles_per_cell.wgsl:14-27) and slots
+//                      claimed with one atomic per neighbour direction.  A
+//                      cell that runs past the wavefront's last particle is
+//                      queued by size class.
 //   cells_kernel       the queued cells: 65..kWideMax particles a wavefront
 //                      each plus one Gauss-Seidel wavefront per three cells;
 //                      2..64 particles in lane groups of 2..64 (one particle
 //                      per lane); one block per larger cell.  Every path
 //                      restores the canonical order (by
-//                      `src`), separate the first 9 particles (Gauss-Seidel,
-//                      particles.rs:62-93), finish every particle (the
-//                      overflow ones without collisions, cell.rs:79-95),
-//                      write pos_out/vel_out at the canonical index and
-//                      claim slots with one atomic per neighbour direction.
+//                      `src`), separates the first 9 particles, finishes every
+//                      particle (the overflow ones without collisions,
+//                      cell.rs:79-95), writes pos_out/vel_out at the
+//                      canonical index and claims slots with one atomic per
+//                      neighbour direction.
 //   scan_kernel        single-pass exclusive scan of the per-cell counts with
 //                      decoupled look-back; writes [0, start_0.., N] and clears
 //                      the counts for the next frame (prefix_sum.wgsl +
@@ -26,7 +32,7 @@
 //                      (pack_new_particle_data.wgsl:39-44); records the source
 //                      index so the next collide pass can rebuild the stable
 //                      order that the reference's CPU packer defines
-//                      (spatial_bin.rs:122-142).
+//                      (spatial_bin.rs:122-142), and the new cell in `key`.
 //
 // In-cell order.  Slots inside a new cell come from atomics, so the stored
 // order is a permutation of the canonical one (ascending source index).  A
@@ -144,6 +150,17 @@ __device__ __forceinline__ uint32_t claim(const FrameBuffers &b, uint32_t k, uin
     const uint32_t old = atomicAdd(&b.cnt[k], count);
     if (old == 0) b.chunk_next[k >> kChunkShift] = 1;
     return old;
+}
+
+// First particle of cell k (k == ncell: the end).  Cells of a chunk the last
+// scan found empty all start at the chunk's prefix (their idx entries are
+// stale); a live chunk's are in idx.
+__device__ __forceinline__ uint32_t cell_start(const FrameParams &p, const FrameBuffers &b,
+                                               uint32_t k) {
+    if (k >= p.ncell) return p.n;
+    const uint32_t ch = k >> kChunkShift;
+    const uint32_t s = b.chunk_live[ch] ? b.idx[k + 1] : b.chunk_prefix[ch];
+    return s < p.n ? s : p.n;
 }
 
 // Writes one finished particle at canonical index `at` and claims a slot of
@@ -561,168 +578,13 @@ __device__ __forceinline__ void clamp_range(const FrameParams &p, uint32_t s, ui
     else if (n > p.n - s) n = p.n - s;
 }
 
-// Four consecutive cells per lane (one 128-bit offset load).  Single-particle
-// cells are finished here, their slot atomics in flight together; every
-// fuller cell is queued for the cell kernels, which spread them over the
-// whole GPU (after a few hundred frames under gravity they crowd into a few
-// rows, which would otherwise land on a few CUs).  One global atomic per
-// block and class.
-constexpr int kCellsPerLane = 4;
-
-// Size class of a cell of n particles (kClasses for none: 0 or 1 particle).
-// For 2 <= n <= 64 the class is ceil(log2(n)) - 1.
+// Size class of a queued cell of n particles (kClasses for none: 0 or 1
+// particle).  For 2 <= n <= 64 the class is ceil(log2(n)) - 1.
 __device__ __forceinline__ uint32_t size_class(uint32_t n) {
     if (n < 2) return (uint32_t)kClasses;
     if (n > kWideMax) return (uint32_t)kBlockClass;
     const uint32_t k = 31u - (uint32_t)__builtin_clz(n - 1);
     return k < (uint32_t)kWaveClass ? k : (uint32_t)kWaveClass;
-}
-
-static_assert(kCollideBlock * kCellsPerLane == (int)kChunkCells, "a collide block covers one chunk");
-
-struct CollideLds {
-    uint32_t cnt[kClasses], base[kClasses], in[kClasses], ovf[kClasses];
-};
-
-// One chunk (kCollideCells cells) by one block.  BATCH: the caller has
-// checked that the last scan found the chunk live.
-template <bool BATCH>
-__device__ __forceinline__ void collide_chunk(const FrameParams &p, const FrameBuffers &b,
-                                              uint32_t chunk, CollideLds &sl) {
-    uint32_t *s_cnt = sl.cnt, *s_base = sl.base, *s_in = sl.in, *s_ovf = sl.ovf;
-    if (threadIdx.x < (uint32_t)kClasses) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    // Lane l of a wavefront takes cells w0 + l, w0 + 64 + l, ... of the
-    // wavefront's 256, so that each atomic instruction covers 64 neighbouring
-    // cells (a few cache lines).  A cell's end is the next lane's start; lane
-    // 63 reads it (idx has ncell + 2 entries, entry c + 1 exists for every
-    // c <= ncell).
-    //
-    // A chunk the last scan found empty holds no particle and its offsets
-    // are stale: the block skips it.  One world: the flag is read beside the
-    // offsets (no added latency); a batch has checked it before.
-    const uint32_t w0 = (chunk * kCollideBlock + (threadIdx.x & ~63u)) * kCellsPerLane;
-    const uint32_t ln = threadIdx.x & 63;
-    uint32_t st[kCellsPerLane], nn[kCellsPerLane], cid[kCellsPerLane];
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        cid[j] = w0 + 64 * j + ln;
-        st[j] = cid[j] <= p.ncell ? b.idx[cid[j] + 1] : p.n;
-    }
-    // A frame after a scan that gave up its look-back (b.error) runs no
-    // kernel that indexes by the offsets' contents: the context is dead.
-    // Vector loads (the lane term is 0 but opaque), queued behind the
-    // offsets: waiting for them waits for nothing the offsets do not.
-    const uint32_t dead = b.error[lane_id() >> 6];
-    if (!BATCH) {
-        const uint32_t live = b.chunk_live[chunk + (lane_id() >> 6)];
-        if (!__builtin_amdgcn_readfirstlane(live)) return;  // the whole block: no barrier follows
-    }
-    if (__builtin_amdgcn_readfirstlane(dead)) return;
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        uint32_t end = (uint32_t)__shfl_down((int)st[j], 1, 64);
-        if (ln == 63) {
-            // The next cell's start; past the chunk's last cell it lies in
-            // the next chunk, whose offsets are stale if that chunk is empty
-            // (its prefix is not).  The three loads go out together.
-            const uint32_t nx = cid[j] + 1;
-            if (nx >= p.ncell) {
-                end = p.n;
-            } else {
-                const uint32_t nc = nx >> kChunkShift;
-                const uint32_t at = b.idx[nx + 1];
-                if (nc != chunk) {
-                    const uint32_t lv = b.chunk_live[nc], pre = b.chunk_prefix[nc];
-                    end = lv ? at : pre;
-                } else {
-                    end = at;
-                }
-            }
-        }
-        nn[j] = cid[j] < p.ncell ? end - st[j] : 0u;
-        clamp_range(p, st[j], nn[j]);
-    }
-
-    // Single-particle cells.
-    float px[kCellsPerLane], py[kCellsPerLane], vx[kCellsPerLane], vy[kCellsPerLane];
-    uint32_t key[kCellsPerLane], slot[kCellsPerLane];
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        key[j] = 0;
-        slot[j] = 0;
-        px[j] = py[j] = vx[j] = vy[j] = 0.0f;
-        if (nn[j] == 1) {
-            const float2 q = b.pos[st[j]];
-            const float2 w = b.vel[st[j]];
-            px[j] = q.x; py[j] = q.y; vx[j] = w.x; vy[j] = w.y;
-            finish_particle(p, px[j], py[j], vx[j], vy[j]);
-            key[j] = cell_key(p, px[j], py[j], world_base(p, cid[j]));
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j)
-        if (nn[j] == 1) slot[j] = claim(b, key[j], 1u);
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        if (nn[j] == 1) {
-            b.pos_out[st[j]] = make_float2(px[j], py[j]);
-            b.vel_out[st[j]] = make_float2(vx[j], vy[j]);
-            b.kslot[st[j]] = make_uint2(key[j], slot[j] | kRunFlag);
-        }
-    }
-
-    // Fuller cells: one list per size class.  LDS positions inside the
-    // block, then one global atomic per class.
-    uint32_t cls[kCellsPerLane], at[kCellsPerLane];
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        cls[j] = size_class(nn[j]);
-        at[j] = cls[j] < (uint32_t)kClasses ? atomicAdd(&s_cnt[cls[j]], 1u) : 0u;
-    }
-    __syncthreads();
-    // Slice blockIdx % kSlices; what does not fit goes to the overflow list.
-    const uint32_t slice = blockIdx.x % kSlices;
-    if (threadIdx.x < (uint32_t)kClasses) {
-        const uint32_t k = threadIdx.x, m = s_cnt[k];
-        uint32_t a = 0, in = 0, o = 0;
-        if (m) {
-            a = atomicAdd(&b.class_ctr[ctr_index(k, slice)], m);
-            const uint32_t room = a < b.slice_cap[k] ? b.slice_cap[k] - a : 0u;
-            in = m < room ? m : room;
-            if (m > in) o = atomicAdd(&b.class_ctr[ctr_index(k, kSlices)], m - in);
-        }
-        s_base[k] = a;
-        s_in[k] = in;
-        s_ovf[k] = o;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kCellsPerLane; ++j) {
-        const uint32_t k = cls[j];
-        if (k < (uint32_t)kClasses) {
-            const uint4 ent = make_uint4(st[j], nn[j], cid[j], 0);
-            if (at[j] < s_in[k]) {
-                b.cells[b.class_off[k] + slice * b.slice_cap[k] + s_base[k] + at[j]] = ent;
-            } else {
-                const uint32_t o = s_ovf[k] + at[j] - s_in[k];
-                if (o < b.class_cap[k]) b.cells[b.class_off[k] + kSlices * b.slice_cap[k] + o] = ent;
-            }
-        }
-    }
-}
-
-__global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
-    __shared__ CollideLds sl;
-    // A block per chunk.  A batch, whose worlds are mostly empty grid, reads
-    // the chunk's flag before its offsets (no wasted loads); one world reads
-    // it beside them (no added latency).
-    if (p.world_cells == p.ncell) {
-        collide_chunk<false>(p, b, blockIdx.x, sl);
-        return;
-    }
-    if (!b.chunk_live[blockIdx.x]) return;  // the whole block: no barrier follows
-    collide_chunk<true>(p, b, blockIdx.x, sl);
 }
 
 // Wave-private LDS of a cell wave of the wide path (below).
@@ -744,40 +606,284 @@ struct WaveLds {
 // of a pair fetch the other's position and evaluate collide_pair's arithmetic
 // on the same operands; the lower rank applies -dv, the higher +dv.  All
 // lanes execute the shuffles.
+// Step t of that schedule for the lane holding rank ri of a cell whose
+// first mi ranks collide, ranks held by lanes gbase + r.
+__device__ __forceinline__ void separate_step(float &x, float &y, int ri, int mi, int gbase, int t) {
+    const int lane = (int)lane_id();
+    const int partner = t - ri;
+    const bool left = ri < partner && partner < mi;
+    const bool right = partner >= 0 && partner < ri && ri < mi;
+    const int from = (left || right) ? gbase + partner : lane;
+    const float qx = __shfl(x, from, 64);
+    const float qy = __shfl(y, from, 64);
+    if (left || right) {
+        const float xi = left ? x : qx, yi = left ? y : qy;
+        const float xj = left ? qx : x, yj = left ? qy : y;
+        const float dx = xi - xj;
+        const float dy = yi - yj;
+        float sq = dx * dx;
+        sq = sq + dy * dy;
+        float d = sqrtf(sq);
+        if (!(d > 1.0f)) {
+            if (d == 0.0f) d = 0.0001f;
+            const float f = (0.5f * (1.0f - d)) / d;
+            const float ex = (xj - xi) * f;
+            const float ey = (yj - yi) * f;
+            if (left) {
+                x = x - ex;
+                y = y - ey;
+            } else {
+                x = x + ex;
+                y = y + ey;
+            }
+        }
+    }
+}
+
 template <int G>
 __device__ __forceinline__ void collide_group(float &x, float &y, uint32_t r, uint32_t m,
                                               uint32_t gbase) {
     constexpr int M = G < kMaxInCell ? G : kMaxInCell;
-    const int lane = (int)lane_id();
-    const int ri = (int)r, mi = (int)m;
 #pragma unroll
-    for (int t = 1; t <= 2 * M - 3; ++t) {
-        const int partner = t - ri;
-        const bool left = ri < partner && partner < mi;
-        const bool right = partner >= 0 && partner < ri && ri < mi;
-        const int from = (left || right) ? (int)gbase + partner : lane;
-        const float qx = __shfl(x, from, 64);
-        const float qy = __shfl(y, from, 64);
-        if (left || right) {
-            const float xi = left ? x : qx, yi = left ? y : qy;
-            const float xj = left ? qx : x, yj = left ? qy : y;
-            const float dx = xi - xj;
-            const float dy = yi - yj;
-            float sq = dx * dx;
-            sq = sq + dy * dy;
-            float d = sqrtf(sq);
-            if (!(d > 1.0f)) {
-                if (d == 0.0f) d = 0.0001f;
-                const float f = (0.5f * (1.0f - d)) / d;
-                const float ex = (xj - xi) * f;
-                const float ey = (yj - yi) * f;
-                if (left) {
-                    x = x - ex;
-                    y = y - ey;
-                } else {
-                    x = x + ex;
-                    y = y + ey;
-                }
+    for (int t = 1; t <= 2 * M - 3; ++t) separate_step(x, y, (int)r, (int)m, (int)gbase, t);
+}
+
+// ---------------------------------------------------------------------------
+// collide_kernel: one packed particle per lane, the 64 consecutive particles
+// i0 .. i0 + 63 per wavefront, whose cells come from `key` (written by the
+// previous frame's scatter or by the upload): a cell is a segment of equal
+// keys.
+//  - A cell of at most kInWaveMax particles that lies inside the wavefront is
+//    handled there, with the arithmetic of the cell kernels' lane groups:
+//    ranked by source index, moved to lane (first lane + rank), separated,
+//    finished and written at i0 + lane (its canonical index), slots claimed
+//    with one atomic per direction.  The wavefront reads whole lines of pos,
+//    vel, src and key and writes whole lines of pos_out, vel_out and kslot;
+//    before the collapse under gravity almost every cell is such a segment.
+//    Every lane runs the separation steps of the wavefront's largest such
+//    cell, so the cap bounds them (2 * 3 - 3 = 3 steps); larger cells go to
+//    the lane groups of cells_kernel, which are sorted by size.
+//  - Any other cell whose first particle lies in the wavefront -- larger, or
+//    running past the wavefront's last particle -- is queued by size class
+//    for cells_kernel, with its range from the offsets.  The range must start
+//    at that particle and agree with the keys, or the kernel raises
+//    kErrLayout and queues nothing for it (keys and offsets that disagree
+//    would make queued ranges overlap).
+//  - The lanes of a cell that began before the wavefront leave it to the
+//    wavefront that queued it.  The kernel has no barrier: a wavefront with
+//    nothing to handle or queue returns after the keys, and queued cells go
+//    to their lists with one atomic each (slice = wavefront % kSlices).
+constexpr uint32_t kNoKey = 0xffffffffu;  // past the particles; cell keys stay below 2^31
+constexpr uint32_t kInWaveMax = 3;
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = (uint32_t)__shfl_xor((int)v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// Start and end of cell k (k < ncell) as cell_start gives them, the six
+// loads in flight together.
+__device__ __forceinline__ void cell_range(const FrameParams &p, const FrameBuffers &b, uint32_t k,
+                                           uint32_t &st, uint32_t &en) {
+    const uint32_t c0 = k >> kChunkShift;
+    const uint32_t live0 = b.chunk_live[c0], at0 = b.idx[k + 1], pre0 = b.chunk_prefix[c0];
+    uint32_t live1 = 1, at1 = p.n, pre1 = p.n;
+    if (k + 1 < p.ncell) {
+        const uint32_t c1 = (k + 1) >> kChunkShift;
+        live1 = b.chunk_live[c1];
+        at1 = b.idx[k + 2];
+        pre1 = b.chunk_prefix[c1];
+    }
+    st = live0 ? at0 : pre0;
+    en = live1 ? at1 : pre1;
+    st = st < p.n ? st : p.n;
+    en = en < p.n ? en : p.n;
+}
+
+// 64-particle windows per wavefront (the kernel takes any number: each
+// window's loads and slot atomics are in flight together).  One is the
+// fastest measured: with 2..4 the frame before the collapse is no faster and
+// the collapsed frame is slower.
+constexpr int kCollideWindows = 1;
+
+__global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, FrameBuffers b) {
+    constexpr int W = kCollideWindows;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t base = (blockIdx.x * kWavesPerBlock + wave) * (64u * W);
+    uint32_t k[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t i = base + 64u * j + lane;
+        k[j] = i < p.n ? b.key[i] : kNoKey;
+    }
+    // The keys on either side of the wavefront's particles: lane 0 reads the
+    // one before, lane 63 the one after.
+    uint32_t before = kNoKey, after = kNoKey;
+    if (lane == 0 && base > 0 && base < p.n) before = b.key[base - 1];
+    if (lane == 63 && base + 64u * W < p.n) after = b.key[base + 64u * W];
+    // A frame after one that raised an error (a scan that gave up its
+    // look-back, keys that disagree with the offsets) reads no offsets: the
+    // context is dead.  A vector load beside the keys (no barrier follows in
+    // this kernel, so wavefronts need not agree).
+    const uint32_t dead = b.error[lane >> 6];
+    before = __builtin_amdgcn_readlane(before, 0);
+    after = __builtin_amdgcn_readlane(after, 63);
+
+    // Per window: my cell's first and last lane (tl == 64: past the window),
+    // whether this window handles it (mine) or queues it (queue).
+    int hl[W], tl[W];
+    uint32_t m[W];
+    bool mine[W], queue[W];
+    bool work = false, bad = false;
+    const unsigned long long lt = lanemask_lt(), le = lt | (1ull << lane);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t i = base + 64u * j + lane;
+        const bool valid = i < p.n;
+        uint32_t kp = (uint32_t)__shfl_up((int)k[j], 1, 64);
+        uint32_t kn = (uint32_t)__shfl_down((int)k[j], 1, 64);
+        const uint32_t wb = j == 0 ? before : __builtin_amdgcn_readlane(k[j > 0 ? j - 1 : 0], 63);
+        const uint32_t wa = j == W - 1 ? after : __builtin_amdgcn_readlane(k[j < W - 1 ? j + 1 : 0], 0);
+        if (lane == 0) kp = wb;
+        if (lane == 63) kn = wa;
+        const bool head = valid && k[j] != kp;
+        const bool tail = valid && k[j] != kn;
+        const unsigned long long heads = __ballot(head), tails = __ballot(tail);
+        const unsigned long long hb = heads & le, ta = tails & ~lt;
+        hl[j] = hb ? 63 - (int)__builtin_clzll(hb) : -1;
+        tl[j] = ta ? (int)__builtin_ctzll(ta) : 64;
+        // A key past the grid is a layout no upload or frame writes.
+        const bool kbad = valid && k[j] >= p.ncell;
+        const bool inside = valid && !kbad && hl[j] >= 0 && tl[j] < 64;
+        m[j] = inside ? (uint32_t)(tl[j] - hl[j] + 1) : 0u;
+        mine[j] = inside && m[j] <= kInWaveMax;
+        queue[j] = head && !kbad && !mine[j];
+        work = work || mine[j] || queue[j];
+        bad = bad || kbad;
+    }
+    if (__builtin_amdgcn_readfirstlane(dead)) return;
+    if (bad) atomicOr(b.error, kErrLayout);
+    if (__ballot(work) == 0) return;  // every lane votes; nothing to handle or queue
+
+    // Loads: the queued cells' ranges and the handled particles, all windows.
+    uint32_t qst[W], qen[W], sv[W];
+    float2 q[W], w[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t i = base + 64u * j + lane;
+        qst[j] = qen[j] = 0;
+        if (queue[j]) cell_range(p, b, k[j], qst[j], qen[j]);
+        sv[j] = 0;
+        q[j] = w[j] = make_float2(0.0f, 0.0f);
+        if (mine[j]) {
+            sv[j] = b.src[i] & kSrcMask;
+            q[j] = b.pos[i];
+            w[j] = b.vel[i];
+        }
+    }
+
+    // The first particle of a cell this window does not handle queues it.
+    uint32_t cls[W], qs[W], qn[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t i = base + 64u * j + lane;
+        const uint32_t i0 = base + 64u * j;
+        cls[j] = (uint32_t)kClasses;
+        qs[j] = qn[j] = 0;
+        if (queue[j]) {
+            const bool ok = tl[j] == 64 ? qen[j] >= i0 + 64 : qen[j] == i0 + (uint32_t)tl[j] + 1;
+            if (qst[j] == i && ok && qen[j] <= p.n) {
+                qs[j] = qst[j];
+                qn[j] = qen[j] - qst[j];
+                cls[j] = size_class(qn[j]);
+            } else {
+                atomicOr(b.error, kErrLayout);
+            }
+        }
+    }
+
+    // Handled cells: rank, separate, finish, claim (every window's atomics
+    // in flight together), then write.
+    uint32_t nkey[W], rin[W], leader[W], got[W];
+    bool near[W];
+    float px[W], py[W], vx[W], vy[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t mw = wave_max_u32(mine[j] ? m[j] : 0u);
+        // Canonical rank: source indices inside a cell are distinct.
+        uint32_t rank = 0;
+        for (uint32_t t = 0; t < mw; ++t) {
+            const bool in = mine[j] && t < m[j];
+            const uint32_t o = (uint32_t)__shfl((int)sv[j], in ? hl[j] + (int)t : (int)lane, 64);
+            rank += (in && o < sv[j]) ? 1u : 0u;
+        }
+        const int to = (mine[j] ? hl[j] + (int)rank : (int)lane) * 4;
+        px[j] = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(q[j].x)));
+        py[j] = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(q[j].y)));
+        vx[j] = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(w[j].x)));
+        vy[j] = __int_as_float(__builtin_amdgcn_ds_permute(to, __float_as_int(w[j].y)));
+        // Lane hl + r now holds rank r of its cell (m <= kInWaveMax < 9: every
+        // rank collides).
+        const int r = mine[j] ? (int)lane - hl[j] : 0;
+        const int mi = mine[j] ? (int)m[j] : 0;
+        for (int t = 1; t <= 2 * (int)mw - 3; ++t) separate_step(px[j], py[j], r, mi, mine[j] ? hl[j] : 0, t);
+
+        uint32_t dir = 10;
+        nkey[j] = 0;
+        if (mine[j]) {
+            uint32_t wbase, cx, cy;
+            cell_coords(p, k[j], wbase, cx, cy);
+            finish_particle(p, px[j], py[j], vx[j], vy[j]);
+            dir = direction(p, px[j], py[j], cx, cy, wbase, nkey[j]);
+        }
+        // Lanes of my cell going the same way.
+        unsigned long long same = 0;
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+            const unsigned long long bm = __ballot(dir == (uint32_t)d);
+            same = dir == (uint32_t)d ? bm : same;
+        }
+        const unsigned long long upto = tl[j] >= 63 ? ~0ull : (1ull << (tl[j] + 1)) - 1;
+        const unsigned long long seg = mine[j] ? upto & ~((1ull << hl[j]) - 1) : 0ull;
+        same &= seg;
+        near[j] = dir < 9;
+        rin[j] = (uint32_t)__popcll(same & lt);
+        leader[j] = near[j] ? (uint32_t)__builtin_ctzll(same) : lane;
+        got[j] = 0;
+        if (mine[j] && (!near[j] || rin[j] == 0))
+            got[j] = claim(b, nkey[j], near[j] ? (uint32_t)__popcll(same) : 1u);
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t i = base + 64u * j + lane;
+        const uint32_t first = (uint32_t)__shfl((int)got[j], (int)leader[j], 64);
+        if (mine[j]) {
+            const uint32_t slot = near[j] ? (first + rin[j]) | (rin[j] == 0 ? kRunFlag : 0u) : (got[j] | kRunFlag);
+            b.pos_out[i] = make_float2(px[j], py[j]);
+            b.vel_out[i] = make_float2(vx[j], vy[j]);
+            b.kslot[i] = make_uint2(nkey[j], slot);
+        }
+    }
+
+    // Queued cells, one global atomic each into slice (wavefront % kSlices)
+    // of their class; what does not fit goes to the class's overflow list.
+    const uint32_t slice = (blockIdx.x * kWavesPerBlock + wave) % kSlices;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const uint32_t c = cls[j];
+        if (c < (uint32_t)kClasses) {
+            const uint4 ent = make_uint4(qs[j], qn[j], k[j], 0);
+            const uint32_t a = atomicAdd(&b.class_ctr[ctr_index((int)c, slice)], 1u);
+            if (a < b.slice_cap[c]) {
+                b.cells[b.class_off[c] + slice * b.slice_cap[c] + a] = ent;
+            } else {
+                const uint32_t o = atomicAdd(&b.class_ctr[ctr_index((int)c, kSlices)], 1u);
+                if (o < b.class_cap[c]) b.cells[b.class_off[c] + kSlices * b.slice_cap[c] + o] = ent;
             }
         }
     }
@@ -1568,7 +1674,13 @@ __global__ __launch_bounds__(THREADS) void scan_kernel(FrameParams p, FrameBuffe
 // flight for all of them at once.
 template <int PER>
 __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, FrameBuffers b) {
-    const uint32_t w0 = (blockIdx.x * kScatterBlock + (threadIdx.x & ~63u)) * PER;
+    // Blocks go to the 8 XCDs in turn (block b to XCD b % 8): block b takes
+    // tile (b % 8) * (grid / 8) + b / 8, so that each XCD scatters one
+    // contiguous eighth of the particles.  Their destinations then share
+    // lines mostly with destinations of the same XCD, whose L2 merges the
+    // partial lines before writing them back.  The grid is a multiple of 8.
+    const uint32_t tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t w0 = (tile * kScatterBlock + (threadIdx.x & ~63u)) * PER;
     const uint32_t ln = threadIdx.x & 63;
     if (blockIdx.x == 0 && threadIdx.x == 0) *b.epoch = *b.epoch + 1;  // the frame's scan has finished
     uint2 ks[PER];
@@ -1600,6 +1712,7 @@ __global__ __launch_bounds__(kScatterBlock) void scatter_kernel(FrameParams p, F
                 b.pos[dest] = q[j];
                 b.vel[dest] = v[j];
                 b.src[dest] = i | (ks[j].y & kRunFlag);
+                b.key[dest] = ks[j].x;
             }
         }
     }
@@ -1630,6 +1743,21 @@ __global__ void copy32_kernel(uint32_t *dst, const uint32_t *src, size_t n) {
 __global__ void iota_kernel(uint32_t *dst, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = i;
+}
+
+// key[i]: the last cell c whose start idx[c + 1] is at most i (offsets given
+// whole, non-decreasing from idx[1] = 0, so that cell's range holds i).
+__global__ void keys_from_offsets_kernel(const uint32_t *idx, uint32_t ncell, uint32_t n,
+                                         uint32_t *key) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = ncell;  // idx[lo + 1] <= i < idx[hi + 1] (idx[ncell + 1] = n)
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (idx[mid + 1] <= i) lo = mid;
+        else hi = mid;
+    }
+    key[i] = lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -1739,6 +1867,7 @@ __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, Fra
     b.pos[dest] = u.pos[i];
     b.vel[dest] = u.vel[i];
     b.src[dest] = i | kRunFlag;
+    b.key[dest] = ks.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -1756,16 +1885,6 @@ __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, Fra
 constexpr float kStencilScale = 1073741824.0f;            // 2^30
 constexpr float kStencilUnit = 9.313225746154785e-10f;    // 2^-30
 
-// First particle of cell k (k == ncell: the end).  Cells of a chunk the last
-// scan found empty all start at the chunk's prefix (their idx entries are
-// stale); a live chunk's are in idx.
-__device__ __forceinline__ uint32_t cell_start(const FrameParams &p, const FrameBuffers &b,
-                                               uint32_t k) {
-    if (k >= p.ncell) return p.n;
-    const uint32_t ch = k >> kChunkShift;
-    const uint32_t s = b.chunk_live[ch] ? b.idx[k + 1] : b.chunk_prefix[ch];
-    return s < p.n ? s : p.n;
-}
 
 // The particles of the 3x3 cells around (cx, cy) as three ranges, one per
 // row cy - 1 .. cy + 1: the cells of one row are contiguous in the packed
@@ -2258,8 +2377,11 @@ static hipError_t launch(K kernel, uint32_t grid, uint32_t block, hipStream_t st
 
 hipError_t launch_collide(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                           hipEvent_t start, hipEvent_t stop) {
-    const uint32_t chunks = (p.ncell + kChunkCells - 1) / kChunkCells;
-    return launch(collide_kernel, chunks, kCollideBlock, stream, start, stop, p, b);
+    // One block of 4 wavefronts per 256 * kCollideWindows packed particles
+    // (one block for none: the events of a profiled frame still get a kernel).
+    constexpr uint32_t per_block = kCollideBlock * kCollideWindows;
+    const uint32_t blocks = p.n ? (p.n + per_block - 1) / per_block : 1u;
+    return launch(collide_kernel, blocks, kCollideBlock, stream, start, stop, p, b);
 }
 
 hipError_t launch_cells(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
@@ -2325,8 +2447,8 @@ static hipError_t launch_scatter_as(const FrameParams &p, const FrameBuffers &b,
                                     hipEvent_t start, hipEvent_t stop) {
     constexpr int per = kScatterPerLane[S];
     constexpr uint32_t per_block = kScatterBlock * per;
-    return launch(scatter_kernel<per>, (p.n + per_block - 1) / per_block, kScatterBlock, stream,
-                  start, stop, p, b);
+    const uint32_t tiles = (p.n + per_block - 1) / per_block;
+    return launch(scatter_kernel<per>, (tiles + 7) / 8 * 8, kScatterBlock, stream, start, stop, p, b);
 }
 
 hipError_t launch_scatter(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
@@ -2361,6 +2483,14 @@ hipError_t launch_copy32(uint32_t *dst, const uint32_t *src, size_t n, hipStream
 hipError_t launch_iota(uint32_t *dst, uint32_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_keys_from_offsets(const uint32_t *idx, uint32_t ncell, uint32_t n, uint32_t *key,
+                                    hipStream_t stream) {
+    if (n == 0 || ncell == 0) return hipSuccess;
+    hipLaunchKernelGGL(keys_from_offsets_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, idx, ncell,
+                       n, key);
     return hipGetLastError();
 }
 
