@@ -175,62 +175,54 @@ MANY_WORLDS = "many-worlds-512"
 
 def bench_many_worlds(args) -> int:
     """512 independent worlds (sides 256/512/1024, 1,000..65,536 particles
-    each): the worlds of one side form one batch (wrach_create_batch, one
-    launch per frame kernel for all of them), the three batches run on their
-    own streams.  A frame is one frame of every batch: per-frame wall time
-    (host clock around step + sync of the three) gives p50/p99; value is all
-    particles x frames over the summed frame times."""
+    each) as one batch of worlds of different sizes (wrach_create_worlds):
+    one launch per frame kernel for all of them over a flattened (world,
+    cell) key space.  Per-frame wall time (host clock around step(1) + sync,
+    one-frame graph) gives p50/p99; value is all particles x frames over the
+    summed frame times."""
     from wrach_amd import World, workloads
 
     ws = workloads.many_worlds()
-    by_side = {}
-    for w in ws:
-        by_side.setdefault(w.config.dimensions[0], []).append(w)
-    batches = []
     t_up = time.perf_counter()
-    for side in sorted(by_side):
-        group = by_side[side]
-        b = World(group[0].config, worlds=len(group))
-        for k, w in enumerate(group):
-            b.add_particles(w.pos, w.vel, world=k)
-        b.set_graph(not args.no_graph)
-        if args.autotune > 0:
-            b.step(args.warmup)
-            b.autotune(args.autotune)
-        batches.append((side, group, b))
+    b = World([w.config for w in ws])
+    for k, w in enumerate(ws):
+        b.add_particles(w.pos, w.vel, world=k)
+    b.set_graph(not args.no_graph)
+    if args.autotune > 0:
+        b.step(args.warmup)
+        b.autotune(args.autotune)
+    b.sync()
     upload_s = time.perf_counter() - t_up
     n = sum(w.n for w in ws)
-    cells = sum(len(g) * g[0].cells for _, g, _ in batches)
-    print(f"[bench] {MANY_WORLDS}: {len(ws)} worlds in {len(batches)} batches, N={n} cells={cells}, "
-          f"upload {upload_s:.2f} s", file=sys.stderr, flush=True)
-    for _, _, b in batches:
-        b.step(args.warmup)
-        b.prepare(1)  # the one-frame graph the timed loop replays
-    for _, _, b in batches:
-        b.sync()
+    cells = b.n_indices - 2
+    print(f"[bench] {MANY_WORLDS}: {len(ws)} worlds in one batch, N={n} cells={cells} (with the padding "
+          f"that puts every grid on a 1,024-cell boundary), upload {upload_s:.2f} s", file=sys.stderr, flush=True)
+    b.step(args.warmup)
+    b.prepare(1)  # the one-frame graph the timed loop replays
+    b.sync()
     times = []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        for _, _, b in batches:
-            b.step(1)
-        for _, _, b in batches:
-            b.sync()
+        b.step(1)
+        b.sync()
         times.append(time.perf_counter() - t0)
     times = np.array(times)
     elapsed = float(times.sum())
     value = n * args.steps / elapsed
-    step_bytes = 72 * n + 16 * (cells + 2 * len(batches))
-    per_batch = {}
-    for side, group, b in batches:
-        ms = b.step_profiled(min(args.profile_steps, 50))
-        per_batch[str(side)] = {"worlds": len(group), "particles": sum(w.n for w in group),
-                                "cells": len(group) * group[0].cells,
-                                "launch_shapes": {k: v for k, v in b.tuning().items() if k != "frame_us"},
-                                "kernel_avg_us": {k: 1e3 * v / min(args.profile_steps, 50)
-                                                  for k, v in ms.items()}}
-        b.close()
-    print(f"[bench] p50 {np.percentile(times, 50) * 1e3:.3f} ms p99 {np.percentile(times, 99) * 1e3:.3f} ms",
-          file=sys.stderr, flush=True)
+    grid_cells = sum(w.cells for w in ws)
+    step_bytes = 72 * n + 16 * (grid_cells + 2)
+    prof = min(args.profile_steps, 50)
+    ms = b.step_profiled(prof)
+    kernel_avg_us = {k: 1e3 * v / prof for k, v in ms.items()}
+    shapes = {k: v for k, v in b.tuning().items() if k != "frame_us"}
+    b.close()
+    print(f"[bench] p50 {np.percentile(times, 50) * 1e3:.3f} ms p99 {np.percentile(times, 99) * 1e3:.3f} ms "
+          f"kernels {kernel_avg_us}", file=sys.stderr, flush=True)
+    sides = {}
+    for w in ws:
+        d = sides.setdefault(str(w.config.dimensions[0]), {"worlds": 0, "particles": 0})
+        d["worlds"] += 1
+        d["particles"] += w.n
     line = {
         "metric": "particle-steps/s, batched worlds (p50/p99 frame time)",
         "value": value,
@@ -244,15 +236,17 @@ def bench_many_worlds(args) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": MANY_WORLDS, "worlds": len(ws), "particles": n, "cells": cells,
-                   "batches": per_batch, "graph": not args.no_graph,
+        "config": {"workload": MANY_WORLDS, "worlds": len(ws), "particles": n, "grid_cells": grid_cells,
+                   "key_space_cells": cells, "batch": "one (wrach_create_worlds)", "sides": sides,
+                   "launch_shapes": shapes, "graph": not args.no_graph,
                    "p50_frame_ms": float(np.percentile(times, 50) * 1e3),
                    "p99_frame_ms": float(np.percentile(times, 99) * 1e3),
-                   "frame_clock": "host, step(1) of every batch + sync"},
+                   "frame_clock": "host, step(1) + sync"},
         "roofline": {"bound": "hbm", "achieved": step_bytes / (elapsed / args.steps) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "scope": "whole frame of all batches, host clock"},
+                     "traffic": None, "scope": "whole frame, host clock",
+                     "kernel_avg_us": kernel_avg_us, "kernel_frames": "the profiled frames after the timed ones"},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

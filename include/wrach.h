@@ -187,19 +187,36 @@ int wrach_load_packed(wrach_ctx *ctx, const uint32_t *indices, size_t n_indices,
  * n_worlds = 1. */
 int wrach_create_batch(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out);
 
+/* A batch of worlds of different sizes (no counterpart in the reference):
+ * world w has the dimensions of cfgs[w]; every other field (cell_size,
+ * boundaries_as_dimensions, gravity, dt, flags, device) must be the same in
+ * all n_worlds entries (WRACH_E_INVALID otherwise).  One launch per frame
+ * kernel for all worlds, as for wrach_create_batch: every world's grid
+ * starts on a multiple of 1,024 cells of the flattened key space (the cells
+ * between grids stay empty), and the kernels take each cell's world from
+ * its chunk of 1,024.  The capacity is the sum of the worlds'.  Entries of
+ * one size give wrach_create_batch's layout. */
+int wrach_create_worlds(const wrach_config *cfgs, uint32_t n_worlds, wrach_ctx **out);
+
 /* wrach_add_particles for world `world` of a batch (0 for a single world):
  * appends to the store (tagged with the world) and re-packs the batch. */
 int wrach_add_particles_to_world(wrach_ctx *ctx, uint32_t world, const float *pos_xy,
                                  const float *vel_xy, uint32_t n);
 
 /* wrach_read of one world of a batch: its particles and its own offsets
- * [0, start_0, ..., N_world] over its C + 2 entries. */
+ * [0, start_0, ..., N_world] over its C + 2 entries (C: that world's grid
+ * cells). */
 int wrach_read_world(wrach_ctx *ctx, uint32_t world, float *pos_xy, float *vel_xy,
                      size_t particle_cap, uint32_t *indices, size_t indices_cap,
                      uint32_t *n_particles);
 
-/* Worlds in the context (1 unless made by wrach_create_batch). */
+/* Worlds in the context (1 unless made by wrach_create_batch or
+ * wrach_create_worlds). */
 int wrach_worlds(wrach_ctx *ctx, uint32_t *n_worlds);
+
+/* wrach_settings of one world of a batch: its view and grid dimensions, the
+ * cell size and its particles (synchronises). */
+int wrach_settings_world(wrach_ctx *ctx, uint32_t world, wrach_world_settings *out);
 
 /* Launch shapes of the frame (no counterpart in the reference, whose passes
  * have fixed workgroup sizes).  Each field takes one of the listed values. */

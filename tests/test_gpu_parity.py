@@ -608,6 +608,122 @@ def test_many_worlds_batches_of_sides_512_and_1024(oracle, side):
         assert b.read()[0][-1] == sum(w.n for w in ws)
 
 
+def _mixed_worlds(cs, seed):
+    """Worlds of different sizes (sides 48x32, 100x100, 17x65, 256x40 and a
+    1x1), of uneven particle counts, crowded cells among them."""
+    r = np.random.default_rng(seed)
+    dims = [(48, 32), (100, 100), (17, 65), (256, 40), (1, 1)]
+    counts = [300, 2500, 0, 900, 3]
+    out = []
+    for (W, H), n in zip(dims, counts):
+        p = r.uniform(0, [W, H], (n, 2))
+        if n >= 900:  # a crowded cell or two
+            p[: n // 4] = np.array([W / 3, H / 2]) + r.uniform(-0.4, 0.4, (n // 4, 2))
+        out.append((WrachConfig(dimensions=(W, H), cell_size=cs, gravity=(0.0, -0.1)),
+                    f32(np.clip(p, 0, [W, H])), f32(r.uniform(-1, 1, (n, 2)))))
+    return out
+
+
+@pytest.mark.parametrize("cs", [1, 3])
+def test_worlds_of_different_sizes_match_each_world_alone(oracle, cs):
+    """wrach_create_worlds: worlds of different sizes in one key space (each
+    grid on a 1,024-cell boundary), stepped together; each world bit for bit
+    against the single-world oracle after the upload, after frames and after
+    an append, and its settings (wrach_settings_world)."""
+    ws = _mixed_worlds(cs, 90 + cs)
+    say(f"[mixed] {len(ws)} worlds of different sizes, cs={cs}")
+    with World([c for c, _, _ in ws]) as b:
+        assert b.worlds == len(ws)
+        for w, (c, p, v) in enumerate(ws):
+            b.add_particles(p, v, world=w)
+        refs = [oracle_world(oracle, c, p, v) for c, p, v in ws]
+        for w, ((c, p, _), (P, ref)) in enumerate(zip(ws, refs)):
+            s_ = b.world_settings(w)
+            gx, gy = c.dimensions[0] // cs + 1, c.dimensions[1] // cs + 1
+            assert (s_.grid_dimensions[0], s_.grid_dimensions[1]) == (gx, gy)
+            assert tuple(s_.view_dimensions) == tuple(float(d) for d in c.dimensions)
+            assert s_.particles_in_frame_count == len(p)
+            assert_same_state(b.read_world(w), ref, f"world {w} upload")
+        for frames in (1, 16, 7):
+            b.step(frames)
+            refs = [(P, oracle.step(P, *ref, nsteps=frames)) for P, ref in refs]
+            for w, (P, ref) in enumerate(refs):
+                assert_same_state(b.read_world(w), ref, f"world {w} after +{frames}")
+        extra = f32([[5.5, 5.5], [6.5, 5.5], [12.25, 30.75]])
+        c1, p1, v1 = ws[1]
+        b.add_particles(extra, f32(np.zeros_like(extra)), world=1)
+        ws[1] = (c1, np.concatenate([p1, extra]), np.concatenate([v1, np.zeros_like(extra)]))
+        refs = [oracle_world(oracle, c, p, v) for c, p, v in ws]
+        b.step(5)
+        for w, (P, ref) in enumerate(refs):
+            assert_same_state(b.read_world(w), oracle.step(P, *ref, nsteps=5), f"world {w} after the append")
+        assert b.read()[0][-1] == sum(len(p) for _, p, _ in ws)
+
+
+def test_worlds_of_different_sizes_mode_s(oracle):
+    """Mode S passes over a batch of worlds of different sizes: every
+    world's 3x3 cells clipped to its own grid, each world bit for bit against
+    the oracle's passes alone."""
+    ws = _mixed_worlds(1, 95)
+    say("[mixed] mode S, 2 passes")
+    with World([c for c, _, _ in ws]) as b:
+        for w, (c, p, v) in enumerate(ws):
+            b.add_particles(p, v, world=w)
+        b.set_solver("S", 2)
+        refs = [oracle_world(oracle, c, p, v) for c, p, v in ws]
+        b.step(4)
+        for w, (P, ref) in enumerate(refs):
+            assert_same_state(b.read_world(w), oracle.step_s(P, *ref, 2, nsteps=4)[:3], f"world {w}, mode S")
+
+
+def test_worlds_of_different_sizes_argument_errors(oracle):
+    say("[mixed] errors")
+    a = WrachConfig(dimensions=(16, 16), cell_size=1)
+    with pytest.raises(WrachError) as e:
+        World([a, WrachConfig(dimensions=(32, 8), cell_size=2)])
+    assert e.value.code == _lib.WRACH_E_INVALID
+    with pytest.raises(WrachError) as e:
+        World([a, WrachConfig(dimensions=(32, 8), cell_size=1, gravity=(0.0, -0.1))])
+    assert e.value.code == _lib.WRACH_E_INVALID
+    with World([a, WrachConfig(dimensions=(32, 8), cell_size=1)]) as b:
+        with pytest.raises(WrachError) as e:
+            b.world_settings(2)
+        assert e.value.code == _lib.WRACH_E_INVALID
+        with pytest.raises(WrachError) as e:
+            b.add_particles(f32([[1.5, 1.5]]), f32([[0.0, 0.0]]), world=2)
+        assert e.value.code == _lib.WRACH_E_INVALID
+        with pytest.raises(WrachError):
+            b.add_particles(f32([[1.5, 1.5]]), f32([[0.0, 0.0]]))
+    # entries of one size: the equal-world batch
+    with World([a, a, a]) as b:
+        assert b.worlds == 3 and b.n_indices == 3 * 17 * 17 + 2
+
+
+def test_many_worlds_in_one_batch(oracle):
+    """The 512-world workload (sides 256, 512 and 1024) as one batch of
+    worlds of different sizes: one launch per frame kernel.  Two worlds of
+    each side bit for bit against the single-world oracle after 10 frames,
+    and the batch's particle total."""
+    ws = workloads.many_worlds()
+    say(f"[mixed] many-worlds: {len(ws)} worlds in one batch, {sum(w.n for w in ws)} particles")
+    with World([w.config for w in ws]) as b:
+        for k, w in enumerate(ws):
+            b.add_particles(w.pos, w.vel, world=k)
+        assert b.n_particles == sum(w.n for w in ws)
+        b.step(10)
+        b.sync()
+        say("[mixed] many-worlds: 10 frames done")
+        picks = []
+        for side in (256, 512, 1024):
+            ks = [k for k, w in enumerate(ws) if w.config.dimensions[0] == side]
+            picks += [ks[0], ks[-1]]
+        for k in picks:
+            P, ref = oracle_world(oracle, ws[k].config, ws[k].pos, ws[k].vel)
+            assert_same_state(b.read_world(k), oracle.step(P, *ref, nsteps=10, nthreads=THREADS),
+                              f"world {k}, side {ws[k].config.dimensions[0]}")
+        assert b.read()[0][-1] == sum(w.n for w in ws)
+
+
 def test_batch_argument_errors(oracle):
     say("[batch] errors")
     cfg = WrachConfig(dimensions=(16, 16), cell_size=1)

@@ -148,6 +148,8 @@ extern "C" {
                             particle_cap: usize, indices: *mut u32, indices_cap: usize,
                             n_particles: *mut u32) -> c_int;
     pub fn wrach_worlds(ctx: *mut wrach_ctx, n_worlds: *mut u32) -> c_int;
+    pub fn wrach_create_worlds(cfgs: *const wrach_config, n_worlds: u32, out: *mut *mut wrach_ctx) -> c_int;
+    pub fn wrach_settings_world(ctx: *mut wrach_ctx, world: u32, out: *mut wrach_world_settings) -> c_int;
     pub fn wrach_get_tuning(ctx: *mut wrach_ctx, out: *mut wrach_tuning) -> c_int;
     pub fn wrach_set_tuning(ctx: *mut wrach_ctx, tuning: *const wrach_tuning) -> c_int;
     pub fn wrach_autotune(ctx: *mut wrach_ctx, frames: u32, out: *mut wrach_tuning) -> c_int;

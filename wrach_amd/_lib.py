@@ -38,6 +38,8 @@ EXPORTS = (
     "wrach_config_default",
     "wrach_create",
     "wrach_create_batch",
+    "wrach_create_worlds",
+    "wrach_settings_world",
     "wrach_add_particles_to_world",
     "wrach_read_world",
     "wrach_worlds",
@@ -184,6 +186,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_device_state.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
     L.wrach_canonical_device.argtypes = [vp, ctypes.POINTER(DeviceViewC)]
     L.wrach_create_batch.argtypes = [ctypes.POINTER(WrachConfigC), ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.wrach_create_worlds.argtypes = [ctypes.POINTER(WrachConfigC), ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.wrach_settings_world.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(WorldSettingsC)]
     L.wrach_add_particles_to_world.argtypes = [vp, ctypes.c_uint32, f32p, f32p, ctypes.c_uint32]
     L.wrach_read_world.argtypes = [vp, ctypes.c_uint32, f32p, f32p, ctypes.c_size_t, u32p,
                                    ctypes.c_size_t, u32p]

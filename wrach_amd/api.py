@@ -90,20 +90,27 @@ class WrachState:
 class World:
     """One simulated world on one GPU (a wrach_ctx)."""
 
-    def __init__(self, config: WrachConfig, worlds: int = 1):
+    def __init__(self, config, worlds: int = 1):
         """worlds > 1: a batch of equal worlds stepped together
-        (wrach_create_batch); add_particles(..., world=w) and read_world(w)
-        address one of them."""
+        (wrach_create_batch); a list of configurations: a batch of worlds of
+        different sizes (wrach_create_worlds).  add_particles(..., world=w)
+        and read_world(w) address one of them."""
         L = _lib.load()
         self._L = L
-        self.config = config
-        self.worlds = int(worlds)
         h = ctypes.c_void_p()
-        cfg = config.to_c()
-        if self.worlds == 1:
-            check(L.wrach_create(ctypes.byref(cfg), ctypes.byref(h)))
+        if isinstance(config, (list, tuple)):
+            self.config = config[0]
+            self.worlds = len(config)
+            cfgs = (_lib.WrachConfigC * self.worlds)(*[c.to_c() for c in config])
+            check(L.wrach_create_worlds(cfgs, self.worlds, ctypes.byref(h)))
         else:
-            check(L.wrach_create_batch(ctypes.byref(cfg), self.worlds, ctypes.byref(h)))
+            self.config = config
+            self.worlds = int(worlds)
+            cfg = config.to_c()
+            if self.worlds == 1:
+                check(L.wrach_create(ctypes.byref(cfg), ctypes.byref(h)))
+            else:
+                check(L.wrach_create_batch(ctypes.byref(cfg), self.worlds, ctypes.byref(h)))
         self._h = h
         cap, nidx = ctypes.c_uint32(), ctypes.c_uint32()
         check(L.wrach_capacity(h, ctypes.byref(cap), ctypes.byref(nidx)))
@@ -139,10 +146,18 @@ class World:
             check(self._L.wrach_add_particles_to_world(self._h, int(world), pos.ctypes.data_as(_f32p),
                                                        vel.ctypes.data_as(_f32p), len(pos)))
 
+    def world_settings(self, world: int):
+        """One world's settings (wrach_settings_world): view and grid
+        dimensions, cell size, particles."""
+        s = _lib.WorldSettingsC()
+        check(self._L.wrach_settings_world(self._h, int(world), ctypes.byref(s)))
+        return s
+
     def read_world(self, world: int):
         """One world of a batch: (indices, positions[N,2], velocities[N,2])."""
-        cells = (self.n_indices - 2) // self.worlds  # one world's grid
-        n = self.n_particles
+        ws = self.world_settings(world)
+        cells = ws.grid_dimensions[0] * ws.grid_dimensions[1]
+        n = ws.particles_in_frame_count
         pos = np.zeros((max(n, 1), 2), np.float32)
         vel = np.zeros((max(n, 1), 2), np.float32)
         idx = np.zeros(cells + 2, np.uint32)

@@ -98,7 +98,13 @@ struct wrach_ctx {
     size_t dstore_n = 0, dstore_cap = 0;
     uint32_t *dstore_world = nullptr;  // world of each store particle (a batch)
     uint32_t worlds = 1;               // worlds of a batch (one grid each, keys one after another)
-    uint64_t world_cells = 0;          // cells of one world
+    uint64_t world_cells = 0;          // cells of one world (worlds of one size)
+    // Worlds of different sizes: each world's grid and configuration on the
+    // host, the grids and the world of every chunk on the device.
+    std::vector<wrach::WorldGrid> grids;
+    std::vector<wrach_config> world_cfg;
+    wrach::WorldGrid *dgrids = nullptr;
+    uint32_t *dchunk_world = nullptr;
     int32_t blx = 0, bly = 0;  // the grid's bottom-left cell: view_anchor / cell_size
     uint32_t cells_grid0 = 1;  // the cells kernel's grid chosen at creation
     uint64_t chunks = 0;       // chunks of kChunkCells cells
@@ -354,6 +360,8 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->dstore_vel);
     (void)hipFree(c->dks);
     (void)hipFree(c->dstore_world);
+    (void)hipFree(c->dgrids);
+    (void)hipFree(c->dchunk_world);
     (void)hipFree(c->dvalid);
     (void)hipFree(c->buf.cells);
     (void)hipFree(c->buf.class_ctr);
@@ -540,6 +548,7 @@ wrach::UploadParams store_params(const wrach_ctx *c, size_t n) {
     u.cap = c->capacity;
     u.world = c->worlds > 1 ? c->dstore_world : nullptr;
     u.world_cells = (uint32_t)c->world_cells;
+    u.grids = c->dgrids;
     u.blx = c->blx;
     u.bly = c->bly;
     return u;
@@ -611,7 +620,10 @@ void wrach_config_default(wrach_config *cfg) {
 
 namespace {
 
-int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
+// world_cfgs (n_worlds entries) or null: worlds of different sizes, each
+// world's dimensions from its entry, every other field from cfg (entry 0).
+int create_ctx(const wrach_config *cfg, uint32_t n_worlds, const wrach_config *world_cfgs,
+               wrach_ctx **out) {
     if (!cfg || !out) return fail(WRACH_E_INVALID, "wrach_create: null argument");
     *out = nullptr;
     if (cfg->cell_size == 0) return fail(WRACH_E_INVALID, "cell_size must be >= 1");
@@ -631,12 +643,40 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     cell_coord((float)cfg->width, (float)cfg->height, cs, tx, ty);
     const uint64_t gx = (uint64_t)tx + 1, gy = (uint64_t)ty + 1;
     const uint64_t world_cells = gx * gy;
-    const uint64_t ncell = world_cells * n_worlds;  // a batch: the worlds' grids one after another
-    if (ncell > kMaxCells)
-        return fail(WRACH_E_TOO_MANY_CELLS, "grid has more cells than the 32-bit offsets allow");
-    const uint32_t world_cap = max_particles_per_frame(world_cells, cfg->cell_size);
-    if (world_cap == 0) return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity does not fit in u32");
-    const uint64_t cap64 = (uint64_t)world_cap * n_worlds;
+    uint64_t ncell = world_cells * n_worlds;  // a batch: the worlds' grids one after another
+    uint64_t cap64 = 0;
+    std::vector<wrach::WorldGrid> grids;
+    if (world_cfgs) {
+        // Each world's grid starts on a chunk boundary, so that the chunk of
+        // a cell names its world (the padding cells between grids stay empty).
+        uint64_t at = 0;
+        for (uint32_t w = 0; w < n_worlds; ++w) {
+            int32_t wx, wy;
+            cell_coord((float)world_cfgs[w].width, (float)world_cfgs[w].height, cs, wx, wy);
+            const uint64_t wgx = (uint64_t)wx + 1, wgy = (uint64_t)wy + 1, wcells = wgx * wgy;
+            at = (at + wrach::kChunkCells - 1) / wrach::kChunkCells * wrach::kChunkCells;
+            if (at + wcells > kMaxCells)
+                return fail(WRACH_E_TOO_MANY_CELLS, "grid has more cells than the 32-bit offsets allow");
+            const uint32_t wcap = max_particles_per_frame(wcells, cfg->cell_size);
+            if (wcap == 0) return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity does not fit in u32");
+            wrach::WorldGrid g{};
+            g.base = (uint32_t)at;
+            g.gx = (uint32_t)wgx;
+            g.gy = (uint32_t)wgy;
+            g.x1 = 0.0f + (float)world_cfgs[w].width;
+            g.y1 = 0.0f + (float)world_cfgs[w].height;
+            grids.push_back(g);
+            cap64 += wcap;
+            at += wcells;
+        }
+        ncell = at;
+    } else {
+        if (ncell > kMaxCells)
+            return fail(WRACH_E_TOO_MANY_CELLS, "grid has more cells than the 32-bit offsets allow");
+        const uint32_t world_cap = max_particles_per_frame(world_cells, cfg->cell_size);
+        if (world_cap == 0) return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity does not fit in u32");
+        cap64 = (uint64_t)world_cap * n_worlds;
+    }
     // Bit 31 of a particle's source index marks the start of a run.
     if (cap64 > 0x7fffffffu)
         return fail(WRACH_E_TOO_MANY_CELLS, "particle capacity exceeds 2^31 - 1");
@@ -757,6 +797,25 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
     WRACH_HIP_C(hipMemsetAsync(b.class_ctr, 0, wrach::kCtrWords * sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.error, 0, sizeof(uint32_t), c->stream));
     WRACH_HIP_C(hipMemsetAsync(b.canon_ctr, 0, sizeof(uint32_t), c->stream));
+    if (world_cfgs) {
+        std::vector<uint32_t> chunk_world(c->chunks);
+        for (uint32_t w = 0; w < n_worlds; ++w) {
+            const uint64_t first = grids[w].base / wrach::kChunkCells;
+            const uint64_t last = w + 1 < n_worlds ? grids[w + 1].base / wrach::kChunkCells : c->chunks;
+            for (uint64_t ch = first; ch < last; ++ch) chunk_world[ch] = w;
+        }
+        WRACH_HIP_C(dmalloc(&c->dgrids, n_worlds));
+        WRACH_HIP_C(dmalloc(&c->dchunk_world, c->chunks));
+        WRACH_HIP_C(hipMemcpyAsync(c->dgrids, grids.data(), n_worlds * sizeof(wrach::WorldGrid),
+                                   hipMemcpyHostToDevice, c->stream));
+        WRACH_HIP_C(hipMemcpyAsync(c->dchunk_world, chunk_world.data(), c->chunks * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, c->stream));
+        c->grids = grids;
+        c->world_cfg.assign(world_cfgs, world_cfgs + n_worlds);
+        fp.grids = c->dgrids;
+        fp.chunk_world = c->dchunk_world;
+        fp.n_grids = n_worlds;
+    }
     WRACH_HIP_C(hipStreamSynchronize(c->stream));
 #undef WRACH_HIP_C
     *out = c;
@@ -766,10 +825,27 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
 
 }  // namespace
 
-int wrach_create(const wrach_config *cfg, wrach_ctx **out) { return create_ctx(cfg, 1, out); }
+int wrach_create(const wrach_config *cfg, wrach_ctx **out) { return create_ctx(cfg, 1, nullptr, out); }
 
 int wrach_create_batch(const wrach_config *cfg, uint32_t n_worlds, wrach_ctx **out) {
-    return create_ctx(cfg, n_worlds, out);
+    return create_ctx(cfg, n_worlds, nullptr, out);
+}
+
+int wrach_create_worlds(const wrach_config *cfgs, uint32_t n_worlds, wrach_ctx **out) {
+    if (!cfgs || !out) return fail(WRACH_E_INVALID, "wrach_create_worlds: null argument");
+    if (n_worlds == 0) return fail(WRACH_E_INVALID, "a batch holds at least one world");
+    bool same = true;
+    for (uint32_t w = 1; w < n_worlds; ++w) {
+        const wrach_config &a = cfgs[0], &b = cfgs[w];
+        if (b.cell_size != a.cell_size || b.boundaries_as_dimensions != a.boundaries_as_dimensions ||
+            b.gravity[0] != a.gravity[0] || b.gravity[1] != a.gravity[1] || b.dt != a.dt ||
+            b.flags != a.flags || b.device != a.device)
+            return fail(WRACH_E_INVALID, "world " + std::to_string(w) +
+                                             ": the worlds of a batch differ only in their dimensions");
+        same = same && b.width == a.width && b.height == a.height;
+    }
+    // Worlds of one size: the equal-world layout (no tables).
+    return create_ctx(&cfgs[0], n_worlds, same ? nullptr : cfgs, out);
 }
 
 void wrach_destroy(wrach_ctx *ctx) { release(ctx); }
@@ -1505,23 +1581,65 @@ int wrach_read(wrach_ctx *c, float *pos_xy, float *vel_xy, size_t particle_cap,
     return WRACH_OK;
 }
 
+namespace {
+
+// Cells of world w's grid, and its first cell key in the batch's key space.
+size_t world_cell_count(const wrach_ctx *c, uint32_t w) {
+    if (!c->grids.empty()) return (size_t)c->grids[w].gx * c->grids[w].gy;
+    return (size_t)c->world_cells;
+}
+
+size_t world_first_cell(const wrach_ctx *c, uint32_t w) {
+    if (!c->grids.empty()) return c->grids[w].base;
+    return (size_t)w * c->world_cells;
+}
+
+}  // namespace
+
+int wrach_settings_world(wrach_ctx *c, uint32_t world, wrach_world_settings *out) {
+    if (!c || !out) return fail(WRACH_E_INVALID, "null argument");
+    if (world >= c->worlds)
+        return fail(WRACH_E_INVALID, "no world " + std::to_string(world) + " in a batch of " +
+                                         std::to_string(c->worlds));
+    wrach_world_settings s = c->settings;
+    if (!c->grids.empty()) {
+        s.view_dimensions[0] = (float)c->world_cfg[world].width;
+        s.view_dimensions[1] = (float)c->world_cfg[world].height;
+        s.grid_dimensions[0] = c->grids[world].gx;
+        s.grid_dimensions[1] = c->grids[world].gy;
+    }
+    // The world's particles: the offsets at its first cell and past its last.
+    const size_t first = world_first_cell(c, world), wc = world_cell_count(c, world);
+    WRACH_HIP(hipSetDevice(c->cfg.device));
+    WRACH_HIP(wrach::launch_materialize(c->fp, c->buf, c->stream));
+    uint32_t ends[2] = {0, 0};
+    WRACH_HIP(hipMemcpyAsync(&ends[0], c->buf.idx + 1 + first, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    WRACH_HIP(hipMemcpyAsync(&ends[1], c->buf.idx + 1 + first + wc, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    WRACH_SETTLE(c);
+    s.particles_in_frame_count = ends[1] >= ends[0] ? ends[1] - ends[0] : 0u;
+    *out = s;
+    return WRACH_OK;
+}
+
 int wrach_read_world(wrach_ctx *c, uint32_t world, float *pos_xy, float *vel_xy, size_t particle_cap,
                      uint32_t *indices, size_t indices_cap, uint32_t *n_particles) {
     if (!c) return fail(WRACH_E_INVALID, "null context");
     if (world >= c->worlds)
         return fail(WRACH_E_INVALID, "no world " + std::to_string(world) + " in a batch of " +
                                          std::to_string(c->worlds));
-    const size_t wc = (size_t)c->world_cells;
+    const size_t wc = world_cell_count(c, world), first = world_first_cell(c, world);
     if (indices && indices_cap < wc + 2) return fail(WRACH_E_BUFFER, "indices array too small");
     wrach_device_view v{};
     {
         const int rc = wrach_canonical_device(c, &v);
         if (rc) return rc;
     }
-    // The world's offsets: entries 1 + world * wc .. 1 + (world + 1) * wc of
-    // the batch's, less the first (the world's first particle).
+    // The world's offsets: entries 1 + first .. 1 + first + wc of the
+    // batch's, less the first (the world's first particle).
     std::vector<uint32_t> g(wc + 1);
-    WRACH_HIP(hipMemcpyAsync(g.data(), v.indices + 1 + (size_t)world * wc, (wc + 1) * sizeof(uint32_t),
+    WRACH_HIP(hipMemcpyAsync(g.data(), v.indices + 1 + first, (wc + 1) * sizeof(uint32_t),
                              hipMemcpyDeviceToHost, c->stream));
     WRACH_SETTLE(c);
     if (g[0] > g[wc] || g[wc] > c->fp.n)

@@ -7,6 +7,14 @@
 
 namespace wrach {
 
+// One world of a batch of worlds of different sizes: its grid starts at
+// cell key `base` (a multiple of kChunkCells), its view at the origin.
+struct WorldGrid {
+    uint32_t base;    // first cell key
+    uint32_t gx, gy;  // grid dimensions
+    float x1, y1;     // the walls: the world's width and height
+};
+
 // Per-frame constants, passed by value to every kernel.  x1/y1 are
 // view_anchor + view_dimensions evaluated in f32, the viewport corner of
 // Particle::enforce_boundaries (shaders/physics/src/particle.rs:46-52).
@@ -26,6 +34,12 @@ struct FrameParams {
     // A batch of equal worlds: world w owns cell keys [w * world_cells,
     // (w + 1) * world_cells); one world: world_cells == ncell.
     uint32_t world_cells;
+    // A batch of worlds of different sizes: one WorldGrid per world and the
+    // world of every chunk of kChunkCells cells (null for one world or a
+    // batch of equal worlds; gx, gy, x1, y1 and world_cells then apply).
+    const WorldGrid *grids;
+    const uint32_t *chunk_world;
+    uint32_t n_grids;
     // The scan's look-back gives up after wait_limit rounds without finding
     // its prefix and raises kErrLookBack in FrameBuffers::error (the host
     // turns that into WRACH_E_DEVICE at its next synchronisation).
@@ -173,6 +187,7 @@ struct UploadParams {
     uint32_t cap;       // packed-array capacity
     const uint32_t *world;  // a batch: world of each store particle (else null)
     uint32_t world_cells;   // cells of one world (a world's keys start at world * world_cells)
+    const WorldGrid *grids;  // worlds of different sizes: each world's grid (else null)
     int32_t blx, bly;       // the grid's bottom-left cell: the view anchor / cell size
 };
 // Cell and slot of every store particle, counts into b.cnt, inside-grid

@@ -61,20 +61,28 @@ constexpr uint32_t kSrcMask = 0x7fffffffu;
 // ---------------------------------------------------------------------------
 // Per-particle arithmetic.
 
+// The grid of one world: its first cell key, dimensions and walls.  One
+// world or a batch of equal worlds: from FrameParams; worlds of different
+// sizes: from the world's WorldGrid (grid_of, below).
+struct Grid {
+    uint32_t base, gx, gy;
+    float x1, y1;
+};
+
 // Particle::integrate + enforce_boundaries + enforce_velocity
 // (shaders/physics/src/particle.rs:40-82) with the gravity/dt extension
 // placed just before the integration.  A zero gravity component adds -0.0
 // (FrameParams), which leaves -0.0 velocities their sign, as skipping the add
-// in the reference does.
-__device__ __forceinline__ void finish_particle(const FrameParams &p, float &px, float &py,
-                                                float &vx, float &vy) {
+// in the reference does.  The walls are the particle's world's.
+__device__ __forceinline__ void finish_particle(const FrameParams &p, const Grid &g, float &px,
+                                                float &py, float &vx, float &vy) {
     vx = vx + p.gdx;
     vy = vy + p.gdy;
     px = px + vx * p.dt;
     py = py + vy * p.dt;
-    if (px > p.x1) { px = p.x1; vx = vx * -1.0f; }
+    if (px > g.x1) { px = g.x1; vx = vx * -1.0f; }
     if (px < p.ax) { px = p.ax; vx = vx * -1.0f; }
-    if (py > p.y1) { py = p.y1; vy = vy * -1.0f; }
+    if (py > g.y1) { py = g.y1; vy = vy * -1.0f; }
     if (py < p.ay) { py = p.ay; vy = vy * -1.0f; }
     if (vx < -1.0f) vx = -1.0f;
     if (vx > 1.0f) vx = 1.0f;
@@ -82,25 +90,6 @@ __device__ __forceinline__ void finish_particle(const FrameParams &p, float &px,
     if (vy > 1.0f) vy = 1.0f;
 }
 
-// Particles::pairs body for one pair (particles.rs:65-92): glam distance,
-// skip beyond MIN_DISTANCE = 1, 0 -> 0.0001, force (0.5*(1-d))/d, equal and
-// opposite displacement.
-__device__ __forceinline__ void collide_pair(float &xi, float &yi, float &xj, float &yj) {
-    const float dx = xi - xj;
-    const float dy = yi - yj;
-    float sq = dx * dx;
-    sq = sq + dy * dy;
-    float d = sqrtf(sq);
-    if (d > 1.0f) return;
-    if (d == 0.0f) d = 0.0001f;
-    const float f = (0.5f * (1.0f - d)) / d;
-    const float ex = (xj - xi) * f;
-    const float ey = (yj - yi) * f;
-    xi = xi - ex;
-    yi = yi - ey;
-    xj = xj + ex;
-    yj = yj + ey;
-}
 
 // Cell coordinates of particles_per_cell.wgsl:14-27: floor((x - ax) / cs)
 // per axis, for a finished particle.  The walls leave x in [ax, x1] with
@@ -129,19 +118,40 @@ __device__ __forceinline__ uint32_t world_base(const FrameParams &p, uint32_t c)
     return c / p.world_cells * p.world_cells;
 }
 
-// Grid coordinates of cell key c inside its world, and the world's base.
-__device__ __forceinline__ void cell_coords(const FrameParams &p, uint32_t c, uint32_t &base,
-                                            uint32_t &cx, uint32_t &cy) {
-    base = world_base(p, c);
-    const uint32_t l = c - base;
-    cy = l / p.gx;
-    cx = l - cy * p.gx;
+// The grid of the world that holds cell key c (c < ncell).  Worlds of
+// different sizes start on chunk boundaries, so the chunk names the world.
+__device__ __forceinline__ Grid grid_of(const FrameParams &p, uint32_t c) {
+    Grid g;
+    if (p.grids) {
+        const uint32_t last = (p.ncell - 1) >> kChunkShift;
+        const uint32_t ch = (c >> kChunkShift) < last ? (c >> kChunkShift) : last;
+        const WorldGrid w = p.grids[p.chunk_world[ch]];
+        g.base = w.base;
+        g.gx = w.gx;
+        g.gy = w.gy;
+        g.x1 = w.x1;
+        g.y1 = w.y1;
+    } else {
+        g.base = world_base(p, c);
+        g.gx = p.gx;
+        g.gy = p.gy;
+        g.x1 = p.x1;
+        g.y1 = p.y1;
+    }
+    return g;
 }
 
-__device__ __forceinline__ uint32_t cell_key(const FrameParams &p, float x, float y, uint32_t base) {
+// Grid coordinates of cell key c inside its world g.
+__device__ __forceinline__ void cell_coords(const Grid &g, uint32_t c, uint32_t &cx, uint32_t &cy) {
+    const uint32_t l = c - g.base;
+    cy = l / g.gx;
+    cx = l - cy * g.gx;
+}
+
+__device__ __forceinline__ uint32_t cell_key(const FrameParams &p, const Grid &g, float x, float y) {
     uint32_t kx, ky;
     cell_xy(p, x, y, kx, ky);
-    return base + ky * p.gx + kx;
+    return g.base + ky * g.gx + kx;
 }
 
 // Claims `count` consecutive slots in cell k.  The frame's first claim in a
@@ -163,24 +173,15 @@ __device__ __forceinline__ uint32_t cell_start(const FrameParams &p, const Frame
     return s < p.n ? s : p.n;
 }
 
-// Writes one finished particle at canonical index `at` and claims a slot of
-// its own (a run of one) in the new cell.
-__device__ __forceinline__ void emit(const FrameParams &p, const FrameBuffers &b, uint32_t at,
-                                     float px, float py, float vx, float vy, uint32_t base) {
-    b.pos_out[at] = make_float2(px, py);
-    b.vel_out[at] = make_float2(vx, vy);
-    const uint32_t k = cell_key(p, px, py, base);
-    const uint32_t slot = claim(b, k, 1u);
-    b.kslot[at] = make_uint2(k, slot | kRunFlag);
-}
 
-// Neighbour direction of a new cell relative to the old one: 0..8 for the
-// 3x3 neighbourhood (row-major from (-1,-1)), 9 for anything farther.
-__device__ __forceinline__ uint32_t direction(const FrameParams &p, float x, float y, uint32_t cx,
-                                              uint32_t cy, uint32_t base, uint32_t &key) {
+// Neighbour direction of a new cell relative to the old one (cx, cy) of
+// world g: 0..8 for the 3x3 neighbourhood (row-major from (-1,-1)), 9 for
+// anything farther.
+__device__ __forceinline__ uint32_t direction(const FrameParams &p, const Grid &g, float x, float y,
+                                              uint32_t cx, uint32_t cy, uint32_t &key) {
     uint32_t kx, ky;
     cell_xy(p, x, y, kx, ky);
-    key = base + ky * p.gx + kx;
+    key = g.base + ky * g.gx + kx;
     const int dx = (int)kx - (int)cx, dy = (int)ky - (int)cy;
     if (dx < -1 || dx > 1 || dy < -1 || dy > 1) return 9;
     return (uint32_t)((dy + 1) * 3 + (dx + 1));
@@ -279,29 +280,17 @@ __device__ __forceinline__ void sort_by_src(uint32_t (&key)[N], float (&px)[N], 
 }
 
 // ---------------------------------------------------------------------------
-// Cells handled by one lane.
+// Cells handled by one lane, in the readback's canonical pass: copy a cell
+// of 1 or 2..NB particles to pos_out/vel_out in canonical order (ascending
+// source index).
 
-template <bool STEP>
-__device__ __forceinline__ void one_particle(const FrameParams &p, const FrameBuffers &b,
-                                             uint32_t c, uint32_t s) {
-    const float2 q = b.pos[s];
-    const float2 v = b.vel[s];
-    if (STEP) {
-        float px = q.x, py = q.y, vx = v.x, vy = v.y;
-        finish_particle(p, px, py, vx, vy);
-        emit(p, b, s, px, py, vx, vy, world_base(p, c));
-    } else {
-        b.pos_out[s] = q;
-        b.vel_out[s] = v;
-    }
+__device__ __forceinline__ void one_particle(const FrameBuffers &b, uint32_t s) {
+    b.pos_out[s] = b.pos[s];
+    b.vel_out[s] = b.vel[s];
 }
 
-// 2..NB particles: load, sort by source index, collide the first min(n,9)
-// in index order (particles.rs:62-81), finish all (overflow ones without
-// collisions, cell.rs:79-95).
-template <int NB, bool STEP>
-__device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuffers &b,
-                                           uint32_t c, uint32_t s, uint32_t n) {
+template <int NB>
+__device__ __forceinline__ void small_cell(const FrameBuffers &b, uint32_t s, uint32_t n) {
     uint32_t key[NB];
     float px[NB], py[NB], vx[NB], vy[NB];
 #pragma unroll
@@ -317,67 +306,11 @@ __device__ __forceinline__ void small_cell(const FrameParams &p, const FrameBuff
         }
     }
     sort_by_src<NB>(key, px, py, vx, vy);
-    if (STEP) {
-        constexpr int M = NB < kMaxInCell ? NB : kMaxInCell;
-        const int m = (int)(n < (uint32_t)kMaxInCell ? n : (uint32_t)kMaxInCell);
 #pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int j = i + 1; j < M; ++j)
-                if (j < m) collide_pair(px[i], py[i], px[j], py[j]);
-        // Finish every particle; direction of its new cell from this one.
-        uint32_t base, cx, cy;
-        cell_coords(p, c, base, cx, cy);
-        uint32_t dir[NB], nkey[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            dir[j] = 10;
-            nkey[j] = 0;
-            if (j < (int)n) {
-                finish_particle(p, px[j], py[j], vx[j], vy[j]);
-                dir[j] = direction(p, px[j], py[j], cx, cy, base, nkey[j]);
-            }
-        }
-        // One run per occupied direction: count, then all atomics at once.
-        uint32_t count[9], first[9], far[NB];
-#pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            count[d] = 0;
-#pragma unroll
-            for (int j = 0; j < NB; ++j) count[d] += dir[j] == (uint32_t)d;
-        }
-#pragma unroll
-        for (int d = 0; d < 9; ++d) {
-            first[d] = 0;
-            if (count[d]) {
-                const uint32_t tx = cx + d % 3 - 1, ty = cy + d / 3 - 1;
-                first[d] = claim(b, base + ty * p.gx + tx, count[d]);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < NB; ++j) far[j] = dir[j] == 9 ? claim(b, nkey[j], 1u) : 0u;
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (j < (int)n) {
-                uint32_t rank = 0, base = 0;
-#pragma unroll
-                for (int i = 0; i < j; ++i) rank += dir[i] == dir[j];
-#pragma unroll
-                for (int d = 0; d < 9; ++d) base = dir[j] == (uint32_t)d ? first[d] : base;
-                const uint32_t slot = dir[j] == 9 ? (far[j] | kRunFlag)
-                                                  : ((base + rank) | (rank == 0 ? kRunFlag : 0u));
-                b.pos_out[s + j] = make_float2(px[j], py[j]);
-                b.vel_out[s + j] = make_float2(vx[j], vy[j]);
-                b.kslot[s + j] = make_uint2(nkey[j], slot);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (j < (int)n) {
-                b.pos_out[s + j] = make_float2(px[j], py[j]);
-                b.vel_out[s + j] = make_float2(vx[j], vy[j]);
-            }
+    for (int j = 0; j < NB; ++j) {
+        if (j < (int)n) {
+            b.pos_out[s + j] = make_float2(px[j], py[j]);
+            b.vel_out[s + j] = make_float2(vx[j], vy[j]);
         }
     }
 }
@@ -497,8 +430,9 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
         __shared__ uint32_t s_wcnt[kWavesPerBlock][9], s_run[9], s_base[9];
         const uint32_t lane = tid & 63, wave = tid >> 6;
         const unsigned long long lt = lanemask_lt();
-        uint32_t wbase, cx, cy;
-        cell_coords(p, c, wbase, cx, cy);
+        const Grid g = grid_of(p, c);
+        uint32_t cx, cy;
+        cell_coords(g, c, cx, cy);
         uint2 *tmp = reinterpret_cast<const void *>(perm) == reinterpret_cast<const void *>(b.kslot + s)
                          ? reinterpret_cast<uint2 *>(b.sortbuf + s)
                          : b.kslot + s;
@@ -514,10 +448,10 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
             if (base == 0 && wave == 0) collide_group<64>(q.x, q.y, lane, kMaxInCell, 0);
             uint32_t dir = 10, key = 0;
             if (r < n) {
-                finish_particle(p, q.x, q.y, v.x, v.y);
+                finish_particle(p, g, q.x, q.y, v.x, v.y);
                 b.pos_out[s + r] = q;
                 b.vel_out[s + r] = v;
-                dir = direction(p, q.x, q.y, cx, cy, wbase, key);
+                dir = direction(p, g, q.x, q.y, cx, cy, key);
             }
             uint32_t rin = 0;
 #pragma unroll
@@ -546,7 +480,7 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
         // One atomic per occupied direction, then the slots.
         if (tid < 9) {
             const uint32_t tx = cx + tid % 3 - 1, ty = cy + tid / 3 - 1;
-            s_base[tid] = s_run[tid] ? claim(b, wbase + ty * p.gx + tx, s_run[tid]) : 0u;
+            s_base[tid] = s_run[tid] ? claim(b, g.base + ty * g.gx + tx, s_run[tid]) : 0u;
         }
         __syncthreads();
         for (uint32_t r = tid; r < n; r += kCollideBlock) {
@@ -554,7 +488,7 @@ __device__ __forceinline__ void big_cell(const FrameParams &p, const FrameBuffer
             if (t.x & kRunFlag) {
                 b.kslot[s + r] = make_uint2(t.x & kSrcMask, t.y);
             } else {
-                const uint32_t key = wbase + (cy + t.x / 3 - 1) * p.gx + (cx + t.x % 3 - 1);
+                const uint32_t key = g.base + (cy + t.x / 3 - 1) * g.gx + (cx + t.x % 3 - 1);
                 b.kslot[s + r] = make_uint2(key, (s_base[t.x] + t.y) | (t.y == 0 ? kRunFlag : 0u));
             }
         }
@@ -607,7 +541,10 @@ struct WaveLds {
 // on the same operands; the lower rank applies -dv, the higher +dv.  All
 // lanes execute the shuffles.
 // Step t of that schedule for the lane holding rank ri of a cell whose
-// first mi ranks collide, ranks held by lanes gbase + r.
+// first mi ranks collide, ranks held by lanes gbase + r.  The pair rule is
+// Particles::pairs (particles.rs:65-92): glam distance, skip beyond
+// MIN_DISTANCE = 1, 0 -> 0.0001, force (0.5*(1-d))/d, equal and opposite
+// displacement.
 __device__ __forceinline__ void separate_step(float &x, float &y, int ri, int mi, int gbase, int t) {
     const int lane = (int)lane_id();
     const int partner = t - ri;
@@ -836,10 +773,11 @@ __global__ __launch_bounds__(kCollideBlock) void collide_kernel(FrameParams p, F
         uint32_t dir = 10;
         nkey[j] = 0;
         if (mine[j]) {
-            uint32_t wbase, cx, cy;
-            cell_coords(p, k[j], wbase, cx, cy);
-            finish_particle(p, px[j], py[j], vx[j], vy[j]);
-            dir = direction(p, px[j], py[j], cx, cy, wbase, nkey[j]);
+            const Grid g = grid_of(p, k[j]);
+            uint32_t cx, cy;
+            cell_coords(g, k[j], cx, cy);
+            finish_particle(p, g, px[j], py[j], vx[j], vy[j]);
+            dir = direction(p, g, px[j], py[j], cx, cy, nkey[j]);
         }
         // Lanes of my cell going the same way.
         unsigned long long same = 0;
@@ -969,12 +907,13 @@ __device__ __forceinline__ void ranked_group(const FrameParams &p, const FrameBu
     collide_group<G>(px, py, j, n < (uint32_t)kMaxInCell ? n : (uint32_t)kMaxInCell, gbase);
 
     const bool has = j < n;
-    uint32_t wbase, cx, cy;
-    cell_coords(p, c, wbase, cx, cy);
+    const Grid g = grid_of(p, c);
+    uint32_t cx, cy;
+    cell_coords(g, c, cx, cy);
     uint32_t dir = 10, nkey = 0;
     if (has) {
-        finish_particle(p, px, py, vx, vy);
-        dir = direction(p, px, py, cx, cy, wbase, nkey);
+        finish_particle(p, g, px, py, vx, vy);
+        dir = direction(p, g, px, py, cx, cy, nkey);
         b.pos_out[s + j] = make_float2(px, py);
         b.vel_out[s + j] = make_float2(vx, vy);
     }
@@ -1141,8 +1080,9 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
                                           const uint32_t (&rk)[kWideChunks]) {
     const uint32_t lane = lane_id();
     const unsigned long long lt = lanemask_lt();
-    uint32_t wbase, cx, cy;
-    cell_coords(p, c, wbase, cx, cy);
+    const Grid g = grid_of(p, c);
+    uint32_t cx, cy;
+    cell_coords(g, c, cx, cy);
     uint32_t *code = L.code;
     uint32_t fkey[kWideChunks];  // a farther mover's new key, or ~0
 #pragma unroll
@@ -1151,11 +1091,11 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
         // rk is ~0 past the particles: one compare covers both bounds.
         if (rk[q] - (uint32_t)kMaxInCell < n - (uint32_t)kMaxInCell) {
             float px = pp[q].x, py = pp[q].y, vx = vv[q].x, vy = vv[q].y;
-            finish_particle(p, px, py, vx, vy);
+            finish_particle(p, g, px, py, vx, vy);
             b.pos_out[s + rk[q]] = make_float2(px, py);
             b.vel_out[s + rk[q]] = make_float2(vx, vy);
             uint32_t key;
-            const uint32_t dir = direction(p, px, py, cx, cy, wbase, key);
+            const uint32_t dir = direction(p, g, px, py, cx, cy, key);
             code[rk[q]] = dir < 9 ? (kRunFlag | (dir << 27)) : 0u;
             if (dir == 9) fkey[q] = key;
         }
@@ -1189,7 +1129,7 @@ __device__ __forceinline__ void wide_rest(const FrameParams &p, const FrameBuffe
     uint32_t got = 0, tkey = 0;
     if (lane < 9) {
         const uint32_t tx = cx + lane % 3 - 1, ty = cy + lane / 3 - 1;
-        tkey = wbase + ty * p.gx + tx;
+        tkey = g.base + ty * g.gx + tx;
         if (mine > 0) got = claim(b, tkey, mine);
     }
     uint32_t fslot[kWideChunks];
@@ -1457,15 +1397,15 @@ __global__ __launch_bounds__(kCollideBlock) void canonical_kernel(FrameParams p,
     uint32_t n = b.idx[c + 2] - s;
     clamp_range(p, s, n);
     if (n == 1) {
-        one_particle<false>(p, b, c, s);
+        one_particle(b, s);
     } else if (n == 2) {
-        small_cell<2, false>(p, b, c, s, n);
+        small_cell<2>(b, s, n);
     } else if (n > 2 && n <= 4) {
-        small_cell<4, false>(p, b, c, s, n);
+        small_cell<4>(b, s, n);
     } else if (n > 4 && n <= 8) {
-        small_cell<8, false>(p, b, c, s, n);
+        small_cell<8>(b, s, n);
     } else if (n > 8 && n <= (uint32_t)kRegCell) {
-        small_cell<kRegCell, false>(p, b, c, s, n);
+        small_cell<kRegCell>(b, s, n);
     } else if (n > (uint32_t)kRegCell) {
         const uint32_t q = atomicAdd(b.canon_ctr, 1u);
         b.cells[q] = make_uint4(s, n, c, 0u);
@@ -1784,14 +1724,15 @@ __device__ __forceinline__ float div_euclid_pos(float a, float b) {
     return fmodf(a, b) < 0.0f ? q - 1.0f : q;
 }
 
-// The packer's cell of a store particle (spatial_bin.rs:48-64); false
-// outside the grid.
-__device__ __forceinline__ bool store_cell(const UploadParams &u, float2 q, uint32_t &key) {
+// The packer's cell of a store particle (spatial_bin.rs:48-64) in a grid of
+// gx x gy cells; false outside it.
+__device__ __forceinline__ bool store_cell(const UploadParams &u, float2 q, uint32_t gx, uint32_t gy,
+                                           uint32_t &key) {
     // The cell relative to the grid's bottom-left cell (the view anchor's).
     const int64_t cx = (int64_t)saturating_i32(div_euclid_pos(q.x, u.cs)) - u.blx;
     const int64_t cy = (int64_t)saturating_i32(div_euclid_pos(q.y, u.cs)) - u.bly;
-    const bool in = cx >= 0 && cy >= 0 && cx < (int64_t)u.gx && cy < (int64_t)u.gy;
-    key = in ? (uint32_t)cy * u.gx + (uint32_t)cx : 0xffffffffu;
+    const bool in = cx >= 0 && cy >= 0 && cx < (int64_t)gx && cy < (int64_t)gy;
+    key = in ? (uint32_t)cy * gx + (uint32_t)cx : 0xffffffffu;
     return in;
 }
 
@@ -1800,9 +1741,22 @@ __global__ __launch_bounds__(256) void upload_bin_kernel(UploadParams u, FrameBu
     bool in = false;
     if (i < u.n) {
         uint2 ks = make_uint2(0xffffffffu, 0u);
-        in = store_cell(u, u.pos[i], ks.x);
+        // Its world's grid in a batch: worlds of one size one after another,
+        // or each world's own grid.
+        uint32_t gx = u.gx, gy = u.gy, base = 0;
+        if (u.world) {
+            if (u.grids) {
+                const WorldGrid g = u.grids[u.world[i]];
+                gx = g.gx;
+                gy = g.gy;
+                base = g.base;
+            } else {
+                base = u.world[i] * u.world_cells;
+            }
+        }
+        in = store_cell(u, u.pos[i], gx, gy, ks.x);
         if (in) {
-            if (u.world) ks.x += u.world[i] * u.world_cells;  // its world's grid in a batch
+            ks.x += base;
             ks.y = claim(b, ks.x, 1u);
         }
         u.ks[i] = ks;
@@ -1823,7 +1777,7 @@ __global__ __launch_bounds__(256) void store_outside_count_kernel(UploadParams u
     bool out = false;
     if (i < u.n) {
         uint32_t key;
-        out = !store_cell(u, u.pos[i], key);
+        out = !store_cell(u, u.pos[i], u.gx, u.gy, key);
     }
     const unsigned long long m = __ballot(out);
     if (lane_id() == 0) s_n[threadIdx.x >> 6] = (uint32_t)__popcll(m);
@@ -1841,7 +1795,7 @@ __global__ __launch_bounds__(256) void store_outside_kernel(UploadParams u, floa
     if (i < u.n) {
         q = u.pos[i];
         uint32_t key;
-        out = !store_cell(u, q, key);
+        out = !store_cell(u, q, u.gx, u.gy, key);
     }
     const unsigned long long m = __ballot(out);
     const uint32_t wave = threadIdx.x >> 6;
@@ -1892,9 +1846,10 @@ constexpr float kStencilUnit = 9.313225746154785e-10f;    // 2^-30
 // row's last column, start(x1 + 1) is the next row's first cell, which
 // begins where the row ends).
 __device__ __forceinline__ void stencil_rows(const FrameParams &p, const FrameBuffers &b,
-                                             uint32_t base, int cx, int cy, uint32_t rs[3],
+                                             const Grid &g, int cx, int cy, uint32_t rs[3],
                                              uint32_t re[3]) {
-    const int gx = (int)p.gx, gy = (int)p.gy;
+    const int gx = (int)g.gx, gy = (int)g.gy;
+    const uint32_t base = g.base;
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gx ? cx + 1 : gx - 1;
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
@@ -1907,19 +1862,27 @@ __device__ __forceinline__ void stencil_rows(const FrameParams &p, const FrameBu
     }
 }
 
-// First cell of the world that holds packed particle i: the last world whose
+// The grid of the world that holds packed particle i: the last world whose
 // first cell starts at or before i (an empty world starts where the next
-// one does, so it is never the last such).  One world: 0.
-__device__ __forceinline__ uint32_t particle_world_base(const FrameParams &p, const FrameBuffers &b,
-                                                        uint32_t i) {
-    if (p.world_cells == p.ncell) return 0u;
-    uint32_t lo = 0, hi = p.ncell / p.world_cells;  // world lo starts at or before i
+// one does, so it is never the last such).  One world: its own grid.
+__device__ __forceinline__ Grid particle_grid(const FrameParams &p, const FrameBuffers &b, uint32_t i) {
+    if (p.grids) {
+        uint32_t lo = 0, hi = p.n_grids;  // world lo starts at or before i
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (cell_start(p, b, p.grids[mid].base) <= i) lo = mid;
+            else hi = mid;
+        }
+        return grid_of(p, p.grids[lo].base);
+    }
+    if (p.world_cells == p.ncell) return grid_of(p, 0u);
+    uint32_t lo = 0, hi = p.ncell / p.world_cells;
     while (hi - lo > 1) {
         const uint32_t mid = lo + (hi - lo) / 2;
         if (cell_start(p, b, mid * p.world_cells) <= i) lo = mid;
         else hi = mid;
     }
-    return lo * p.world_cells;
+    return grid_of(p, lo * p.world_cells);
 }
 
 // collide_pair's rule for particle i alone: whether j is within reach, the
@@ -1958,8 +1921,10 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
                                                   long long &ay, uint32_t &pairs,
                                                   unsigned long long &osum, bool sums) {
     uint32_t rs[3], re[3];
-    const uint32_t base = world_base(p, k), local = k - base;
-    stencil_rows(p, b, base, (int)(local % p.gx), (int)(local / p.gx), rs, re);
+    const Grid g = grid_of(p, k);
+    uint32_t cx, cy;
+    cell_coords(g, k, cx, cy);
+    stencil_rows(p, b, g, (int)cx, (int)cy, rs, re);
     const uint64_t total = (uint64_t)(re[0] - rs[0]) + (re[1] - rs[1]) + (re[2] - rs[2]);
     float worst = 0.0f;
     if (total > kStencilBudget) return worst;
@@ -2031,7 +1996,7 @@ __device__ __forceinline__ void block_report(uint32_t *w, float v, unsigned long
 __device__ __forceinline__ uint32_t stencil_cell(const FrameParams &p, const FrameBuffers &b,
                                                  const StencilArgs &a, uint32_t i, float2 q) {
     if (!a.first) return a.cell[i];
-    const uint32_t k = cell_key(p, q.x, q.y, particle_world_base(p, b, i));
+    const uint32_t k = cell_key(p, particle_grid(p, b, i), q.x, q.y);
     a.cell[i] = k;
     return k;
 }
@@ -2208,12 +2173,14 @@ __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, Fram
     if (i < p.n) {
         const float2 q = a.pos[i];
         const float2 w = a.vel[i];
-        const uint32_t base = particle_world_base(p, b, i);
-        const uint32_t k = cell_key(p, q.x, q.y, base);
+        const Grid g = particle_grid(p, b, i);
+        const uint32_t base = g.base;
+        const uint32_t k = cell_key(p, g, q.x, q.y);
         if (a.cell) a.cell[i] = k;
-        const uint32_t local = k - base;
-        const int cx = (int)(local % p.gx), cy = (int)(local / p.gx);
-        const int gx = (int)p.gx, gy = (int)p.gy;
+        uint32_t ucx, ucy;
+        cell_coords(g, k, ucx, ucy);
+        const int cx = (int)ucx, cy = (int)ucy;
+        const int gx = (int)g.gx, gy = (int)g.gy;
         const int x0 = cx > 1 ? cx - 2 : 0, x1 = cx + 2 < gx ? cx + 2 : gx - 1;
         const int i0 = cx > 0 ? cx - 1 : 0, i1 = cx + 1 < gx ? cx + 1 : gx - 1;
         uint32_t rs[5], re[5], is[5], ie[5];
