@@ -51,6 +51,7 @@
 
 #include "frame.h"
 #include "learn.h"
+#include "select_nets.h"
 
 namespace wrach {
 namespace {
@@ -2219,21 +2220,29 @@ __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, Fram
                 }
             }
         }
-        // The 16 nearest: the first kSortCands candidates sorted in
-        // registers by a bitonic network (240 compare-exchanges for 32,
-        // where inserting each into a sorted list of 16 takes 15), then any
-        // further candidates inserted into the best 16.
+        // The 16 nearest.  The first kSortCands candidates are ordered in
+        // registers by Batcher's odd-even merge network for 32 keys, pruned
+        // to the 174 comparators that decide the first 17 places
+        // (select_nets.h), comparing the 64-bit key (squared distance bits,
+        // packed index) as one integer.  That is the exact order unless two
+        // of the first 17 share a distance: a wavefront where any lane finds
+        // such a tie orders its keys again with the exact comparator (a
+        // bitonic network of 240 compare-exchanges over cand_less).  Any
+        // further candidates are inserted into the best 16.  (Networks for
+        // 24 and 28 keys beside it, chosen per wavefront, took 137-165
+        // VGPRs: one wavefront less per SIMD.)
         unsigned long long key[kNbrs];
 #pragma unroll
         for (int r = 0; r < kNbrs; ++r) key[r] = ~0ull;
-        if (feats) {
-            uint32_t cum[5], m5 = 0;
+        uint32_t cum[5], m5 = 0;
 #pragma unroll
-            for (int t = 0; t < 5; ++t) {
-                cum[t] = m5;
-                m5 += re[t] - rs[t];
-            }
-            unsigned long long ck[kSortCands];
+        for (int t = 0; t < 5; ++t) {
+            cum[t] = m5;
+            m5 += re[t] - rs[t];
+        }
+        unsigned long long ck[kSortCands];
+        bool tie = false;
+        if (feats) {
 #pragma unroll
             for (int u = 0; u < kSortCands; ++u) {
                 ck[u] = ~0ull;
@@ -2253,20 +2262,39 @@ __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, Fram
                     }
                 }
             }
+#define WRACH_SEL_CE(x, y)                                 \
+    {                                                      \
+        const unsigned long long lo_ = ck[x], hi_ = ck[y]; \
+        const bool sw_ = hi_ < lo_;                        \
+        ck[x] = sw_ ? hi_ : lo_;                           \
+        ck[y] = sw_ ? lo_ : hi_;                           \
+    }
+            SEL_NET_32(WRACH_SEL_CE)
+#undef WRACH_SEL_CE
+            // Real keys never have the empty key's high word (distances are
+            // finite), so only real neighbours can tie.
 #pragma unroll
-            for (int kb = 1; kb <= kSortLog2; ++kb) {
+            for (int r = 0; r < kNbrs; ++r)
+                tie = tie || (ck[r + 1] != ~0ull && (uint32_t)(ck[r] >> 32) == (uint32_t)(ck[r + 1] >> 32));
+        }
+        const bool exact = __ballot(tie) != 0;  // every lane votes
+        if (feats) {
+            if (exact) {
 #pragma unroll
-                for (int jb = kb - 1; jb >= 0; --jb) {
-                    const int kk = 1 << kb, jj = 1 << jb;
+                for (int kb = 1; kb <= kSortLog2; ++kb) {
 #pragma unroll
-                    for (int u = 0; u < kSortCands; ++u) {
-                        const int l = u ^ jj;
-                        if (l > u) {
-                            const bool up = (u & kk) == 0;
-                            const unsigned long long x = ck[u], y = ck[l];
-                            const bool sw = up ? cand_less(a.pos, a.vel, y, x) : cand_less(a.pos, a.vel, x, y);
-                            ck[u] = sw ? y : x;
-                            ck[l] = sw ? x : y;
+                    for (int jb = kb - 1; jb >= 0; --jb) {
+                        const int kk = 1 << kb, jj = 1 << jb;
+#pragma unroll
+                        for (int u = 0; u < kSortCands; ++u) {
+                            const int l = u ^ jj;
+                            if (l > u) {
+                                const bool up = (u & kk) == 0;
+                                const unsigned long long x = ck[u], y = ck[l];
+                                const bool sw = up ? cand_less(a.pos, a.vel, y, x) : cand_less(a.pos, a.vel, x, y);
+                                ck[u] = sw ? y : x;
+                                ck[l] = sw ? x : y;
+                            }
                         }
                     }
                 }
