@@ -702,25 +702,27 @@ def test_worlds_of_different_sizes_argument_errors(oracle):
 def test_many_worlds_in_one_batch(oracle):
     """The 512-world workload (sides 256, 512 and 1024) as one batch of
     worlds of different sizes: one launch per frame kernel.  Two worlds of
-    each side bit for bit against the single-world oracle after 10 frames,
-    and the batch's particle total."""
+    each side bit for bit against the single-world oracle after 10 and 100
+    frames, and the batch's particle total."""
     ws = workloads.many_worlds()
     say(f"[mixed] many-worlds: {len(ws)} worlds in one batch, {sum(w.n for w in ws)} particles")
     with World([w.config for w in ws]) as b:
         for k, w in enumerate(ws):
             b.add_particles(w.pos, w.vel, world=k)
         assert b.n_particles == sum(w.n for w in ws)
-        b.step(10)
-        b.sync()
-        say("[mixed] many-worlds: 10 frames done")
         picks = []
         for side in (256, 512, 1024):
             ks = [k for k, w in enumerate(ws) if w.config.dimensions[0] == side]
             picks += [ks[0], ks[-1]]
-        for k in picks:
-            P, ref = oracle_world(oracle, ws[k].config, ws[k].pos, ws[k].vel)
-            assert_same_state(b.read_world(k), oracle.step(P, *ref, nsteps=10, nthreads=THREADS),
-                              f"world {k}, side {ws[k].config.dimensions[0]}")
+        refs = {k: oracle_world(oracle, ws[k].config, ws[k].pos, ws[k].vel) for k in picks}
+        for frames in (10, 90):
+            b.step(frames)
+            b.sync()
+            say(f"[mixed] many-worlds: +{frames} frames done")
+            for k in picks:
+                P, ref = refs[k]
+                refs[k] = (P, oracle.step(P, *ref, nsteps=frames, nthreads=THREADS))
+                assert_same_state(b.read_world(k), refs[k][1], f"world {k}, side {ws[k].config.dimensions[0]}")
         assert b.read()[0][-1] == sum(w.n for w in ws)
 
 
@@ -795,12 +797,13 @@ def test_lattice_1m_full_size(oracle):
 
 
 def test_lattice_1m_rollout_and_invariants(oracle):
-    """800 frames, past the collapse onto the floor (about 2,050 cells of
-    ~460 particles from frame ~500 on): still exact against the oracle at
-    frames 300 and 800, plus the size-independent invariants (particle count,
-    sorted offsets, every particle in its cell, inside the walls, |v| <= 1)."""
+    """The workload's 1,000 frames, past the collapse onto the floor (about
+    2,050 cells of ~460 particles from frame ~500 on): still exact against
+    the oracle at frames 300, 800 and 1,000, plus the size-independent
+    invariants at the end (particle count, sorted offsets, every particle in
+    its cell, inside the walls, |v| <= 1)."""
     wl = workloads.lattice_1m()
-    say(f"[rollout] {wl.name} 800 frames")
+    say(f"[rollout] {wl.name} 1000 frames")
     P, ref = oracle_world(oracle, wl.config, wl.pos, wl.vel)
     with World(wl.config) as w:
         w.add_particles(wl.pos, wl.vel)
@@ -808,9 +811,12 @@ def test_lattice_1m_rollout_and_invariants(oracle):
         ref = oracle.step(P, *ref, nsteps=300, nthreads=THREADS)
         assert_same_state(w.read(), ref, "lattice 300")
         w.step(500)
+        ref = oracle.step(P, *ref, nsteps=500, nthreads=THREADS)
+        assert_same_state(w.read(), ref, "lattice 800")
+        w.step(200)
         idx, pos, vel = w.read()
-    ref = oracle.step(P, *ref, nsteps=500, nthreads=THREADS)
-    assert_same_state((idx, pos, vel), ref, "lattice 800")
+    ref = oracle.step(P, *ref, nsteps=200, nthreads=THREADS)
+    assert_same_state((idx, pos, vel), ref, "lattice 1000")
     W, H = wl.config.dimensions
     gx = W + 1
     assert idx[0] == 0 and idx[1] == 0 and idx[-1] == wl.n
