@@ -1830,15 +1830,52 @@ __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, Fra
 // the reference step (an extension: the reference collides within one cell,
 // shaders/physics/src/cell.rs:60-69).  A pass is Jacobi -- every particle
 // reads the positions the pass started from -- and a particle's pair terms
-// are rounded to multiples of 2^-30 and summed in int64, exactly, so its
+// are rounded to multiples of 2^-30 and summed exactly, so its
 // displacement does not depend on the order of the particles inside a cell
 // (which the device layout keeps canonical only through `src`).
 
-// A term is at most 0.5 (plus rounding) in magnitude, so term * 2^30 is
-// below 2^31: rintf (round half to even, as llrintf) and a 32-bit
-// conversion give it exactly, in place of the long float-to-int64 sequence.
+// A term is below 0.62 in magnitude (|dx| / d is at most 1 + 2^-22, or
+// 1.22 where dx * dx is subnormal), so term * 2^30 is below 2^31 and
+// rintf (round half to even, as llrintf) makes it an integer-valued float.
+// The terms are summed as doubles: integers below 2^53 add exactly, and at
+// most kStencilBudget of them stay far below that, so the double holds the
+// same integer as an int64 sum, in two instructions a term in place of
+// five (conversion, sign extension, 64-bit add).
 constexpr float kStencilScale = 1073741824.0f;            // 2^30
 constexpr float kStencilUnit = 9.313225746154785e-10f;    // 2^-30
+
+// sqrtf(x) for x in [2^-96, 2]: v_sqrt_f32 (within one unit in the last
+// place), then its neighbour one unit below when x - below * s <= 0, or
+// one unit above when x - above * s > 0 (fused multiply-adds): the
+// correctly rounded root.  This is hipcc's own sqrtf sequence without its
+// two range steps, which are identities on these inputs: the scaling of
+// inputs below 2^-96 by 2^32 (and of their root by 2^-16), and the
+// pass-through of zero and infinity.
+__device__ __forceinline__ float sqrt_rn_above_2m96(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float below = __int_as_float(__float_as_int(s) - 1);
+    const float above = __int_as_float(__float_as_int(s) + 1);
+    float r = __builtin_fmaf(-below, s, x) <= 0.0f ? below : s;
+    return __builtin_fmaf(-above, s, x) > 0.0f ? above : r;
+}
+
+// a / b correctly rounded, for b in [2^-80, 1] and a = 0 or a in
+// [2^-26, 1]: hipcc's f32 division (the reciprocal, one Newton step on
+// it, the quotient and two residual corrections) without v_div_scale and
+// v_div_fixup.  For such operands v_div_scale scales nothing (it returns
+// NaN for a = 0, which v_div_fixup turns into +0, the quotient this
+// sequence gives), and v_div_fixup rewrites only special operands and sets
+// the sign the positive quotient already has.
+__device__ __forceinline__ float div_rn_bounded(float a, float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const float y = __builtin_fmaf(e, y0, y0);
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y, q1);
+}
 
 
 // The particles of the 3x3 cells around (cx, cy) as three ranges, one per
@@ -1886,68 +1923,66 @@ __device__ __forceinline__ Grid particle_grid(const FrameParams &p, const FrameB
     return grid_of(p, lo * p.world_cells);
 }
 
-// collide_pair's rule for particle i alone: whether j is within reach, the
-// overlap 1 - d, and (p_j - p_i) * force, which the reference subtracts
-// from p_i (particles.rs:65-92).
-__device__ __forceinline__ bool stencil_pair(float xi, float yi, float xj, float yj, float &ex,
-                                             float &ey, float &ov) {
-    const float dx = xi - xj;
-    const float dy = yi - yj;
+// collide_pair's rule for particle i alone, given dx = x_i - x_j and
+// dy = y_i - y_j: whether j is within reach, the overlap 1 - d (in [0, 1))
+// and f = (0.5 * (1 - d)) / d.  The reference subtracts (p_j - p_i) * f
+// from p_i (particles.rs:65-92), which is exactly (dx, dy) * f added:
+// p_j - p_i = -(p_i - p_j) in IEEE arithmetic, up to the sign of a zero,
+// which the rounding of the term to 2^-30 units drops.
+__device__ __forceinline__ bool stencil_reach(float dx, float dy, float &ov, float &f) {
     float sq = dx * dx;
     sq = sq + dy * dy;
     // sqrtf(sq) > 1 exactly when sq > 1 + 2^-23: sqrtf(1 + 2^-23) rounds to
-    // 1 and the root of the next float above it does not.  Most neighbours of
-    // a crowded cell are out of reach; they skip the root and the division
-    // with the same outcome.
+    // 1 and the root of the next float above it does not, so d <= 1 below.
+    // Most neighbours of a crowded cell are out of reach; they skip the root
+    // and the division with the same outcome.
     if (sq > 1.00000011920928955078125f) return false;
-    float d = sqrtf(sq);
-    if (d > 1.0f) return false;
+    float d = sq < 0x1p-96f ? sqrtf(sq) : sqrt_rn_above_2m96(sq);
     if (d == 0.0f) d = 0.0001f;
     ov = 1.0f - d;
-    const float f = (0.5f * (1.0f - d)) / d;
-    ex = (xj - xi) * f;
-    ey = (yj - yi) * f;
+    // 0.5 * ov is 0 or at least 2^-25; d is at least 2^-75 (the root of
+    // the least subnormal) and at most 1: div_rn_bounded's range.
+    f = div_rn_bounded(0.5f * ov, d);
     return true;
 }
 
 // Particle i (in cell (cx, cy) at the frame's start) against the 3x3 cells
 // around it: the largest overlap, the number of overlapping pairs, with
-// MOVE the sum of its pair terms in 2^-30 units, and with `sums` the sum of
+// MOVE the sum of its pair terms in 2^-30 units, and with SUMS the sum of
 // the overlaps in 2^-24 units.  Over kStencilBudget particles in those
 // cells: none of these (a bound on the work of one lane).
-template <bool MOVE>
+template <bool MOVE, bool SUMS>
 __device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
                                                   const float2 *pin, uint32_t i, uint32_t k,
-                                                  float xi, float yi, long long &ax,
-                                                  long long &ay, uint32_t &pairs,
-                                                  unsigned long long &osum, bool sums) {
+                                                  float xi, float yi, double &ax, double &ay,
+                                                  uint32_t &pairs, unsigned long long &osum) {
     uint32_t rs[3], re[3];
     const Grid g = grid_of(p, k);
     uint32_t cx, cy;
     cell_coords(g, k, cx, cy);
     stencil_rows(p, b, g, (int)cx, (int)cy, rs, re);
     const uint64_t total = (uint64_t)(re[0] - rs[0]) + (re[1] - rs[1]) + (re[2] - rs[2]);
-    float worst = 0.0f;
-    if (total > kStencilBudget) return worst;
+    if (total > kStencilBudget) return 0.0f;
+    uint32_t worst = 0;  // the largest overlap's bits (overlaps lie in [0, 1))
     for (int t = 0; t < 3; ++t) {
 #pragma unroll 4
         for (uint32_t j = rs[t]; j < re[t]; ++j) {
-            if (j == i) continue;
             const float2 q = pin[j];
-            float ex, ey, ov;
-            if (!stencil_pair(xi, yi, q.x, q.y, ex, ey, ov)) continue;
-            worst = fmaxf(worst, ov);
+            const float dx = xi - q.x, dy = yi - q.y;
+            float ov, f;
+            if (j == i || !stencil_reach(dx, dy, ov, f)) continue;
+            worst = max(worst, __float_as_uint(ov));
             pairs += 1;
             // ov * 2^24 < 2^24: rintf (round half to even, as llrintf) is
             // integral and converts exactly through 32 bits.
-            if (sums) osum += (uint32_t)rintf(ov * kOverlapScale);
+            if (SUMS) osum += (uint32_t)rintf(ov * kOverlapScale);
             if (MOVE) {
-                ax -= (int32_t)rintf(ex * kStencilScale);
-                ay -= (int32_t)rintf(ey * kStencilScale);
+                ax += (double)rintf((dx * f) * kStencilScale);
+                ay += (double)rintf((dy * f) * kStencilScale);
             }
         }
     }
-    return worst;
+    return __uint_as_float(worst);
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -2039,9 +2074,10 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
     if (i < p.n) {
         const float2 q = a.pin[i];
         const uint32_t k = stencil_cell(p, b, a, i, q);
-        long long ax = 0, ay = 0;
+        double ax = 0.0, ay = 0.0;
         if (k < p.ncell)
-            worst = stencil_particle<false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, a.stat != 0);
+            worst = a.stat ? stencil_particle<false, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum)
+                           : stencil_particle<false, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
     }
     block_report(&a.words[a.in_slot], worst, stencil_sums(a), osum, pairs);
 }
@@ -2060,7 +2096,7 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
         return;  // the whole grid: no barrier follows
     }
     float worst = 0.0f;
-    long long ax = 0, ay = 0;
+    double ax = 0.0, ay = 0.0;  // integers: the sums of the terms in 2^-30 units
     uint32_t pairs = 0;
     unsigned long long osum = 0;
     float2 q = make_float2(0.0f, 0.0f);
@@ -2073,14 +2109,15 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
         q = a.pin[i];
         const uint32_t k = stencil_cell(p, b, a, i, q);
         if (k < p.ncell)
-            worst = stencil_particle<true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, a.stat != 0);
+            worst = a.stat ? stencil_particle<true, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum)
+                           : stencil_particle<true, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
     }
     block_report(&a.words[a.in_slot], worst, stencil_sums(a), osum, pairs);
     if (i >= p.n) return;
     // The mean of the pair corrections (constraint averaging): a particle
     // moves at most 0.5 per pass.
-    float dx = __ll2float_rn(ax) * kStencilUnit;
-    float dy = __ll2float_rn(ay) * kStencilUnit;
+    float dx = (float)ax * kStencilUnit;  // the integer rounded once to f32
+    float dy = (float)ay * kStencilUnit;
     if (pairs > 1) {
         dx = dx / (float)pairs;
         dy = dy / (float)pairs;
@@ -2212,8 +2249,8 @@ __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, Fram
                 for (uint32_t j = is[t]; j < ie[t]; ++j) {
                     if (j == i) continue;
                     const float2 o = a.pos[j];
-                    float ex, ey, ov;
-                    if (stencil_pair(q.x, q.y, o.x, o.y, ex, ey, ov)) {
+                    float ov, f;
+                    if (stencil_reach(q.x - o.x, q.y - o.y, ov, f)) {
                         osum += (uint32_t)rintf(ov * kOverlapScale);
                         ocnt += 1;
                     }
