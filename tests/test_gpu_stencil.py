@@ -229,3 +229,106 @@ def test_stencil_quiet_skips_only_the_closing_measure(oracle, tol):
         with pytest.raises(WrachError) as e:
             _lib.check(w._L.wrach_set_solver(w._h, _lib.WRACH_SOLVER_R | _lib.WRACH_SOLVER_QUIET, 0, 0.0))
         assert e.value.code == _lib.WRACH_E_INVALID
+
+
+# Crowded cells: 64..1,024 particles with at most 4,096 in their 3x3 cells
+# (a lane's work stays within the budget), beside cells of every other kind.
+CROWD_MIN, CROWD_MAX = 64, 1024
+
+
+def crowded_cells_world(seed, side=112, origin=(0, 0)):
+    """Cells of exactly 63, 64, 1,024 and 1,025 particles; a 3x3 block of
+    crowded cells (crowded neighbours in all eight directions); crowded
+    cells beside a cell over the budget, at a corner and on an edge;
+    diagonal crowded neighbours; over a sparse background.  Cell size 1,
+    cells given relative to `origin`.  (Worlds hold about as many particles
+    as cells, so the world is larger than the cells it uses.)"""
+    rng = np.random.default_rng(seed)
+    designed = {(10 + i, 10 + j): 100 for i in range(3) for j in range(3)}
+    designed.update({(20, 10): 64, (21, 10): 63, (20, 11): 64, (21, 11): 64,
+                     (30, 10): 1024, (31, 10): 1025, (30, 11): 70,
+                     (40, 30): 200, (41, 30): 4000, (39, 30): 100, (39, 31): 80,
+                     (0, 0): 300, (1, 0): 65, (0, 1): 90,
+                     (side - 1, 20): 500, (side - 1, 21): 200, (side - 2, 19): 64,
+                     (25, 40): 128, (26, 41): 128, (24, 39): 128, (26, 39): 128, (24, 41): 128})
+    bg = rng.uniform(0, side, (2000, 2))
+    keep = np.array([tuple(c) not in designed for c in np.floor(bg).astype(int)])
+    parts = [bg[keep]]
+    for (cx, cy), k in designed.items():
+        parts.append(np.stack([cx + rng.uniform(0.001, 0.999, k), cy + rng.uniform(0.001, 0.999, k)], 1))
+    pos = f32(np.concatenate(parts) + np.asarray(origin, np.float64))
+    vel = f32(rng.normal(0, 0.05, pos.shape))
+    return pos, vel
+
+
+def crowded_cells(P, idx):
+    """The crowded cells of a packed layout (a check on the test's inputs,
+    computed from the oracle's offsets), and each cell's count."""
+    gx, gy = P.grid_dimensions[0], P.grid_dimensions[1]
+    n = P.particles_in_frame_count
+    st = np.minimum(np.asarray(idx[1:gx * gy + 2], np.int64), n)
+    cnt = (st[1:] - st[:-1]).reshape(gy, gx)
+    pad = np.pad(cnt, 1)
+    tot = sum(pad[1 + dy:1 + dy + gy, 1 + dx:1 + dx + gx] for dy in (-1, 0, 1) for dx in (-1, 0, 1))
+    return (cnt >= CROWD_MIN) & (cnt <= CROWD_MAX) & (tot <= 4096), cnt
+
+
+def test_stencil_crowded_cells(oracle):
+    """Crowded cells beside cells of every other kind, 6 frames of 2 passes
+    (graphs) and 4 frames of 3 passes with a tolerance (plain launches),
+    bit for bit with the statistics."""
+    pos, vel = crowded_cells_world(31)
+    cfg = WrachConfig(dimensions=(112, 112), cell_size=1)
+    P, (idx, _, _) = oracle_world(oracle, cfg, pos, vel)
+    crowd, cnt = crowded_cells(P, idx)
+    assert crowd[10:13, 10:13].all() and crowd[10, 20] and not crowd[10, 21] and cnt[10, 21] == 63
+    assert crowd[10, 30] and cnt[10, 30] == CROWD_MAX and not crowd[10, 31] and cnt[10, 31] == CROWD_MAX + 1
+    assert not crowd[30, 40] and crowd[30, 39] and crowd[0, 0] and crowd[20, 111]
+    run_stencil(oracle, cfg, pos, vel, 6, 2, every=1, label="crowded cells")
+    run_stencil(oracle, cfg, pos, vel, 4, 3, tol=0.2, every=2, label="crowded cells, tolerance", graph=False)
+
+
+def test_stencil_crowded_cells_in_a_batch(oracle):
+    """Two worlds of different sizes with crowded cells in one batch: each
+    cell's 3x3 cells clipped to its own world, each world bit for bit
+    against the oracle alone."""
+    ws = [(WrachConfig(dimensions=(112, 112), cell_size=1), *crowded_cells_world(32)),
+          (WrachConfig(dimensions=(128, 120), cell_size=1), *crowded_cells_world(33, side=120, origin=(4, 0)))]
+    say("[stencil] crowded cells in a batch of 2 worlds, 2 passes")
+    with World([c for c, _, _ in ws]) as b:
+        for w, (c, p, v) in enumerate(ws):
+            b.add_particles(p, v, world=w)
+        b.set_solver("S", 2)
+        refs = [oracle_world(oracle, c, p, v) for c, p, v in ws]
+        for frames in (1, 3):
+            b.step(frames)
+            for w, (P, ref) in enumerate(refs):
+                ref = oracle.step_s(P, *ref, 2, nsteps=frames)[:3]
+                refs[w] = (P, ref)
+                assert_same_state(b.read_world(w), ref, f"world {w}")
+
+
+def test_stencil_collapsed_lattice(oracle):
+    """A lattice 256 cells wide and 488 rows high collapsed under gravity
+    (1,000 frames of the reference step, about 380 particles in each floor
+    cell), then mode S frames from that state: the oracle's passes from the
+    same state, bit for bit."""
+    wl = workloads.lattice_1m(n=256 * 488, width=256, height=1024)
+    cfg = wl.config
+    say(f"[stencil] collapsed lattice {wl.name}")
+    with World(cfg) as w:
+        w.add_particles(wl.pos, wl.vel)
+        w.step(1000)
+        state = w.read()
+        W, H = cfg.dimensions
+        P = oracle.params(W, H, cfg.cell_size, len(state[1]), gravity=cfg.gravity, dt=cfg.dt)
+        crowd, cnt = crowded_cells(P, state[0])
+        say(f"[stencil] collapsed: {int(crowd.sum())} crowded cells holding {int(cnt[crowd].sum())} particles")
+        assert crowd.sum() >= 200
+        w.set_solver("S", 2)
+        ref = state
+        for frame in range(2):
+            w.step(1)
+            *ref, st = oracle.step_s(P, *ref, max_passes=2, nsteps=1)
+            assert_same_state(w.read(), tuple(ref), f"collapsed, mode S frame {frame + 1}")
+            assert w.solver_stats() == st

@@ -1859,13 +1859,14 @@ __device__ __forceinline__ float sqrt_rn_above_2m96(float x) {
     return __builtin_fmaf(-above, s, x) > 0.0f ? above : r;
 }
 
-// a / b correctly rounded, for b in [2^-80, 1] and a = 0 or a in
-// [2^-26, 1]: hipcc's f32 division (the reciprocal, one Newton step on
+// a / b correctly rounded, for b in [2^-48, 1] and a = 0 or a in
+// [2^-26, 2^30): hipcc's f32 division (the reciprocal, one Newton step on
 // it, the quotient and two residual corrections) without v_div_scale and
-// v_div_fixup.  For such operands v_div_scale scales nothing (it returns
-// NaN for a = 0, which v_div_fixup turns into +0, the quotient this
-// sequence gives), and v_div_fixup rewrites only special operands and sets
-// the sign the positive quotient already has.
+// v_div_fixup.  For such operands v_div_scale scales nothing (the exponent
+// difference stays below 96, neither operand nor the quotient is
+// subnormal; for a = 0 it returns NaN, which v_div_fixup turns into +0,
+// the quotient this sequence gives), and v_div_fixup rewrites only special
+// operands and sets the sign the positive quotient already has.
 __device__ __forceinline__ float div_rn_bounded(float a, float b) {
     const float y0 = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, y0, 1.0f);
@@ -1924,26 +1925,56 @@ __device__ __forceinline__ Grid particle_grid(const FrameParams &p, const FrameB
 }
 
 // collide_pair's rule for particle i alone, given dx = x_i - x_j and
-// dy = y_i - y_j: whether j is within reach, the overlap 1 - d (in [0, 1))
-// and f = (0.5 * (1 - d)) / d.  The reference subtracts (p_j - p_i) * f
-// from p_i (particles.rs:65-92), which is exactly (dx, dy) * f added:
-// p_j - p_i = -(p_i - p_j) in IEEE arithmetic, up to the sign of a zero,
-// which the rounding of the term to 2^-30 units drops.
-__device__ __forceinline__ bool stencil_reach(float dx, float dy, float &ov, float &f) {
+// dy = y_i - y_j: whether j is within reach, the overlap 1 - d (a multiple
+// of 2^-24 in [0, 1]) and F = f * 2^30 for f = (0.5 * (1 - d)) / d.  The
+// reference subtracts (p_j - p_i) * f from p_i (particles.rs:65-92), which
+// is exactly (dx, dy) * f added: p_j - p_i = -(p_i - p_j) in IEEE
+// arithmetic, up to the sign of a zero, which the rounding of the term to
+// 2^-30 units drops.  That term, rint((dx * f) * 2^30), is rint(dx * F):
+// f is 0 or at least 2^-25 (normal), so F is exactly f * 2^30, and
+// dx * f * 2^30 rounds as dx * f does, scaled, unless dx * f is subnormal,
+// where both round to 0.
+// `other`: j is not i (tested with the distance, so that the caller's load
+// of p_j need not wait for it).
+__device__ __forceinline__ bool stencil_reach(float dx, float dy, bool other, float &ov, float &F) {
     float sq = dx * dx;
     sq = sq + dy * dy;
     // sqrtf(sq) > 1 exactly when sq > 1 + 2^-23: sqrtf(1 + 2^-23) rounds to
     // 1 and the root of the next float above it does not, so d <= 1 below.
     // Most neighbours of a crowded cell are out of reach; they skip the root
     // and the division with the same outcome.
-    if (sq > 1.00000011920928955078125f) return false;
-    float d = sq < 0x1p-96f ? sqrtf(sq) : sqrt_rn_above_2m96(sq);
-    if (d == 0.0f) d = 0.0001f;
+    if (!other | (sq > 1.00000011920928955078125f)) return false;
+    if (sq < 0x1p-96f) {  // d below 2^-48, or 0: the full sequences
+        float d = sqrtf(sq);
+        if (d == 0.0f) d = 0.0001f;
+        ov = 1.0f - d;
+        F = ((0.5f * ov) / d) * kStencilScale;
+        return true;
+    }
+    const float d = sqrt_rn_above_2m96(sq);  // in [2^-48, 1]: never 0
     ov = 1.0f - d;
-    // 0.5 * ov is 0 or at least 2^-25; d is at least 2^-75 (the root of
-    // the least subnormal) and at most 1: div_rn_bounded's range.
-    f = div_rn_bounded(0.5f * ov, d);
+    // ov * 2^29 is 0 or in [2^5, 2^29]: div_rn_bounded's range, and the
+    // quotient is (0.5 * ov / d) * 2^30 exactly, as f is normal.
+    F = div_rn_bounded(ov * 0x1p29f, d);
     return true;
+}
+
+// One neighbour of particle i (`other`: not i itself) for stencil_particle.
+// ov * 2^24 is an integer: ov is a multiple of 2^-24 in [0, 1].
+template <bool MOVE, bool SUMS>
+__device__ __forceinline__ void stencil_visit(bool other, float xi, float yi, float2 q,
+                                              uint32_t &worst, uint32_t &pairs,
+                                              unsigned long long &osum, double &ax, double &ay) {
+    const float dx = xi - q.x, dy = yi - q.y;
+    float ov, F;
+    if (!stencil_reach(dx, dy, other, ov, F)) return;
+    worst = max(worst, __float_as_uint(ov));
+    pairs += 1;
+    if (SUMS) osum += (uint32_t)(ov * kOverlapScale);
+    if (MOVE) {
+        ax += (double)rintf(dx * F);
+        ay += (double)rintf(dy * F);
+    }
 }
 
 // Particle i (in cell (cx, cy) at the frame's start) against the 3x3 cells
@@ -1963,24 +1994,20 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
     stencil_rows(p, b, g, (int)cx, (int)cy, rs, re);
     const uint64_t total = (uint64_t)(re[0] - rs[0]) + (re[1] - rs[1]) + (re[2] - rs[2]);
     if (total > kStencilBudget) return 0.0f;
-    uint32_t worst = 0;  // the largest overlap's bits (overlaps lie in [0, 1))
+    uint32_t worst = 0;  // the largest overlap's bits (overlaps lie in [0, 1])
     for (int t = 0; t < 3; ++t) {
-#pragma unroll 4
-        for (uint32_t j = rs[t]; j < re[t]; ++j) {
-            const float2 q = pin[j];
-            const float dx = xi - q.x, dy = yi - q.y;
-            float ov, f;
-            if (j == i || !stencil_reach(dx, dy, ov, f)) continue;
-            worst = max(worst, __float_as_uint(ov));
-            pairs += 1;
-            // ov * 2^24 < 2^24: rintf (round half to even, as llrintf) is
-            // integral and converts exactly through 32 bits.
-            if (SUMS) osum += (uint32_t)rintf(ov * kOverlapScale);
-            if (MOVE) {
-                ax += (double)rintf((dx * f) * kStencilScale);
-                ay += (double)rintf((dy * f) * kStencilScale);
-            }
+        uint32_t j = rs[t];
+        const uint32_t e = re[t];
+        // Four neighbours a step, all loaded before the first is used.
+        for (; j + 4 <= e; j += 4) {
+            float2 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = pin[j + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                stencil_visit<MOVE, SUMS>(j + u != i, xi, yi, q[u], worst, pairs, osum, ax, ay);
         }
+        for (; j < e; ++j) stencil_visit<MOVE, SUMS>(j != i, xi, yi, pin[j], worst, pairs, osum, ax, ay);
     }
     return __uint_as_float(worst);
 }
@@ -2250,7 +2277,7 @@ __global__ __launch_bounds__(256) void learn_features_kernel(FrameParams p, Fram
                     if (j == i) continue;
                     const float2 o = a.pos[j];
                     float ov, f;
-                    if (stencil_reach(q.x - o.x, q.y - o.y, ov, f)) {
+                    if (stencil_reach(q.x - o.x, q.y - o.y, true, ov, f)) {
                         osum += (uint32_t)rintf(ov * kOverlapScale);
                         ocnt += 1;
                     }
