@@ -717,7 +717,7 @@ int create_ctx(const wrach_config *cfg, uint32_t n_worlds, const wrach_config *w
     fp.epoch_skew = 0;
 
     wrach::FrameBuffers &b = c->buf;
-    b.scan_shape = 0;
+    b.scan_shape = wrach::default_scan_shape(ncell);
     b.scatter_shape = 0;
     b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell, b.scan_shape);
     // Queued-cell regions: a class of cells holding at least m particles has
@@ -2230,9 +2230,9 @@ int wrach_frame_noalloc(const wrach_world_settings *ws, const float gravity[2], 
         if (rc) return rc;
     }
     carve_noalloc(base, n, ncell, b, ncells_total);
-    b.scan_shape = 0;
+    b.scan_shape = wrach::default_scan_shape(ncell);
     b.scatter_shape = 0;
-    b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell, 0);
+    b.scan_tiles = wrach::scan_tiles_for((uint32_t)ncell, b.scan_shape);
     {
         const uint32_t want = (b.class_cap[0] * 2 / 64 + wrach::kWavesPerBlock) / wrach::kWavesPerBlock;
         const uint32_t resident = wrach::resident_blocks_cells();

@@ -98,9 +98,9 @@ constexpr int kCollideBlock = 256;
 constexpr uint32_t kChunkShift = 10;  // a chunk: the cells of one collide block
 constexpr uint32_t kChunkCells = 1u << kChunkShift;
 // Launch shapes the autotuner (wrach_autotune) chooses among; entry 0 of
-// each table is the default.  Scan tiles: threads x counts per thread
-// (1024 x 8 is the fastest in the 1M-particle frame).  Scatter: particles
-// per lane of a 256-thread block.
+// each table is the default (the scan's: see default_scan_shape).  Scan
+// tiles: threads x counts per thread (1024 x 8 is the fastest in the
+// 1M-particle frame).  Scatter: particles per lane of a 256-thread block.
 struct ScanShape {
     int threads, items;
 };
@@ -108,6 +108,18 @@ struct ScanShape {
 // mostly empty (a batch of worlds), where the number of tiles sets the time.
 constexpr ScanShape kScanShapes[] = {{1024, 8}, {512, 16}, {1024, 4}, {512, 8}, {1024, 16}, {1024, 32}};
 constexpr int kNumScanShapes = 6;
+// The scan shape a context starts with: entry 0 up to kLargeScanGrid cells;
+// above, where the look-back over thousands of tiles dominates, 1024 x 32
+// (a quarter of the tiles).  Measured: the 1M lattice's 2.1M cells scan in
+// 7.6 us at 1024 x 8 and 18.3 us at 1024 x 32; the 4M blobs' 16.8M cells in
+// 66.5 and 57.9 us; the 512-world batch's 227M cells in 827 and 509 us.
+constexpr uint64_t kLargeScanGrid = 1ull << 23;
+constexpr uint32_t kLargeScanShape = 5;
+static_assert(kScanShapes[kLargeScanShape].threads == 1024 && kScanShapes[kLargeScanShape].items == 32,
+              "the large-grid scan shape");
+constexpr uint32_t default_scan_shape(uint64_t ncell) {
+    return ncell > kLargeScanGrid ? kLargeScanShape : 0u;
+}
 constexpr uint32_t kMinScanTile = 4096;  // smallest threads x items above (sizes tile_state)
 constexpr int kScatterBlock = 256;
 constexpr int kScatterPerLane[] = {2, 1, 4};
