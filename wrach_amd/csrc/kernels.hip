@@ -955,15 +955,19 @@ constexpr int kWideCells = kWavesPerBlock - 1;
 
 struct WideStage {
     float4 q[kWideCells][kMaxInCell];  // ranks 0..8 of each cell: x, y, stored position (bits)
+    float2 v[kWideCells][kMaxInCell];  // and their velocities
+    uint2 sc[kWideCells];              // each cell's first particle and key
     uint32_t on[kWideCells];           // nonzero: wave 3 handles ranks 0..8
 };
 
 // Cell wave, before the block barrier: loads in stored order, canonical rank
-// of every stored position (rk), ranks 0..8 staged for wave 3.  The ranks
+// of every stored position (rk), ranks 0..8 (position, stored position and
+// velocity) staged for wave 3.  The ranks
 // come from the runs; a cell of more than kWideRuns runs is ranked pairwise
 // (source indices in a cell are distinct).
 __device__ __forceinline__ void wide_order(const FrameBuffers &b, uint32_t s, uint32_t n,
-                                           WaveLds &L, float4 *stage, float2 (&pp)[kWideChunks],
+                                           WaveLds &L, float4 *stage, float2 *vstage,
+                                           float2 (&pp)[kWideChunks],
                                            float2 (&vv)[kWideChunks],
                                            uint32_t (&rk)[kWideChunks]) {
     const uint32_t lane = lane_id();
@@ -1065,8 +1069,10 @@ __device__ __forceinline__ void wide_order(const FrameBuffers &b, uint32_t s, ui
 #pragma unroll
     for (int q = 0; q < kWideChunks; ++q) {
         const uint32_t i = 64 * q + lane;
-        if (i < n && rk[q] < (uint32_t)kMaxInCell)
+        if (i < n && rk[q] < (uint32_t)kMaxInCell) {
             stage[rk[q]] = make_float4(pp[q].x, pp[q].y, __uint_as_float(i), 0.0f);
+            vstage[rk[q]] = vv[q];
+        }
     }
 }
 
@@ -1180,11 +1186,14 @@ __device__ __forceinline__ void wide_item(const FrameParams &p, const FrameBuffe
         if (mine) {
             cell = list[e];
             if (cell.y <= kWideMax) {  // always, by the class; guards a corrupt list
-                wide_order(b, cell.x, cell.y, lds[wave], st.q[wave], pp, vv, rk);
+                wide_order(b, cell.x, cell.y, lds[wave], st.q[wave], st.v[wave], pp, vv, rk);
                 fast = true;
             }
         }
-        if (lane == 0) st.on[wave] = fast ? 1u : 0u;
+        if (lane == 0) {
+            st.on[wave] = fast ? 1u : 0u;
+            st.sc[wave] = make_uint2(cell.x, cell.z);
+        }
     }
     __syncthreads();
     if (wave == (uint32_t)kWideCells) {
@@ -1195,13 +1204,15 @@ __device__ __forceinline__ void wide_item(const FrameParams &p, const FrameBuffe
         float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float2 v = make_float2(0.0f, 0.0f);
         if (g < (uint32_t)kWideCells && e < count && st.on[g]) {
-            const uint4 ce = list[e];
-            s = ce.x;
-            c = ce.z;
+            // The cell waves staged all of it: no global load before the
+            // Gauss-Seidel chain.
+            const uint2 sc = st.sc[g];
+            s = sc.x;
+            c = sc.y;
             m = kMaxInCell;
             if (j < (uint32_t)kMaxInCell) {
                 q = st.q[g][j];
-                v = b.vel[s + __float_as_uint(q.z)];  // stored position of rank j
+                v = st.v[g][j];
             }
         }
         ranked_group<16>(p, b, q.x, q.y, v.x, v.y, j, m, s, c, 16 * g);
