@@ -498,6 +498,22 @@ def test_every_launch_shape_matches_the_oracle(oracle, scan, per_lane):
         assert_same_state(w.read(), oracle.step(P, *ref, nsteps=17), "17 frames")
 
 
+def test_scan_shape_follows_the_grid_size():
+    """A context starts with 1024 x 8 scan tiles up to 2^23 cells and with
+    1024 x 32 above, where the look-back over thousands of tiles costs more
+    than the larger tiles: worlds of exactly 2^23 cells and just above."""
+    say("[tuning] default scan shape by grid size")
+    with World(WrachConfig(dimensions=(2048, 1024), cell_size=1)) as w:
+        t = w.tuning()
+        assert (t["scan_threads"], t["scan_items"]) == (1024, 8)
+    with World(WrachConfig(dimensions=(4095, 2047), cell_size=1)) as w:  # 4096 x 2048 cells
+        t = w.tuning()
+        assert (t["scan_threads"], t["scan_items"]) == (1024, 8)
+    with World(WrachConfig(dimensions=(4096, 2048), cell_size=1)) as w:  # 4097 x 2049 cells
+        t = w.tuning()
+        assert (t["scan_threads"], t["scan_items"]) == (1024, 32)
+
+
 def test_autotune_keeps_the_state(oracle):
     """wrach_autotune times every candidate on the live state and restores
     it after each trial: the world continues as if it had not run."""
