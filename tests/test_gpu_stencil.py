@@ -332,3 +332,26 @@ def test_stencil_collapsed_lattice(oracle):
             *ref, st = oracle.step_s(P, *ref, max_passes=2, nsteps=1)
             assert_same_state(w.read(), tuple(ref), f"collapsed, mode S frame {frame + 1}")
             assert w.solver_stats() == st
+
+
+def test_stencil_pair_arithmetic_at_its_edges(oracle):
+    """The pass's root and division take short sequences for squared
+    distances from 2^-96 to 1 + 2^-23 and hipcc's full ones below: pairs on
+    both sides of 2^-96, subnormal squares, coincident pairs, d exactly 1,
+    the reach bound 1 + 2^-23 and just past it, near the world's origin
+    where differences can be that small; bit for bit over 3 frames."""
+    base = []
+    for e in (20, 40, 47, 48, 49, 60, 70, 74, 75, 100, 140, 149):
+        d = np.float32(2.0 ** -e)
+        base += [[0.0, 0.0], [float(d), 0.0], [0.0, float(d)], [float(d), float(d)]]
+    base += [[0.25, 0.25], [0.25, 0.25], [0.25, 0.25]]                      # coincident
+    base += [[0.5, 0.3], [1.5, 0.3]]                                          # d == 1
+    one = float(np.sqrt(np.float32(1.0 + 2.0 ** -23)))
+    base += [[0.1, 0.7], [0.1 + one, 0.7]]                                    # d rounds to 1
+    base += [[0.2, 0.9], [float(np.nextafter(np.float32(1.2), np.float32(2))), 0.9]]  # just past reach
+    r = np.random.default_rng(41)
+    tiny = r.uniform(0, 2.0 ** -40, (200, 2))                                 # a crowd at the origin
+    pos = f32(np.concatenate([f32(base), tiny, r.uniform(0, 24, (300, 2))]))
+    vel = f32(np.zeros_like(pos))
+    cfg = WrachConfig(dimensions=(24, 24), cell_size=1)
+    run_stencil(oracle, cfg, pos, vel, 3, 2, every=1, label="pair arithmetic edges")
