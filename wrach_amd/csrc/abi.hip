@@ -117,6 +117,9 @@ struct wrach_ctx {
     uint32_t *scell = nullptr;   // frame-start cell of each particle (capacity entries)
     uint32_t *swords = nullptr;  // wrach::kStencilWords words
     unsigned long long *ssums = nullptr;  // wrach::kStencilSumEntries entries
+    uint32_t *sperm = nullptr;   // the separation kernels' lane order (capacity entries)
+    uint32_t *scrowd = nullptr;  // the frame's crowded cells (crowd_cap entries)
+    uint32_t crowd_cap = 0;
     uint32_t *err_host = nullptr;  // pinned copy of buf.error, read at each synchronisation
     // The learned warm start (wrach_learn_*, WRACH_SOLVER_L): the network's
     // master parameters, gradient and Adam moments (f32, wrach::kParams
@@ -174,17 +177,29 @@ int settle(wrach_ctx *c) {
         if (rc_ != WRACH_OK) return rc_; \
     } while (0)
 
+// The separation kernels' buffers; the lane order when both of its
+// buffers are there.
+wrach::StencilArgs stencil_args(const wrach_ctx *c) {
+    wrach::StencilArgs a{};
+    a.cell = c->scell;
+    a.words = c->swords;
+    a.sums = c->ssums;
+    if (c->sperm && c->scrowd) {
+        a.perm = c->sperm;
+        a.crowd = c->scrowd;
+        a.crowd_cap = c->crowd_cap;
+    }
+    return a;
+}
+
 // Mode S's passes: P0 = pos; pass k reads pos or pos_out (k even or odd)
 // and writes the other (pos_out is free until the collide kernel writes
 // it); an odd count leaves the result in pos_out and a copy brings it back.
 int enqueue_stencil(wrach_ctx *c) {
     if (c->solver != WRACH_SOLVER_S || c->passes == 0 || c->fp.n == 0) return WRACH_OK;
-    wrach::StencilArgs a{};
-    a.cell = c->scell;
-    a.words = c->swords;
-    a.sums = c->ssums;
+    wrach::StencilArgs a = stencil_args(c);
     a.tol = c->overlap_tol;
-    WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->ssums, c->stream));
+    WRACH_HIP(wrach::launch_stencil_clear(c->fp, c->buf, a, c->buf.pos, c->stream));
     const uint32_t passes = c->passes;
     // With a tolerance, the start of every pass is measured before it (the
     // pass runs only above the tolerance).  With tolerance 0 a pass over no
@@ -242,7 +257,8 @@ int enqueue_learned(wrach_ctx *c) {
     if (c->solver != WRACH_SOLVER_L || c->fp.n == 0) return WRACH_OK;
     const uint32_t k = c->passes;
     float2 *X = c->buf.pos, *Y = c->buf.pos_out, *Z = c->lbuf;
-    WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->ssums, c->stream));
+    wrach::StencilArgs a = stencil_args(c);
+    WRACH_HIP(wrach::launch_stencil_clear(c->fp, c->buf, a, X, c->stream));
     wrach::FeatureArgs f{};
     f.pos = X;
     f.vel = c->buf.vel;
@@ -278,10 +294,6 @@ int enqueue_learned(wrach_ctx *c) {
             cur = nx;
         }
     }
-    wrach::StencilArgs a{};
-    a.cell = c->scell;
-    a.words = c->swords;
-    a.sums = c->ssums;
     a.tol = 0.0f;
     a.first = 0;  // the cells came from the features
     for (uint32_t j = 0; j < plan.size(); ++j) {
@@ -368,6 +380,8 @@ void release(wrach_ctx *c) {
     (void)hipFree(c->scell);
     (void)hipFree(c->swords);
     (void)hipFree(c->ssums);
+    (void)hipFree(c->sperm);
+    (void)hipFree(c->scrowd);
     for (void *q : {(void *)c->lparams, (void *)c->lgrad, (void *)c->lm, (void *)c->lv, (void *)c->lw3,
                     (void *)c->lfrag, (void *)c->lfeat, (void *)c->lbuf, (void *)c->tfeat,
                     (void *)c->ttarget, (void *)c->tsel, (void *)c->txt, (void *)c->ta1t,
@@ -1023,6 +1037,19 @@ int ensure_stencil(wrach_ctx *c) {
         WRACH_HIP(dmalloc(&sums, wrach::kStencilSumEntries));
         WRACH_HIP(hipMemset(sums, 0, wrach::kStencilSumEntries * sizeof(unsigned long long)));
         c->ssums = sums;
+    }
+    if (!c->sperm) {
+        uint32_t *perm = nullptr;
+        WRACH_HIP(dmalloc(&perm, c->capacity ? c->capacity : 1));
+        c->sperm = perm;
+    }
+    if (!c->scrowd) {
+        // A crowded cell holds at least kCrowdMin particles.
+        const uint32_t cap = c->capacity / wrach::kCrowdMin + 1;
+        uint32_t *crowd = nullptr;
+        WRACH_HIP(dmalloc(&crowd, cap));
+        c->scrowd = crowd;
+        c->crowd_cap = cap;
     }
     return WRACH_OK;
 }
@@ -1979,11 +2006,8 @@ int wrach_learn_sample(wrach_ctx *c, uint32_t target_passes, uint64_t *rows_tota
         WRACH_HIP(wrach::launch_learn_features(c->fp, c->buf, f, c->stream));
         // The plain passes from the state, which stays as it is: pos ->
         // pos_out -> lbuf -> pos_out ...
-        wrach::StencilArgs a{};
-        a.cell = c->scell;
-        a.words = c->swords;
-        a.sums = c->ssums;
-        WRACH_HIP(wrach::launch_stencil_clear(c->swords, c->ssums, c->stream));
+        wrach::StencilArgs a = stencil_args(c);
+        WRACH_HIP(wrach::launch_stencil_clear(c->fp, c->buf, a, c->buf.pos, c->stream));
         const float2 *in = c->buf.pos;
         for (uint32_t k = 0; k < target_passes; ++k) {
             float2 *out = k % 2 == 0 ? c->buf.pos_out : c->lbuf;

@@ -221,7 +221,7 @@ hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, h
 // largest overlap (float bits) after 0..max_passes passes for the last
 // frame; the passes that ran are those whose start exceeded the tolerance.
 constexpr uint32_t kMaxStencilPasses = 64;
-constexpr uint32_t kStencilWords = kMaxStencilPasses + 2;
+constexpr uint32_t kStencilWords = kMaxStencilPasses + 3;
 constexpr uint32_t kStencilBudget = 4096;  // most particles one lane visits in its 3x3 cells
 // Overlap sums (1 - d in 2^-24 units) and pair counts of the frame's first
 // and last measure (sets 0 and 1), and of the learned warm start's output
@@ -237,7 +237,15 @@ constexpr uint32_t kStencilSumEntries = kStencilSumSets * kStencilSumSlots * kSt
 // frame-start positions (the warm start left more mean overlap than it
 // found), else 0.
 constexpr uint32_t kFallbackWord = kMaxStencilPasses + 1;
+// words[kCrowdCountWord]: the crowded cells listed for the frame.
+constexpr uint32_t kCrowdCountWord = kMaxStencilPasses + 2;
 constexpr float kOverlapScale = 16777216.0f;  // 2^24
+// Crowded cells (kCrowdMin..kCrowdMax particles): the separation kernels'
+// lanes take their particles in order of x (the lane order; the results
+// do not depend on it), so that a wavefront's particles lie close together
+// and skip more of their neighbours together.
+constexpr uint32_t kCrowdMin = 64;
+constexpr uint32_t kCrowdMax = 1024;
 struct StencilArgs {
     const float2 *pin;  // positions entering the kernel
     float2 *pout;       // a move's output
@@ -254,12 +262,22 @@ struct StencilArgs {
     // warm start (sums set 2) exceeds that of the frame start (set 0), and
     // block 0 records the choice in words[kFallbackWord].
     const float2 *pin_alt;
+    // The lane order (null: lane t takes particle t): lane t takes particle
+    // perm[t], a permutation that orders each crowded cell by x; the
+    // crowded cells of the frame (words[kCrowdCountWord] of them, at most
+    // crowd_cap).
+    uint32_t *perm;
+    uint32_t *crowd;
+    uint32_t crowd_cap;
 };
 hipError_t launch_stencil_measure(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
                                   hipStream_t stream);
 hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
                                hipStream_t stream);
-hipError_t launch_stencil_clear(uint32_t *words, unsigned long long *sums, hipStream_t stream);
+// The frame's first stencil launch: clears the overlap words and sums and,
+// with a.perm, orders the lanes of the crowded cells by the x of `pos`.
+hipError_t launch_stencil_clear(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
+                                const float2 *pos, hipStream_t stream);
 hipError_t launch_copy_pos(float2 *dst, const float2 *src, uint32_t n, hipStream_t stream);
 
 }  // namespace wrach

@@ -2103,9 +2103,14 @@ __device__ __forceinline__ bool warm_rejected(const StencilArgs &a) {
     return s_rej != 0;
 }
 
+// The particle of lane t (see StencilArgs::perm).
+__device__ __forceinline__ uint32_t lane_particle(const FrameParams &p, const StencilArgs &a, uint32_t t) {
+    return a.perm && t < p.n ? a.perm[t] : t;
+}
+
 __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, FrameBuffers b,
                                                               StencilArgs a) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t i = lane_particle(p, a, blockIdx.x * 256 + threadIdx.x);
     float worst = 0.0f;
     uint32_t pairs = 0;
     unsigned long long osum = 0;
@@ -2128,7 +2133,7 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
 // moves nothing) needs no measure between passes.
 __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameBuffers b,
                                                            StencilArgs a) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t i = lane_particle(p, a, blockIdx.x * 256 + threadIdx.x);
     if (a.check && !(__uint_as_float(a.words[a.in_slot]) > a.tol)) {
         if (i < p.n) a.pout[i] = a.pin[i];
         return;  // the whole grid: no barrier follows
@@ -2174,6 +2179,93 @@ __global__ __launch_bounds__(256) void stencil_clear_kernel(uint32_t *w, unsigne
 __global__ __launch_bounds__(256) void copy_pos_kernel(float2 *dst, const float2 *src, uint32_t n) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) dst[i] = src[i];
+}
+
+// The lane order of the separation kernels.  A pass's sums are exact and
+// each lane writes its own particle, so any order gives the same result;
+// this one puts each crowded cell's particles in order of x (columns of
+// particles are the common crowd), so that the 64 lanes of a wavefront lie
+// close together and more of their neighbours are out of every lane's
+// reach at once.  Cells of kCrowdMin..kCrowdMax particles, one thread per
+// cell, one atomic per wavefront; the grid also sets perm to the identity.
+__global__ __launch_bounds__(256) void crowd_list_kernel(FrameParams p, FrameBuffers b, StencilArgs a) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    for (uint64_t t = c; t < p.n; t += (uint64_t)gridDim.x * 256) a.perm[t] = (uint32_t)t;
+    bool crowded = false;
+    if (c < p.ncell) {
+        const uint32_t s0 = cell_start(p, b, c), s1 = cell_start(p, b, c + 1);
+        crowded = s1 >= s0 && s1 - s0 >= kCrowdMin && s1 - s0 <= kCrowdMax;
+    }
+    const unsigned long long m = __ballot(crowded);
+    if (m == 0) return;
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(&a.words[kCrowdCountWord], (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader, 64);
+    if (crowded) {
+        const uint32_t slot = base + (uint32_t)__popcll(m & lanemask_lt());
+        if (slot < a.crowd_cap) a.crowd[slot] = c;
+    }
+}
+
+constexpr uint32_t kOrderSlices = 64;    // x slices of a crowded cell: one per lane
+constexpr uint32_t kCrowdOrderGrid = 1024;  // blocks of crowd_order_kernel (4 wavefronts each)
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// One wavefront per listed cell (the wavefronts stride over the list): the
+// cell's particles counted into kOrderSlices equal slices of their x range
+// (LDS atomics give each its rank in its slice), the slices' starts by a
+// wavefront scan, then perm over the cell's range in slice order; inside a
+// slice the order is the atomics'.  Slices of 1/64 of the range are far
+// finer than the 64 lanes of a wavefront need.  perm holds the identity
+// before.
+__global__ __launch_bounds__(256) void crowd_order_kernel(FrameParams p, FrameBuffers b, StencilArgs a,
+                                                          const float2 *pos) {
+    __shared__ uint32_t slice_count[4][kOrderSlices];
+    __shared__ uint32_t tag[4][kCrowdMax];  // a particle's slice << 16 | its rank in the slice
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = lane_id();
+    const uint32_t listed = min(a.words[kCrowdCountWord], a.crowd_cap);
+    for (uint32_t e = blockIdx.x * 4 + wave; e < listed; e += gridDim.x * 4) {
+        const uint32_t c = a.crowd[e];
+        const uint32_t s0 = cell_start(p, b, c), s1 = cell_start(p, b, c + 1);
+        const uint32_t n = s1 > s0 ? s1 - s0 : 0u;
+        if (n > kCrowdMax) continue;  // never: the list holds cells of at most kCrowdMax
+        float lo = INFINITY, hi = -INFINITY;
+        for (uint32_t r = lane; r < n; r += 64) {
+            const float x = pos[s0 + r].x;
+            lo = fminf(lo, x);
+            hi = fmaxf(hi, x);
+        }
+        lo = wave_min(lo);
+        hi = wave_max(hi);
+        // Not finite (an infinite range): every particle in slice 0.
+        const float scale = hi > lo ? (float)kOrderSlices / (hi - lo) : 0.0f;
+        slice_count[wave][lane] = 0;
+        wave_sync();
+        for (uint32_t r = lane; r < n; r += 64) {
+            // fmaxf takes 0 for a NaN; the slice is at most kOrderSlices - 1.
+            const float f = fminf(fmaxf((pos[s0 + r].x - lo) * scale, 0.0f), (float)(kOrderSlices - 1));
+            const uint32_t sl = (uint32_t)f;
+            const uint32_t rank = atomicAdd(&slice_count[wave][sl], 1u);
+            tag[wave][r] = sl << 16 | rank;
+        }
+        wave_sync();
+        const uint32_t cnt = slice_count[wave][lane];
+        wave_sync();
+        slice_count[wave][lane] = wave_inclusive_sum(cnt) - cnt;
+        wave_sync();
+        for (uint32_t r = lane; r < n; r += 64) {
+            const uint32_t t = tag[wave][r];
+            const uint32_t slot = slice_count[wave][t >> 16] + (t & 0xffffu);
+            if (slot < n) a.perm[s0 + slot] = s0 + r;  // always: the counts sum to n
+        }
+        wave_sync();  // this cell's LDS reads are done before the next cell's writes
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2612,8 +2704,13 @@ hipError_t launch_learn_features(const FrameParams &p, const FrameBuffers &b, co
     return hipGetLastError();
 }
 
-hipError_t launch_stencil_clear(uint32_t *words, unsigned long long *sums, hipStream_t stream) {
-    hipLaunchKernelGGL(stencil_clear_kernel, dim3(1), dim3(256), 0, stream, words, sums);
+hipError_t launch_stencil_clear(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
+                                const float2 *pos, hipStream_t stream) {
+    hipLaunchKernelGGL(stencil_clear_kernel, dim3(1), dim3(256), 0, stream, a.words, a.sums);
+    if (a.perm && p.n && p.ncell) {
+        hipLaunchKernelGGL(crowd_list_kernel, dim3((p.ncell + 255) / 256), dim3(256), 0, stream, p, b, a);
+        hipLaunchKernelGGL(crowd_order_kernel, dim3(kCrowdOrderGrid), dim3(256), 0, stream, p, b, a, pos);
+    }
     return hipGetLastError();
 }
 
