@@ -355,3 +355,19 @@ def test_stencil_pair_arithmetic_at_its_edges(oracle):
     vel = f32(np.zeros_like(pos))
     cfg = WrachConfig(dimensions=(24, 24), cell_size=1)
     run_stencil(oracle, cfg, pos, vel, 3, 2, every=1, label="pair arithmetic edges")
+
+
+def test_pair_arithmetic_short_sequences_are_exact():
+    """The root and force sequences that modes S and L use for squared
+    distances in [2^-96, 1 + 2^-23] and distances in [2^-48, 1] give hipcc's
+    correctly rounded sqrtf and division on every float of those ranges.  The
+    force's one-correction sequence depends on the hardware reciprocal, so
+    this runs on the device itself, over all 805,306,370 and 402,653,185
+    inputs."""
+    import ctypes
+
+    say("[stencil] pair arithmetic: exhaustive root and force check on device 0")
+    L = _lib.load()
+    out = (ctypes.c_uint64 * 4)()
+    _lib.check(L.wrach_check_pair_arithmetic(0, out))
+    assert list(out) == [0, 0, 805306370, 402653185]

@@ -160,6 +160,7 @@ extern "C" {
     pub fn wrach_step_profiled(ctx: *mut wrach_ctx, nframes: u32, kernel_ms: *mut f64) -> c_int;
     pub fn wrach_prepare(ctx: *mut wrach_ctx, nframes: u32) -> c_int;
     pub fn wrach_test_scan_wait(ctx: *mut wrach_ctx, wait_limit: u32, epoch_skew: u32) -> c_int;
+    pub fn wrach_check_pair_arithmetic(device: c_int, out: *mut u64) -> c_int;
     pub fn wrach_learn_set_params(ctx: *mut wrach_ctx, params: *const f32, n: usize) -> c_int;
     pub fn wrach_learn_get_params(ctx: *mut wrach_ctx, params: *mut f32, n: usize) -> c_int;
     pub fn wrach_learn_features(ctx: *mut wrach_ctx, rows: *mut u16, rows_cap: usize, overlap: *mut u64) -> c_int;

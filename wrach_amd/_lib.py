@@ -76,6 +76,7 @@ EXPORTS = (
     "wrach_learn_profile",
     "wrach_frame_noalloc",
     "wrach_test_scan_wait",
+    "wrach_check_pair_arithmetic",
     "wrach_last_error",
 )
 
@@ -219,6 +220,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     L.wrach_frame_noalloc.argtypes = [ctypes.POINTER(WorldSettingsC), f32p, ctypes.c_float, vp, vp, vp,
                                       ctypes.c_uint32, vp]
     L.wrach_test_scan_wait.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
+    L.wrach_check_pair_arithmetic.argtypes = [ctypes.c_int, u64p]
     L.wrach_last_error.argtypes = []
     L.wrach_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:

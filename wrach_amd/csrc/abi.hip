@@ -2316,6 +2316,25 @@ int wrach_frame_noalloc(const wrach_world_settings *ws, const float gravity[2], 
     return WRACH_OK;
 }
 
+int wrach_check_pair_arithmetic(int device, uint64_t out[4]) {
+    if (!out) return fail(WRACH_E_INVALID, "null argument");
+    int ndev = 0;
+    WRACH_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(WRACH_E_INVALID, "no such device");
+    WRACH_HIP(hipSetDevice(device));
+    unsigned long long *d = nullptr;
+    WRACH_HIP(dmalloc(&d, 4));
+    unsigned long long h[4] = {};
+    hipError_t e = hipMemset(d, 0, sizeof h);
+    if (e == hipSuccess) e = wrach::launch_check_pair_arithmetic(d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);  // waits for the kernel
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(WRACH_E_DEVICE, std::string("pair arithmetic check: ") + hipGetErrorString(e));
+    for (int k = 0; k < 4; ++k) out[k] = h[k];
+    g_err.clear();
+    return WRACH_OK;
+}
+
 int wrach_worlds(wrach_ctx *c, uint32_t *n_worlds) {
     if (!c || !n_worlds) return fail(WRACH_E_INVALID, "null argument");
     *n_worlds = c->worlds;

@@ -279,5 +279,9 @@ hipError_t launch_stencil_move(const FrameParams &p, const FrameBuffers &b, cons
 hipError_t launch_stencil_clear(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
                                 const float2 *pos, hipStream_t stream);
 hipError_t launch_copy_pos(float2 *dst, const float2 *src, uint32_t n, hipStream_t stream);
+// Adds to out[0..3] (zeroed by the caller) the mismatches of the pair
+// arithmetic's short root and force sequences against the correctly
+// rounded ones over their whole input ranges, and the inputs tried.
+hipError_t launch_check_pair_arithmetic(unsigned long long *out, hipStream_t stream);
 
 }  // namespace wrach

@@ -1870,25 +1870,54 @@ __device__ __forceinline__ float sqrt_rn_above_2m96(float x) {
     return __builtin_fmaf(-above, s, x) > 0.0f ? above : r;
 }
 
-// a / b correctly rounded, for b in [2^-48, 1] and a = 0 or a in
-// [2^-26, 2^30): hipcc's f32 division (the reciprocal, one Newton step on
-// it, the quotient and two residual corrections) without v_div_scale and
-// v_div_fixup.  For such operands v_div_scale scales nothing (the exponent
-// difference stays below 96, neither operand nor the quotient is
-// subnormal; for a = 0 it returns NaN, which v_div_fixup turns into +0,
-// the quotient this sequence gives), and v_div_fixup rewrites only special
-// operands and sets the sign the positive quotient already has.
-__device__ __forceinline__ float div_rn_bounded(float a, float b) {
-    const float y0 = __builtin_amdgcn_rcpf(b);
-    const float e = __builtin_fmaf(-b, y0, 1.0f);
-    const float y = __builtin_fmaf(e, y0, y0);
-    const float q0 = a * y;
-    const float r0 = __builtin_fmaf(-b, q0, a);
-    const float q1 = __builtin_fmaf(r0, y, q0);
-    const float r1 = __builtin_fmaf(-b, q1, a);
-    return __builtin_fmaf(r1, y, q1);
+// (1 - d) * 2^29 / d correctly rounded, for d in [2^-48, 1] (so 1 - d is
+// the overlap and the numerator a = (1 - d) * 2^29 is 0 or in
+// [2^5, 2^29]): the hardware reciprocal y of d, the quotient a * y and one
+// residual correction fma(fma(-d, q, a), y, q).  hipcc's f32 division adds
+// a Newton step on y, a second correction, and v_div_scale / v_div_fixup
+// (identities here: the exponent difference stays below 96, nothing is
+// subnormal, and the fix-up only rewrites special operands).  That the
+// short sequence equals the correctly rounded quotient for every d of the
+// range is not a theorem but a property of the hardware reciprocal;
+// check_pair_arithmetic_kernel tests it for all 402,653,185 values of d
+// on the device in use (wrach_check_pair_arithmetic).
+__device__ __forceinline__ float overlap_force_rn(float d) {
+    const float a = (1.0f - d) * 0x1p29f;
+    const float y = __builtin_amdgcn_rcpf(d);
+    const float q = a * y;
+    const float r = __builtin_fmaf(-d, q, a);
+    return __builtin_fmaf(r, y, q);
 }
 
+
+// The pair arithmetic's short sequences against hipcc's correctly rounded
+// ones over their whole input ranges: sqrt_rn_above_2m96(x) against
+// sqrtf(x) for every float x in [2^-96, 1 + 2^-23], overlap_force_rn(d)
+// against ((1 - d) * 2^29) / d for every float d in [2^-48, 1].  Adds the
+// mismatches to out[0] and out[1] and the inputs tried to out[2] and
+// out[3] (one atomic per wavefront and counter).
+__global__ __launch_bounds__(256) void check_pair_arithmetic_kernel(unsigned long long *out) {
+    const uint32_t s_lo = __float_as_uint(0x1p-96f), s_hi = __float_as_uint(1.00000011920928955078125f);
+    const uint32_t d_lo = __float_as_uint(0x1p-48f), d_hi = __float_as_uint(1.0f);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bad_s = 0, bad_d = 0, n_s = 0, n_d = 0;
+    for (uint64_t u = s_lo + t0; u <= s_hi; u += stride) {
+        const float x = __uint_as_float((uint32_t)u);
+        bad_s += __float_as_uint(sqrt_rn_above_2m96(x)) != __float_as_uint(sqrtf(x));
+        ++n_s;
+    }
+    for (uint64_t u = d_lo + t0; u <= d_hi; u += stride) {
+        const float d = __uint_as_float((uint32_t)u);
+        const float full = ((1.0f - d) * 0x1p29f) / d;
+        bad_d += __float_as_uint(overlap_force_rn(d)) != __float_as_uint(full);
+        ++n_d;
+    }
+    const uint32_t v[4] = {wave_sum(bad_s), wave_sum(bad_d), wave_sum(n_s), wave_sum(n_d)};
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k)
+            if (v[k]) atomicAdd(&out[k], (unsigned long long)v[k]);
+}
 
 // The particles of the 3x3 cells around (cx, cy) as three ranges, one per
 // row cy - 1 .. cy + 1: the cells of one row are contiguous in the packed
@@ -1964,9 +1993,9 @@ __device__ __forceinline__ bool stencil_reach(float dx, float dy, bool other, fl
     }
     const float d = sqrt_rn_above_2m96(sq);  // in [2^-48, 1]: never 0
     ov = 1.0f - d;
-    // ov * 2^29 is 0 or in [2^5, 2^29]: div_rn_bounded's range, and the
-    // quotient is (0.5 * ov / d) * 2^30 exactly, as f is normal.
-    F = div_rn_bounded(ov * 0x1p29f, d);
+    // d is in [2^-48, 1]: overlap_force_rn's range, and the quotient is
+    // (0.5 * ov / d) * 2^30 exactly, as f is normal.
+    F = overlap_force_rn(d);
     return true;
 }
 
@@ -2011,9 +2040,11 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
         const uint32_t e = re[t];
         // Four neighbours a step, all loaded before the first is used.
         for (; j + 4 <= e; j += 4) {
+            // One address, four loads at constant offsets from it.
+            const float2 *qp = pin + j;
             float2 q[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = pin[j + u];
+            for (int u = 0; u < 4; ++u) q[u] = qp[u];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 stencil_visit<MOVE, SUMS>(j + u != i, xi, yi, q[u], worst, pairs, osum, ax, ay);
@@ -2711,6 +2742,11 @@ hipError_t launch_stencil_clear(const FrameParams &p, const FrameBuffers &b, con
         hipLaunchKernelGGL(crowd_list_kernel, dim3((p.ncell + 255) / 256), dim3(256), 0, stream, p, b, a);
         hipLaunchKernelGGL(crowd_order_kernel, dim3(kCrowdOrderGrid), dim3(256), 0, stream, p, b, a, pos);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_check_pair_arithmetic(unsigned long long *out, hipStream_t stream) {
+    hipLaunchKernelGGL(check_pair_arithmetic_kernel, dim3(4096), dim3(256), 0, stream, out);
     return hipGetLastError();
 }
 
