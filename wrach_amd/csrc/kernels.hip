@@ -2000,17 +2000,19 @@ __device__ __forceinline__ bool stencil_reach(float dx, float dy, bool other, fl
 }
 
 // One neighbour of particle i (`other`: not i itself) for stencil_particle.
-// ov * 2^24 is an integer: ov is a multiple of 2^-24 in [0, 1].
+// ov is a multiple of 2^-24 in [0, 1], so a lane's sum of at most
+// kStencilBudget of them is a multiple of 2^-24 below 2^12: exact in a
+// double, and the sum times 2^24 an integer below 2^36.
 template <bool MOVE, bool SUMS>
 __device__ __forceinline__ void stencil_visit(bool other, float xi, float yi, float2 q,
                                               uint32_t &worst, uint32_t &pairs,
-                                              unsigned long long &osum, double &ax, double &ay) {
+                                              double &osum, double &ax, double &ay) {
     const float dx = xi - q.x, dy = yi - q.y;
     float ov, F;
     if (!stencil_reach(dx, dy, other, ov, F)) return;
     worst = max(worst, __float_as_uint(ov));
     pairs += 1;
-    if (SUMS) osum += (uint32_t)(ov * kOverlapScale);
+    if (SUMS) osum += (double)ov;
     if (MOVE) {
         ax += (double)rintf(dx * F);
         ay += (double)rintf(dy * F);
@@ -2026,7 +2028,7 @@ template <bool MOVE, bool SUMS>
 __device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
                                                   const float2 *pin, uint32_t i, uint32_t k,
                                                   float xi, float yi, double &ax, double &ay,
-                                                  uint32_t &pairs, unsigned long long &osum) {
+                                                  uint32_t &pairs, double &osum) {
     uint32_t rs[3], re[3];
     const Grid g = grid_of(p, k);
     uint32_t cx, cy;
@@ -2144,7 +2146,7 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
     const uint32_t i = lane_particle(p, a, blockIdx.x * 256 + threadIdx.x);
     float worst = 0.0f;
     uint32_t pairs = 0;
-    unsigned long long osum = 0;
+    double osum = 0.0;  // overlaps: multiples of 2^-24, at most kStencilBudget of them
     if (i < p.n) {
         const float2 q = a.pin[i];
         const uint32_t k = stencil_cell(p, b, a, i, q);
@@ -2153,7 +2155,8 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
             worst = a.stat ? stencil_particle<false, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum)
                            : stencil_particle<false, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
     }
-    block_report(&a.words[a.in_slot], worst, stencil_sums(a), osum, pairs);
+    block_report(&a.words[a.in_slot], worst, stencil_sums(a),
+                 (unsigned long long)(osum * (double)kOverlapScale), pairs);
 }
 
 // One pass: a.pin -> a.pout.  With a.check it runs only while the
@@ -2172,7 +2175,7 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
     float worst = 0.0f;
     double ax = 0.0, ay = 0.0;  // integers: the sums of the terms in 2^-30 units
     uint32_t pairs = 0;
-    unsigned long long osum = 0;
+    double osum = 0.0;  // overlaps: multiples of 2^-24, at most kStencilBudget of them
     float2 q = make_float2(0.0f, 0.0f);
     if (a.pin_alt) {
         const bool rej = warm_rejected(a);
@@ -2186,7 +2189,8 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
             worst = a.stat ? stencil_particle<true, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum)
                            : stencil_particle<true, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
     }
-    block_report(&a.words[a.in_slot], worst, stencil_sums(a), osum, pairs);
+    block_report(&a.words[a.in_slot], worst, stencil_sums(a),
+                 (unsigned long long)(osum * (double)kOverlapScale), pairs);
     if (i >= p.n) return;
     // The mean of the pair corrections (constraint averaging): a particle
     // moves at most 0.5 per pass.
