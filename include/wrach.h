@@ -223,8 +223,9 @@ int wrach_settings_world(wrach_ctx *ctx, uint32_t world, wrach_world_settings *o
 typedef struct wrach_tuning {
     uint32_t scan_threads;     /* threads per scan tile: 1024 or 512 */
     uint32_t scan_items;       /* counts per thread: (1024, 8), (512, 16), (1024, 4), (512, 8),
-                                  (1024, 16), (1024, 32); a context starts at (1024, 8), or
-                                  at (1024, 32) above 2^23 cells */
+                                  (1024, 16), (1024, 32), (1024, 64), (512, 64), (256, 128); a
+                                  context starts at (1024, 8), at (1024, 64) above 2^23
+                                  cells and at (256, 128) above 2^26 */
     uint32_t scatter_per_lane; /* particles per lane of the scatter: 2, 1 or 4 */
     uint32_t cells_grid;       /* blocks of the queued-cell kernel, 1..4x the default */
     float frame_us;            /* wrach_autotune: the chosen shapes' frame time; else 0 */

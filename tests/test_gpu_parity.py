@@ -469,7 +469,8 @@ def test_store_write_back_then_append(oracle, host):
         assert_same_state(w.read(), oracle.step(P, *ref, nsteps=5), "5 frames after the append")
 
 
-SCAN_SHAPES = [(1024, 8), (512, 16), (1024, 4), (512, 8), (1024, 16), (1024, 32)]
+SCAN_SHAPES = [(1024, 8), (512, 16), (1024, 4), (512, 8), (1024, 16), (1024, 32), (1024, 64), (512, 64),
+               (256, 128)]
 
 
 @pytest.mark.parametrize("scan", SCAN_SHAPES, ids=lambda s: f"scan{s[0]}x{s[1]}")
@@ -498,20 +499,41 @@ def test_every_launch_shape_matches_the_oracle(oracle, scan, per_lane):
         assert_same_state(w.read(), oracle.step(P, *ref, nsteps=17), "17 frames")
 
 
+@pytest.mark.parametrize("scan", [(1024, 64), (512, 64), (256, 128)], ids=lambda s: f"scan{s[0]}x{s[1]}")
+def test_largest_scan_tiles_over_many_tiles(oracle, scan):
+    """Scan tiles of 32k and 64k counts over a 1201 x 801 grid (15 or 30
+    tiles, the last one partial), with chunks that hold no particle between
+    live ones."""
+    say(f"[tuning] scan {scan} over many tiles")
+    r = np.random.default_rng(12)
+    cfg = WrachConfig(dimensions=(1200, 800), cell_size=1, gravity=(0.0, -0.1))
+    pos = np.concatenate([r.uniform((0, 0), (1200, 40), (6000, 2)), r.uniform((0, 300), (1200, 310), (3000, 2)),
+                          np.repeat(f32([[1199.5, 799.5], [600.25, 400.75]]), [70, 200], 0)
+                          + r.uniform(-0.2, 0.2, (270, 2))]).astype(np.float32)
+    vel = f32(r.uniform(-1, 1, pos.shape))
+    P, ref = oracle_world(oracle, cfg, pos, vel)
+    with World(cfg) as w:
+        w.set_tuning(scan_threads=scan[0], scan_items=scan[1])
+        w.add_particles(pos, vel)
+        assert_same_state(w.read(), ref, "upload (scan of the store)")
+        w.step(9)
+        assert_same_state(w.read(), oracle.step(P, *ref, nsteps=9), "9 frames")
+
+
 def test_scan_shape_follows_the_grid_size():
-    """A context starts with 1024 x 8 scan tiles up to 2^23 cells and with
-    1024 x 32 above, where the look-back over thousands of tiles costs more
-    than the larger tiles: worlds of exactly 2^23 cells and just above."""
+    """A context starts with 1024 x 8 scan tiles up to 2^23 cells, 1024 x 64
+    up to 2^26 and 256 x 128 above, where the look-back over thousands of
+    tiles costs more than the larger tiles: worlds of exactly 2^23 and 2^26
+    cells and just above."""
     say("[tuning] default scan shape by grid size")
-    with World(WrachConfig(dimensions=(2048, 1024), cell_size=1)) as w:
-        t = w.tuning()
-        assert (t["scan_threads"], t["scan_items"]) == (1024, 8)
-    with World(WrachConfig(dimensions=(4095, 2047), cell_size=1)) as w:  # 4096 x 2048 cells
-        t = w.tuning()
-        assert (t["scan_threads"], t["scan_items"]) == (1024, 8)
-    with World(WrachConfig(dimensions=(4096, 2048), cell_size=1)) as w:  # 4097 x 2049 cells
-        t = w.tuning()
-        assert (t["scan_threads"], t["scan_items"]) == (1024, 32)
+    for dims, shape in (((2048, 1024), (1024, 8)),
+                        ((4095, 2047), (1024, 8)),      # 4096 x 2048 cells
+                        ((4096, 2048), (1024, 64)),     # 4097 x 2049 cells
+                        ((8191, 8191), (1024, 64)),     # 8192 x 8192 cells
+                        ((8192, 8191), (256, 128))):    # 8193 x 8192 cells
+        with World(WrachConfig(dimensions=dims, cell_size=1)) as w:
+            t = w.tuning()
+            assert (t["scan_threads"], t["scan_items"]) == shape, dims
 
 
 def test_autotune_keeps_the_state(oracle):

@@ -104,21 +104,32 @@ constexpr uint32_t kChunkCells = 1u << kChunkShift;
 struct ScanShape {
     int threads, items;
 };
-// The last two, 16k and 32k cells per tile, are for large grids that are
-// mostly empty (a batch of worlds), where the number of tiles sets the time.
-constexpr ScanShape kScanShapes[] = {{1024, 8}, {512, 16}, {1024, 4}, {512, 8}, {1024, 16}, {1024, 32}};
-constexpr int kNumScanShapes = 6;
-// The scan shape a context starts with: entry 0 up to kLargeScanGrid cells;
-// above, where the look-back over thousands of tiles dominates, 1024 x 32
-// (a quarter of the tiles).  Measured: the 1M lattice's 2.1M cells scan in
-// 7.6 us at 1024 x 8 and 18.3 us at 1024 x 32; the 4M blobs' 16.8M cells in
-// 66.5 and 57.9 us; the 512-world batch's 227M cells in 827 and 509 us.
+// The shapes of 16k to 64k cells per tile are for large grids that are
+// mostly empty (one large world, a batch of worlds), where the number of
+// tiles and the look-back between them set the time.  A 1024 x 32 or
+// 1024 x 64 tile needs 74 or 105 registers, so one tile runs on a CU at a
+// time; 512 x 64 and 256 x 128 tiles (103 and 200 registers) run two to a
+// CU, one loading while the other looks back.
+constexpr ScanShape kScanShapes[] = {{1024, 8}, {512, 16}, {1024, 4}, {512, 8}, {1024, 16}, {1024, 32},
+                                     {1024, 64}, {512, 64}, {256, 128}};
+constexpr int kNumScanShapes = 9;
+// The scan shape a context starts with: entry 0 up to kLargeScanGrid cells,
+// 1024 x 64 up to kHugeScanGrid, 256 x 128 above.  Measured (median of
+// interleaved rounds): the 1M lattice's 2.1M cells scan in 8.1 us at
+// 1024 x 8, 18.3 at 1024 x 32, 31.7 at 1024 x 64 and 17.4 at 256 x 128;
+// the 4M blobs' 16.8M cells in 67.5, 57.8, 50.3 and 53.4 us; the
+// 512-world batch's 227M cells at 1024 x 32, 1024 x 64, 512 x 64 and
+// 256 x 128 in 401, 344, 338 and 322 us.
 constexpr uint64_t kLargeScanGrid = 1ull << 23;
-constexpr uint32_t kLargeScanShape = 5;
-static_assert(kScanShapes[kLargeScanShape].threads == 1024 && kScanShapes[kLargeScanShape].items == 32,
+constexpr uint64_t kHugeScanGrid = 1ull << 26;
+constexpr uint32_t kLargeScanShape = 6;
+constexpr uint32_t kHugeScanShape = 8;
+static_assert(kScanShapes[kLargeScanShape].threads == 1024 && kScanShapes[kLargeScanShape].items == 64,
               "the large-grid scan shape");
+static_assert(kScanShapes[kHugeScanShape].threads == 256 && kScanShapes[kHugeScanShape].items == 128,
+              "the huge-grid scan shape");
 constexpr uint32_t default_scan_shape(uint64_t ncell) {
-    return ncell > kLargeScanGrid ? kLargeScanShape : 0u;
+    return ncell > kHugeScanGrid ? kHugeScanShape : ncell > kLargeScanGrid ? kLargeScanShape : 0u;
 }
 constexpr uint32_t kMinScanTile = 4096;  // smallest threads x items above (sizes tile_state)
 constexpr int kScatterBlock = 256;

@@ -2628,8 +2628,11 @@ static hipError_t launch_scan_as(const FrameParams &p, const FrameBuffers &b, hi
 
 hipError_t launch_scan(const FrameParams &p, const FrameBuffers &b, hipStream_t stream,
                        hipEvent_t start, hipEvent_t stop) {
-    static_assert(kNumScanShapes == 6, "one case per scan shape");
+    static_assert(kNumScanShapes == 9, "one case per scan shape");
     switch (b.scan_shape) {
+        case 8: return launch_scan_as<8>(p, b, stream, start, stop);
+        case 7: return launch_scan_as<7>(p, b, stream, start, stop);
+        case 6: return launch_scan_as<6>(p, b, stream, start, stop);
         case 1: return launch_scan_as<1>(p, b, stream, start, stop);
         case 2: return launch_scan_as<2>(p, b, stream, start, stop);
         case 3: return launch_scan_as<3>(p, b, stream, start, stop);
