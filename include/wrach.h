@@ -338,9 +338,11 @@ int wrach_test_scan_wait(wrach_ctx *ctx, uint32_t wait_limit, uint32_t epoch_ske
 /* Self-check of the separation passes' pair arithmetic on one device (no
  * counterpart in the reference).  Modes S and L take the root of a squared
  * distance x in [2^-96, 1 + 2^-23] and the force (1 - d) * 2^29 / d for d
- * in [2^-48, 1] by sequences shorter than hipcc's sqrtf and division; the
- * root's is correctly rounded by construction, the force's by a property
- * of the hardware reciprocal.  This runs both over every float of their
+ * in [2^-48, 1] by sequences shorter than hipcc's sqrtf and division
+ * (one Newton step from the hardware reciprocal root, one residual
+ * correction from the hardware reciprocal), which are correctly rounded by
+ * a property of those hardware approximations, not by a theorem.  This
+ * runs both over every float of their
  * ranges against the correctly rounded sequences: out[0] and out[1] are
  * the mismatches of the root and the force (0 and 0 where modes S and L
  * are exact), out[2] and out[3] the inputs tried (805,306,370 and

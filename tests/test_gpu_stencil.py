@@ -361,9 +361,10 @@ def test_pair_arithmetic_short_sequences_are_exact():
     """The root and force sequences that modes S and L use for squared
     distances in [2^-96, 1 + 2^-23] and distances in [2^-48, 1] give hipcc's
     correctly rounded sqrtf and division on every float of those ranges.  The
-    force's one-correction sequence depends on the hardware reciprocal, so
-    this runs on the device itself, over all 805,306,370 and 402,653,185
-    inputs."""
+    root's Newton step from the hardware reciprocal root and the force's
+    one-correction sequence from the hardware reciprocal are exact only by
+    a property of those approximations, so this runs on the device itself,
+    over all 805,306,370 and 402,653,185 inputs."""
     import ctypes
 
     say("[stencil] pair arithmetic: exhaustive root and force check on device 0")

@@ -1855,19 +1855,19 @@ __global__ __launch_bounds__(256) void upload_scatter_kernel(UploadParams u, Fra
 constexpr float kStencilScale = 1073741824.0f;            // 2^30
 constexpr float kStencilUnit = 9.313225746154785e-10f;    // 2^-30
 
-// sqrtf(x) for x in [2^-96, 2]: v_sqrt_f32 (within one unit in the last
-// place), then its neighbour one unit below when x - below * s <= 0, or
-// one unit above when x - above * s > 0 (fused multiply-adds): the
-// correctly rounded root.  This is hipcc's own sqrtf sequence without its
-// two range steps, which are identities on these inputs: the scaling of
-// inputs below 2^-96 by 2^32 (and of their root by 2^-16), and the
-// pass-through of zero and infinity.
+// sqrtf(x) for x in [2^-96, 1 + 2^-23]: the hardware reciprocal root y,
+// s = x * y, and one Newton step on the residual, fma(x - s * s, y / 2, s).
+// That this equals the correctly rounded root for every x of the range is
+// a property of the hardware reciprocal root, not a theorem;
+// check_pair_arithmetic_kernel tests it for all 805,306,370 values of x on
+// the device in use (wrach_check_pair_arithmetic).  hipcc's sqrtf takes
+// v_sqrt_f32 (one unit in the last place either way), corrects it with
+// two residual tests, and scales inputs below 2^-96.
 __device__ __forceinline__ float sqrt_rn_above_2m96(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float below = __int_as_float(__float_as_int(s) - 1);
-    const float above = __int_as_float(__float_as_int(s) + 1);
-    float r = __builtin_fmaf(-below, s, x) <= 0.0f ? below : s;
-    return __builtin_fmaf(-above, s, x) > 0.0f ? above : r;
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float s = x * y;
+    const float e = __builtin_fmaf(-s, s, x);
+    return __builtin_fmaf(e, 0.5f * y, s);
 }
 
 // (1 - d) * 2^29 / d correctly rounded, for d in [2^-48, 1] (so 1 - d is
