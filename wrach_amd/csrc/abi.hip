@@ -229,7 +229,9 @@ int enqueue_stencil(wrach_ctx *c) {
             a.in_slot = k + 1;
             a.check = 0;
             a.stat = k + 1 == passes ? 2 : 0;  // the last measure: the sums after the passes
+            a.half = 1;  // the frame's first kernel has run: each pair from one side
             WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
+            a.half = 0;
         }
     }
     if (passes % 2) WRACH_HIP(wrach::launch_copy_pos(c->buf.pos, c->buf.pos_out, c->fp.n, c->stream));

@@ -232,7 +232,7 @@ hipError_t launch_upload_scatter(const UploadParams &u, const FrameBuffers &b, h
 // largest overlap (float bits) after 0..max_passes passes for the last
 // frame; the passes that ran are those whose start exceeded the tolerance.
 constexpr uint32_t kMaxStencilPasses = 64;
-constexpr uint32_t kStencilWords = kMaxStencilPasses + 3;
+constexpr uint32_t kStencilWords = kMaxStencilPasses + 4;
 constexpr uint32_t kStencilBudget = 4096;  // most particles one lane visits in its 3x3 cells
 // Overlap sums (1 - d in 2^-24 units) and pair counts of the frame's first
 // and last measure (sets 0 and 1), and of the learned warm start's output
@@ -250,6 +250,9 @@ constexpr uint32_t kStencilSumEntries = kStencilSumSets * kStencilSumSlots * kSt
 constexpr uint32_t kFallbackWord = kMaxStencilPasses + 1;
 // words[kCrowdCountWord]: the crowded cells listed for the frame.
 constexpr uint32_t kCrowdCountWord = kMaxStencilPasses + 2;
+// words[kOverBudgetWord]: nonzero when the frame's first stencil kernel met
+// a particle whose 3x3 cells hold more than kStencilBudget particles.
+constexpr uint32_t kOverBudgetWord = kMaxStencilPasses + 3;
 constexpr float kOverlapScale = 16777216.0f;  // 2^24
 // Crowded cells (kCrowdMin..kCrowdMax particles): the separation kernels'
 // lanes take their particles in order of x (the lane order; the results
@@ -280,6 +283,12 @@ struct StencilArgs {
     uint32_t *perm;
     uint32_t *crowd;
     uint32_t crowd_cap;
+    // A measure after the frame's first stencil kernel: each pair from one
+    // side only (the particle's own cell past it, the cell to its right and
+    // the three cells above), the pair count and overlap sum doubled, unless
+    // words[kOverBudgetWord] is set (a particle over the budget takes no
+    // pair from its own side, so the two sides differ).
+    uint32_t half;
 };
 hipError_t launch_stencil_measure(const FrameParams &p, const FrameBuffers &b, const StencilArgs &a,
                                   hipStream_t stream);

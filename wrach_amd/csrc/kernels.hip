@@ -2024,20 +2024,28 @@ __device__ __forceinline__ void stencil_visit(bool other, float xi, float yi, fl
 // MOVE the sum of its pair terms in 2^-30 units, and with SUMS the sum of
 // the overlaps in 2^-24 units.  Over kStencilBudget particles in those
 // cells: none of these (a bound on the work of one lane).
-template <bool MOVE, bool SUMS>
+template <bool MOVE, bool SUMS, bool HALF>
 __device__ __forceinline__ float stencil_particle(const FrameParams &p, const FrameBuffers &b,
                                                   const float2 *pin, uint32_t i, uint32_t k,
                                                   float xi, float yi, double &ax, double &ay,
-                                                  uint32_t &pairs, double &osum) {
+                                                  uint32_t &pairs, double &osum, bool &over) {
     uint32_t rs[3], re[3];
     const Grid g = grid_of(p, k);
     uint32_t cx, cy;
     cell_coords(g, k, cx, cy);
     stencil_rows(p, b, g, (int)cx, (int)cy, rs, re);
     const uint64_t total = (uint64_t)(re[0] - rs[0]) + (re[1] - rs[1]) + (re[2] - rs[2]);
-    if (total > kStencilBudget) return 0.0f;
+    if (total > kStencilBudget) {
+        over = true;
+        return 0.0f;
+    }
     uint32_t worst = 0;  // the largest overlap's bits (overlaps lie in [0, 1])
-    for (int t = 0; t < 3; ++t) {
+    // HALF: each pair once.  A particle takes its pairs with the later
+    // particles of its own cell, with the cell to its right and with the
+    // three cells above; every other pair of its 3x3 cells is one of these
+    // seen from the other particle.
+    if (HALF) rs[1] = min(max(rs[1], cell_start(p, b, k)), re[1]);
+    for (int t = HALF ? 1 : 0; t < 3; ++t) {
         uint32_t j = rs[t];
         const uint32_t e = re[t];
         // Four neighbours a step, all loaded before the first is used.
@@ -2049,9 +2057,11 @@ __device__ __forceinline__ float stencil_particle(const FrameParams &p, const Fr
             for (int u = 0; u < 4; ++u) q[u] = qp[u];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                stencil_visit<MOVE, SUMS>(j + u != i, xi, yi, q[u], worst, pairs, osum, ax, ay);
+                stencil_visit<MOVE, SUMS>(HALF && t == 1 ? j + u > i : j + u != i, xi, yi, q[u], worst, pairs,
+                                          osum, ax, ay);
         }
-        for (; j < e; ++j) stencil_visit<MOVE, SUMS>(j != i, xi, yi, pin[j], worst, pairs, osum, ax, ay);
+        for (; j < e; ++j)
+            stencil_visit<MOVE, SUMS>(HALF && t == 1 ? j > i : j != i, xi, yi, pin[j], worst, pairs, osum, ax, ay);
     }
     return __uint_as_float(worst);
 }
@@ -2147,16 +2157,25 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
     float worst = 0.0f;
     uint32_t pairs = 0;
     double osum = 0.0;  // overlaps: multiples of 2^-24, at most kStencilBudget of them
+    bool over = false;
+    const bool half = a.half && a.words[kOverBudgetWord] == 0;  // the whole grid alike
     if (i < p.n) {
         const float2 q = a.pin[i];
         const uint32_t k = stencil_cell(p, b, a, i, q);
         double ax = 0.0, ay = 0.0;
-        if (k < p.ncell)
-            worst = a.stat ? stencil_particle<false, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum)
-                           : stencil_particle<false, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
+        if (k < p.ncell) {
+            if (half)
+                worst = a.stat ? stencil_particle<false, true, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over)
+                               : stencil_particle<false, false, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over);
+            else
+                worst = a.stat ? stencil_particle<false, true, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over)
+                               : stencil_particle<false, false, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over);
+        }
     }
+    if (a.first && over) atomicOr(&a.words[kOverBudgetWord], 1u);
+    const unsigned long long sides = half ? 2ull : 1ull;  // each pair seen once or from both sides
     block_report(&a.words[a.in_slot], worst, stencil_sums(a),
-                 (unsigned long long)(osum * (double)kOverlapScale), pairs);
+                 sides * (unsigned long long)(osum * (double)kOverlapScale), sides * pairs);
 }
 
 // One pass: a.pin -> a.pout.  With a.check it runs only while the
@@ -2176,6 +2195,7 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
     double ax = 0.0, ay = 0.0;  // integers: the sums of the terms in 2^-30 units
     uint32_t pairs = 0;
     double osum = 0.0;  // overlaps: multiples of 2^-24, at most kStencilBudget of them
+    bool over = false;
     float2 q = make_float2(0.0f, 0.0f);
     if (a.pin_alt) {
         const bool rej = warm_rejected(a);
@@ -2186,9 +2206,10 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
         q = a.pin[i];
         const uint32_t k = stencil_cell(p, b, a, i, q);
         if (k < p.ncell)
-            worst = a.stat ? stencil_particle<true, true>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum)
-                           : stencil_particle<true, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum);
+            worst = a.stat ? stencil_particle<true, true, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over)
+                           : stencil_particle<true, false, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over);
     }
+    if (a.first && over) atomicOr(&a.words[kOverBudgetWord], 1u);
     block_report(&a.words[a.in_slot], worst, stencil_sums(a),
                  (unsigned long long)(osum * (double)kOverlapScale), pairs);
     if (i >= p.n) return;
