@@ -314,6 +314,7 @@ int enqueue_learned(wrach_ctx *c) {
         a.in_slot = k;
         a.stat = 2;  // set 1: after the passes
         a.pin_alt = nullptr;
+        a.half = plan.empty() ? 0 : 1;  // a pass has checked the budget
         WRACH_HIP(wrach::launch_stencil_measure(c->fp, c->buf, a, c->stream));
     }
     return WRACH_OK;

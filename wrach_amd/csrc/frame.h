@@ -250,8 +250,10 @@ constexpr uint32_t kStencilSumEntries = kStencilSumSets * kStencilSumSlots * kSt
 constexpr uint32_t kFallbackWord = kMaxStencilPasses + 1;
 // words[kCrowdCountWord]: the crowded cells listed for the frame.
 constexpr uint32_t kCrowdCountWord = kMaxStencilPasses + 2;
-// words[kOverBudgetWord]: nonzero when the frame's first stencil kernel met
-// a particle whose 3x3 cells hold more than kStencilBudget particles.
+// words[kOverBudgetWord]: nonzero once a stencil kernel of the frame has met
+// a particle whose 3x3 cells hold more than kStencilBudget particles (the
+// cells are the frame start's, so the frame's first pass or measure finds
+// every such particle).
 constexpr uint32_t kOverBudgetWord = kMaxStencilPasses + 3;
 constexpr float kOverlapScale = 16777216.0f;  // 2^24
 // Crowded cells (kCrowdMin..kCrowdMax particles): the separation kernels'
@@ -283,7 +285,7 @@ struct StencilArgs {
     uint32_t *perm;
     uint32_t *crowd;
     uint32_t crowd_cap;
-    // A measure after the frame's first stencil kernel: each pair from one
+    // A measure after a pass or measure of the frame: each pair from one
     // side only (the particle's own cell past it, the cell to its right and
     // the three cells above), the pair count and overlap sum doubled, unless
     // words[kOverBudgetWord] is set (a particle over the budget takes no

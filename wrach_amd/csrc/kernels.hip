@@ -2172,7 +2172,7 @@ __global__ __launch_bounds__(256) void stencil_measure_kernel(FrameParams p, Fra
                                : stencil_particle<false, false, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over);
         }
     }
-    if (a.first && over) atomicOr(&a.words[kOverBudgetWord], 1u);
+    if (over) atomicOr(&a.words[kOverBudgetWord], 1u);
     const unsigned long long sides = half ? 2ull : 1ull;  // each pair seen once or from both sides
     block_report(&a.words[a.in_slot], worst, stencil_sums(a),
                  sides * (unsigned long long)(osum * (double)kOverlapScale), sides * pairs);
@@ -2209,7 +2209,7 @@ __global__ __launch_bounds__(256) void stencil_move_kernel(FrameParams p, FrameB
             worst = a.stat ? stencil_particle<true, true, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over)
                            : stencil_particle<true, false, false>(p, b, a.pin, i, k, q.x, q.y, ax, ay, pairs, osum, over);
     }
-    if (a.first && over) atomicOr(&a.words[kOverBudgetWord], 1u);
+    if (over) atomicOr(&a.words[kOverBudgetWord], 1u);
     block_report(&a.words[a.in_slot], worst, stencil_sums(a),
                  (unsigned long long)(osum * (double)kOverlapScale), pairs);
     if (i >= p.n) return;
