@@ -346,7 +346,7 @@ int wrach_test_scan_wait(wrach_ctx *ctx, uint32_t wait_limit, uint32_t epoch_ske
  * ranges against the correctly rounded sequences: out[0] and out[1] are
  * the mismatches of the root and the force (0 and 0 where modes S and L
  * are exact), out[2] and out[3] the inputs tried (805,306,370 and
- * 402,653,185).  About a millisecond on an MI355X. */
+ * 402,653,185).  One kernel over 1.2 * 10^9 inputs; the call waits for it. */
 int wrach_check_pair_arithmetic(int device, uint64_t out[4]);
 
 /* The learned warm start (an extension with no counterpart in the
